@@ -1,0 +1,217 @@
+"""Benchmark of the delay-and-sum hot path on one MI355X.
+
+Workload (default): the differentiation configuration -- 2-D das forward + adjoint, 128-element
+linear array, 11 plane waves (-15..+15 deg), 2048 samples, 16 frames, 256 x 256 voxel grid,
+channel data held in FP16 (FP32 arithmetic).  One step = das forward, the L2-loss gradient
+2 (out - target), and the adjoint (grad wrt channel data, transmit delays and the delay
+scale).  Inputs are resident in HBM before timing starts.
+
+Metric: Gvoxel*ch*tx/s, one unit U = one (voxel, channel, transmit, frame) interpolation,
+counted once per forward+adjoint step.  Prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--storage fp16|bf16|fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gvoxel·ch·tx/s DAS fwd+adjoint, fp16-in vs fp32 at rel-L2≤1e-3 vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--storage", default="fp16", choices=["fp16", "bf16", "fp32"])
+    ap.add_argument("--cpu-sample-rows", type=int, default=16,
+                    help="voxel rows (of 256) in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    return ap.parse_args()
+
+
+class Problem:
+    def __init__(self, w, dev, storage):
+        import torch
+        self.w = w
+        self.dev = dev
+        x32 = torch.from_numpy(w.x).to(dev)
+        self.x = {"fp16": x32.half(), "bf16": x32.bfloat16(), "fp32": x32}[storage]
+        self.x32 = x32
+        self.srcpos = torch.from_numpy(w.srcpos).to(dev)
+        self.dstpos = torch.from_numpy(w.dstpos).to(dev)
+        self.offsets = torch.from_numpy(w.offsets).to(dev)
+        self.weights = torch.from_numpy(w.weights).to(dev)
+        g = torch.Generator(device="cpu").manual_seed(1234)
+        B = w.x.shape[0]
+        self.target = torch.randn((B,) + tuple(w.dstpos.shape[:-1]), generator=g).to(dev)
+        self.scale = 1.0
+
+
+def step(p, x, scale=1.0):
+    import ffdas_amd
+    out = ffdas_amd.das(x, p.srcpos, p.dstpos, p.offsets, p.weights, scale=scale)
+    g = 2.0 * (out - p.target)
+    gx, go, _, gs = ffdas_amd.das_backward(x, p.srcpos, p.dstpos, p.offsets, p.weights, g,
+                                           scale=scale, need_weights=False)
+    return out, gx, go, gs
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def kernel_times(p, x, steps):
+    """Average duration of each library kernel, timed alone with HIP events on the stream the
+    library launches on (torch's current stream is bound to the library handle)."""
+    import torch
+    import ffdas_amd
+    out = ffdas_amd.das(x, p.srcpos, p.dstpos, p.offsets, p.weights)
+    g = 2.0 * (out - p.target)
+    runs = {
+        "das_fwd_kernel": lambda: ffdas_amd.das(x, p.srcpos, p.dstpos, p.offsets, p.weights,
+                                                out=out),
+        "das_bwd_param_kernel": lambda: ffdas_amd.das_backward(
+            x, p.srcpos, p.dstpos, p.offsets, p.weights, g, need_x=False, need_weights=False),
+        "das_bwd_x_kernel": lambda: ffdas_amd.das_backward(
+            x, p.srcpos, p.dstpos, p.offsets, p.weights, g, need_offsets=False,
+            need_weights=False, need_scale=False),
+    }
+    res = {}
+    for name, fn in runs.items():
+        ms, _ = time_steps(fn, steps, 2)
+        res[name] = ms
+    return res
+
+
+def accuracy(p, storage):
+    """rel-L2 of the forward output against the FP64 oracle on frame 0."""
+    import oracle
+    import ffdas_amd
+    w = p.w
+    out = ffdas_amd.das(p.x[:1], p.srcpos, p.dstpos, p.offsets, p.weights).cpu().numpy()
+    ref = oracle.das(w.x[:1], w.srcpos, w.dstpos, w.offsets, w.weights, bits=64)
+    return oracle.rel_l2(out, ref)
+
+
+def cpu_baseline(w, rows):
+    """The FP32 build of the oracle on the host cores: forward + adjoint (grad x, offsets,
+    scale) over a bounded sample (the first `rows` voxel rows of the grid, all frames,
+    channels and transmits)."""
+    import oracle
+    nthreads = oracle.hardware_threads()
+    dst = np.ascontiguousarray(w.dstpos[:rows])
+    ofs = np.ascontiguousarray(w.offsets[:, :rows])
+    wt = np.ascontiguousarray(w.weights[:, :rows])
+    B, C, T, S = w.x.shape
+    U = B * C * T * dst.shape[0] * dst.shape[1]
+    g = np.random.default_rng(0).standard_normal((B,) + dst.shape[:-1]).astype(np.float32)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.das(w.x, w.srcpos, dst, ofs, wt, bits=32, threads=nthreads)
+        oracle.das_adjoint(w.x, w.srcpos, dst, ofs, wt, g, bits=32, threads=nthreads)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": U / t / 1e9, "unit": "Gvoxel·ch·tx/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"FP32 oracle fwd+adjoint on {dst.shape[0]}x{dst.shape[1]} of the "
+                      f"256x256 voxels, all 16 frames/128 ch/11 tx ({U:.3g} U), median of 3, "
+                      f"{nthreads} threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
+                                   "described in DESIGN.md but not built", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    import ffdas_amd
+    from ffdas_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    w = W.config4()
+    log(f"workload generated in {time.time() - t0:.1f}s: x {w.x.shape}, grid "
+        f"{w.dstpos.shape[:-1]}, U/step {w.units:.4g}")
+    p = Problem(w, dev, a.storage)
+    U = w.units
+
+    ms, wall = time_steps(lambda: step(p, p.x), a.steps, a.warmup)
+    value = U / (ms * 1e-3) / 1e9
+    log(f"{a.storage} fwd+adj: {ms:.3f} ms/step, {value:.1f} G U/s")
+
+    # plain FP32 path for the headline ratio
+    ms32, _ = time_steps(lambda: step(p, p.x32), max(3, a.steps // 2), a.warmup)
+    v32 = U / (ms32 * 1e-3) / 1e9
+    log(f"fp32 fwd+adj: {ms32:.3f} ms/step, {v32:.1f} G U/s, ratio {value / v32:.2f}x")
+
+    kt = kernel_times(p, p.x, max(3, a.steps // 2))
+    log("kernel ms: " + ", ".join(f"{k}={v:.3f}" for k, v in kt.items()))
+    esz = p.x.element_size()
+    bytes_per_u = {"das_fwd_kernel": 2 * esz, "das_bwd_param_kernel": 2 * esz,
+                   "das_bwd_x_kernel": 2 * 4}
+    dom = max(kt, key=kt.get)
+    achieved = U * bytes_per_u[dom] / (kt[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_unit": bytes_per_u[dom], "units_per_launch": U,
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+
+    acc = None
+    if not a.no_accuracy:
+        acc = accuracy(p, a.storage)
+        log(f"rel-L2 vs FP64 oracle (frame 0): {acc:.3e}")
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(w, a.cpu_sample_rows)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Gvoxel·ch·tx/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "2-D das fwd+adjoint: 128-ch linear array, 11 plane waves, "
+                               "2048 samples, 16 frames, 256x256 voxels",
+                   "storage": a.storage, "units_per_step": U,
+                   "fp32_plain_value": round(v32, 2),
+                   "ratio_vs_fp32": round(value / v32, 3),
+                   "rel_l2_vs_fp64": acc, "wall_s": round(wall, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

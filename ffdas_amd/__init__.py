@@ -1,0 +1,19 @@
+"""ffdas for MI355X: delay-and-sum (``das``, ``das_sparse``) on hand-written gfx950 kernels.
+
+The operators keep the reference's Python signature (bindings/python/ffdas/das.py) and call
+the C-ABI library libffdas.so (include/ffdas.h) built from ffdas_amd/csrc.
+"""
+from ._core import (TensorView, bfloat16, bfloat162, device_count, device_get, device_set,
+                    double2, float2, float32, float64, get_library_handle, half, half2, view)
+from ._lib import Algorithm, ComputeType, DType, FfdasError
+from .autograd import DasFunction, das_autograd
+from .das import DowncastWarning, das, das_backward, das_sparse
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "das", "das_sparse", "das_backward", "das_autograd", "DasFunction", "Algorithm",
+    "ComputeType", "DType", "FfdasError", "DowncastWarning", "TensorView", "view", "half",
+    "half2", "bfloat16", "bfloat162", "float32", "float64", "float2", "double2",
+    "get_library_handle", "device_count", "device_get", "device_set",
+]

@@ -1,0 +1,209 @@
+// C-ABI entry points of the delay-and-sum path: ffdas_das, ffdas_das_sparse (reference
+// src/das/das.cu:9-83, src/das/das_sparse.cu:6-80), the scaled/sparse forward ffdas_das_ex,
+// the adjoint ffdas_das_backward and ffdas_contiguous_copy.
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace ffdas {
+
+size_t storage_bytes(Storage s) {
+    switch (s) {
+        case Storage::F16: case Storage::BF16: return 2;
+        case Storage::F32: return 4;
+        case Storage::F64: return 8;
+    }
+    return 0;
+}
+
+namespace {
+
+// beta is a host scalar of the output type (src/das/das_impl.cuh:89)
+void read_beta(const void *beta, const DasProblem &p, double &re, double &im) {
+    re = im = 0.0;
+    if (!beta) return;
+    if (p.acc64) {
+        const double *b = static_cast<const double *>(beta);
+        re = b[0];
+        im = p.cplx ? b[1] : 0.0;
+    } else {
+        const float *b = static_cast<const float *>(beta);
+        re = b[0];
+        im = p.cplx ? b[1] : 0.0;
+    }
+}
+
+// Produce the view of x the kernels gather from: the caller's tensor when it already has
+// the kernel storage type, otherwise a contiguous converted copy in the handle workspace.
+ffdas_error_t prepare_x(ffdas_context &h, const DasProblem &p, const void *x, XView &xv) {
+    if (!p.x_needs_copy) {
+        xv.ptr = x;
+        xv.sb = p.xs_b;
+        xv.sc = p.xs_c;
+        xv.st = p.xs_t;
+        return FFDAS_SUCCESS;
+    }
+    const size_t cw = p.cplx ? 2 : 1;
+    const size_t bytes = (size_t)p.B * p.C * p.T * p.S * cw * storage_bytes(p.k_storage);
+    void *ws = nullptr;
+    FFDAS_TRY(h.workspace(bytes, &ws));
+    FFDAS_TRY(launch_cast(h, p, x, ws));
+    xv.ptr = ws;
+    xv.st = p.S;
+    xv.sc = (int64_t)p.T * p.S;
+    xv.sb = (int64_t)p.C * p.T * p.S;
+    return FFDAS_SUCCESS;
+}
+
+ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                                 float wavenum, float scale, ffdas_tensor_desc_t x_desc,
+                                 const void *x, const float *dstpos, const float *offsets,
+                                 const float *weights, bool sparse_mode, int sparse_count,
+                                 const int *sparse_indices, const void *beta,
+                                 ffdas_tensor_desc_t out_desc, void *out,
+                                 ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    FFDAS_TRY(handle->check_device());
+    if (!x_desc || !out_desc) return FFDAS_ERROR_NULL_POINTER;
+    const ffdas_tensor_desc xd = *x_desc, od = *out_desc;   // copied at entry
+    DasProblem p;
+    FFDAS_TRY(plan_das(xd, od, srcpos, dstpos, offsets, weights, sparse_mode, sparse_count,
+                       sparse_indices, compute_type, alg, p));
+    if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
+    if (!(scale > 0.0f) || !std::isfinite(scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
+    double br, bi;
+    read_beta(beta, p, br, bi);
+    XView xv;
+    FFDAS_TRY(prepare_x(*handle, p, x, xv));
+    DasGeomArgs a;
+    a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
+    a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
+    return launch_forward(*handle, p, a, xv, out, br, bi);
+}
+
+}  // namespace
+}  // namespace ffdas
+
+using namespace ffdas;
+
+extern "C" {
+
+ffdas_error_t ffdas_das(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                        float wavenum, ffdas_tensor_desc_t x_desc, const void *x,
+                        const float *dstpos, const float *offsets, const float *weights,
+                        const void *beta, ffdas_tensor_desc_t out_desc, void *out,
+                        ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    return das_forward_common(handle, srcpos, srcdir, wavenum, 1.0f, x_desc, x, dstpos, offsets,
+                              weights, false, 0, nullptr, beta, out_desc, out, compute_type, alg);
+}
+
+ffdas_error_t ffdas_das_sparse(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                               float wavenum, ffdas_tensor_desc_t x_desc, const void *x,
+                               const float *dstpos, const float *offsets, const float *weights,
+                               int sparse_count, const int *sparse_indices, const void *beta,
+                               ffdas_tensor_desc_t out_desc, void *out,
+                               ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    return das_forward_common(handle, srcpos, srcdir, wavenum, 1.0f, x_desc, x, dstpos, offsets,
+                              weights, true, sparse_count, sparse_indices, beta, out_desc, out,
+                              compute_type, alg);
+}
+
+ffdas_error_t ffdas_das_ex(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                           float wavenum, float scale, ffdas_tensor_desc_t x_desc, const void *x,
+                           const float *dstpos, const float *offsets, const float *weights,
+                           int sparse_count, const int *sparse_indices, const void *beta,
+                           ffdas_tensor_desc_t out_desc, void *out,
+                           ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    if (!sparse_indices && sparse_count != 0) return FFDAS_ERROR_INVALID_ARGUMENT;
+    return das_forward_common(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos, offsets,
+                              weights, sparse_indices != nullptr, sparse_count, sparse_indices,
+                              beta, out_desc, out, compute_type, alg);
+}
+
+ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                                 float wavenum, float scale, ffdas_tensor_desc_t x_desc,
+                                 const void *x, const float *dstpos, const float *offsets,
+                                 const float *weights, int sparse_count,
+                                 const int *sparse_indices, ffdas_tensor_desc_t grad_out_desc,
+                                 const void *grad_out, ffdas_tensor_desc_t grad_x_desc,
+                                 void *grad_x, float *grad_offsets, float *grad_weights,
+                                 float *grad_scale, ffdas_compute_type_t compute_type,
+                                 ffdas_alg_t alg) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    FFDAS_TRY(handle->check_device());
+    if (!x_desc || !grad_out_desc) return FFDAS_ERROR_NULL_POINTER;
+    if (!sparse_indices && sparse_count != 0) return FFDAS_ERROR_INVALID_ARGUMENT;
+    const ffdas_tensor_desc xd = *x_desc, gd = *grad_out_desc;
+    DasProblem p;
+    FFDAS_TRY(plan_das(xd, gd, srcpos, dstpos, offsets, weights, sparse_indices != nullptr,
+                       sparse_count, sparse_indices, compute_type, alg, p));
+    if (!x || !grad_out) return FFDAS_ERROR_NULL_POINTER;
+    if (!(scale > 0.0f) || !std::isfinite(scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
+    if (grad_x) {
+        if (!grad_x_desc) return FFDAS_ERROR_NULL_POINTER;
+        const ffdas_tensor_desc &xg = *grad_x_desc;
+        const ffdas_datatype_t want = p.acc64 ? (p.cplx ? FFDAS_C_64F : FFDAS_R_64F)
+                                              : (p.cplx ? FFDAS_C_32F : FFDAS_R_32F);
+        if (xg.dtype != want) return FFDAS_ERROR_UNSUPPORTED_TYPE;
+        if (xg.dims != xd.dims) return FFDAS_ERROR_INVALID_DIMS;
+        int64_t s = 1;
+        for (int i = (int)xg.dims.size() - 1; i >= 0; i--) {
+            if (xg.dims[i] > 1 && xg.strides[i] != s) return FFDAS_ERROR_NON_CONTIGUOUS;
+            s *= xg.dims[i];
+        }
+    }
+    XView xv;
+    FFDAS_TRY(prepare_x(*handle, p, x, xv));
+    DasGeomArgs a;
+    a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
+    a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
+    return launch_backward(*handle, p, a, xv, grad_out, p.os_b, grad_x, grad_offsets,
+                           grad_weights, grad_scale);
+}
+
+ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
+                                    const void *x, ffdas_tensor_desc_t out_desc, void *out) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    FFDAS_TRY(handle->check_device());
+    if (!x_desc || !out_desc) return FFDAS_ERROR_NULL_POINTER;
+    if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
+    const ffdas_tensor_desc &xd = *x_desc, &od = *out_desc;
+    if (xd.dims != od.dims) return FFDAS_ERROR_INVALID_DIMS;
+    // express the copy as ([B,]C,T,S) rows: fold to 4-D with a unit sample stride
+    if (xd.dims.empty() || xd.dims.size() > 4) return FFDAS_ERROR_INVALID_DIMS;
+    if (xd.strides.back() != 1) return FFDAS_ERROR_INVALID_ARGUMENT;
+    int64_t s = 1;
+    for (int i = (int)od.dims.size() - 1; i >= 0; i--) {
+        if (od.dims[i] > 1 && od.strides[i] != s) return FFDAS_ERROR_NON_CONTIGUOUS;
+        s *= od.dims[i];
+    }
+    DasProblem p;
+    bool cx = is_complex(xd.dtype), co = is_complex(od.dtype);
+    if (cx != co) return FFDAS_ERROR_UNSUPPORTED_TYPE;
+    auto st_of = [](ffdas_datatype_t t, Storage &o) {
+        switch (t) {
+            case FFDAS_R_16F: case FFDAS_C_16F: o = Storage::F16; return true;
+            case FFDAS_R_16BF: case FFDAS_C_16BF: o = Storage::BF16; return true;
+            case FFDAS_R_32F: case FFDAS_C_32F: o = Storage::F32; return true;
+            case FFDAS_R_64F: case FFDAS_C_64F: o = Storage::F64; return true;
+            default: return false;
+        }
+    };
+    if (!st_of(xd.dtype, p.x_storage) || !st_of(od.dtype, p.k_storage))
+        return FFDAS_ERROR_UNSUPPORTED_TYPE;
+    p.cplx = cx;
+    std::vector<int64_t> d(4, 1), st(4, 0);
+    const int nd = (int)xd.dims.size();
+    for (int i = 0; i < nd; i++) {
+        d[4 - nd + i] = xd.dims[i];
+        st[4 - nd + i] = xd.strides[i];
+    }
+    for (int i = 0; i < 4; i++)
+        if (d[i] > INT32_MAX || st[i] < 0) return FFDAS_ERROR_INVALID_DIMS;
+    p.B = (int)d[0]; p.C = (int)d[1]; p.T = (int)d[2]; p.S = (int)d[3];
+    p.xs_b = st[0]; p.xs_c = st[1]; p.xs_t = st[2];
+    return launch_cast(*handle, p, x, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// Internal types of libffdas.so (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ffdas.h"
+
+// Tensor descriptor: dims/strides in elements (complex = one element).
+struct ffdas_tensor_desc {
+    std::vector<int64_t> dims;
+    std::vector<int64_t> strides;
+    ffdas_datatype_t dtype = FFDAS_R_32F;
+
+    int64_t numel() const {
+        int64_t n = 1;
+        for (int64_t d : dims) n *= d;
+        return n;
+    }
+};
+
+// Library handle: device, stream, workspace.
+struct ffdas_context {
+    int device_id = 0;
+    int cu_count = 0;
+    int lds_bytes = 0;
+    std::string arch;          // e.g. "gfx950"
+    hipStream_t stream = nullptr;
+    // stream-ordered workspace (grow-only, released with hipFreeAsync on the owning stream)
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    hipStream_t ws_stream = nullptr;
+
+    ffdas_error_t check_device() const;
+    ffdas_error_t workspace(size_t bytes, void **ptr);
+};
+
+namespace ffdas {
+
+inline bool is_complex(ffdas_datatype_t t) {
+    return t == FFDAS_C_16F || t == FFDAS_C_32F || t == FFDAS_C_64F || t == FFDAS_C_16BF ||
+           t == FFDAS_C_32I || t == FFDAS_C_16I;
+}
+
+// Storage classes understood by the kernels.
+enum class Storage : int { F16 = 0, BF16 = 1, F32 = 2, F64 = 3 };
+
+// Decoded problem (see das_plan.cpp).
+struct DasProblem {
+    int B = 1, C = 0, T = 0, S = 0, K = 0;
+    int64_t N = 0;
+    bool have_batch = false;
+    bool cplx = false;
+    bool sparse = false;
+    Storage x_storage = Storage::F32;     // storage of the input as given
+    Storage k_storage = Storage::F32;     // storage the kernel gathers from
+    bool acc64 = false;                   // double accumulation (double data)
+    bool x_needs_copy = false;            // cast / relayout into a contiguous workspace
+    int64_t xs_b = 0, xs_c = 0, xs_t = 0; // strides of x as given (elements)
+    int64_t os_b = 0;                     // batch stride of out (elements)
+    // spatial shape of the output (without batch), used for voxel tiling
+    std::vector<int64_t> spatial;
+};
+
+ffdas_error_t plan_das(const ffdas_tensor_desc &x, const ffdas_tensor_desc &out,
+                       const float *srcpos, const float *dstpos, const float *offsets,
+                       const float *weights, bool sparse_mode, int sparse_count,
+                       const int *sparse_indices, ffdas_compute_type_t ct, ffdas_alg_t alg,
+                       DasProblem &p);
+
+ffdas_error_t hip_status(hipError_t e, ffdas_error_t kind = FFDAS_ERROR_CUDA_RUNTIME);
+
+}  // namespace ffdas
+
+#define FFDAS_TRY(expr)                               \
+    do {                                              \
+        ffdas_error_t _e = (expr);                    \
+        if (_e != FFDAS_SUCCESS) return _e;           \
+    } while (0)
+
+namespace ffdas {
+
+// Geometry pointers and scalars of one call (device pointers).
+struct DasGeomArgs {
+    const float *srcpos = nullptr;
+    const float *srcdir = nullptr;
+    const float *dstpos = nullptr;
+    const float *offsets = nullptr;
+    const float *weights = nullptr;
+    const int *sparse_indices = nullptr;
+    float wavenum = 0.0f;
+    float scale = 1.0f;
+};
+
+// Channel data as the kernels see it: storage p.k_storage, strides in elements.
+struct XView {
+    const void *ptr = nullptr;
+    int64_t sb = 0, sc = 0, st = 0;
+};
+
+ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
+                             const XView &xv, void *out, double beta_re, double beta_im);
+ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
+                              const XView &xv, const void *gout, int64_t gs_b, void *gx,
+                              float *gofs, float *gwt, float *gscale);
+ffdas_error_t launch_cast(ffdas_context &h, const DasProblem &p, const void *x, void *dst);
+ffdas_error_t launch_fill_zero(ffdas_context &h, float *p, int64_t n);
+
+size_t storage_bytes(Storage s);
+
+}  // namespace ffdas

@@ -1,0 +1,85 @@
+"""Adjoint parity (GPU): ffdas_das_backward against the FP64 oracle's adjoint, the
+dot-product test and autograd plumbing."""
+import numpy as np
+import pytest
+
+import oracle
+from ffdas_amd import workloads as W
+from tests.helpers import rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _grads(w, dev, gout, scale=1.0, **kw):
+    import torch
+    import ffdas_amd
+    d = to_dev(w, dev)
+    g = torch.from_numpy(gout).to(dev)
+    return ffdas_amd.das_backward(d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"], g,
+                                  srcdir=d.get("srcdir"), wavenum=d.get("wavenum", 0.0),
+                                  sparse_indices=d.get("sparse_indices"), scale=scale, **kw)
+
+
+@pytest.mark.parametrize("case", ["dense", "dir", "sparse", "complex", "scaled"])
+def test_small_adjoint_fp32(dev, case):
+    kw = {}
+    scale = 1.0
+    if case == "dense":
+        w = W.small_random(seed=11)
+    elif case == "dir":
+        w = W.small_random(seed=12, srcdir=True)
+    elif case == "sparse":
+        w = W.small_random(seed=13, T=5, K=3, sparse=True)
+    elif case == "complex":
+        w = W.small_random(seed=14, complex_iq=True)
+    else:
+        w = W.small_random(seed=15)
+        scale = 1.03
+    rng = np.random.default_rng(7)
+    B = w.x.shape[0]
+    shape = (B,) + w.dstpos.shape[:-1]
+    gout = rng.standard_normal(shape).astype(np.float32)
+    if np.iscomplexobj(w.x):
+        gout = (gout + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    ref = oracle.das_adjoint(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             srcdir=w.srcdir, wavenum=w.wavenum, sparse_indices=w.sparse_indices,
+                             scale=scale)
+    gx, go, gw, gs = _grads(w, dev, gout, scale=scale)
+    assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
+    assert rel_l2(go.cpu().numpy(), ref["grad_offsets"]) < 1e-4
+    assert rel_l2(gw.cpu().numpy(), ref["grad_weights"]) < 1e-5
+    assert abs(float(gs) - ref["grad_scale"]) <= 1e-4 * max(1.0, abs(ref["grad_scale"]))
+
+
+def test_dot_product_config1_fp32(dev):
+    """<A x, y> == <x, A^T y> with the library's own forward and adjoint."""
+    import torch
+    import ffdas_amd
+    w = W.config1()
+    d = to_dev(w, dev)
+    y = torch.randn((1,) + tuple(w.dstpos.shape[:-1]), device=dev, generator=None)
+    ax = ffdas_amd.das(d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"])
+    gx, _, _, _ = ffdas_amd.das_backward(d["x"], d["srcpos"], d["dstpos"], d["offsets"],
+                                         d["weights"], y, need_offsets=False,
+                                         need_weights=False, need_scale=False)
+    lhs = float((ax.double() * y.double()).sum())
+    rhs = float((d["x"].double() * gx.double()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + abs(rhs))
+
+
+def test_autograd_matches_backward(dev):
+    import torch
+    import ffdas_amd
+    w = W.small_random(seed=21, B=3)
+    d = to_dev(w, dev)
+    x = d["x"].clone().requires_grad_(True)
+    ofs = d["offsets"].clone().requires_grad_(True)
+    s = torch.tensor(1.01, device=dev, requires_grad=True)
+    out = ffdas_amd.das(x, d["srcpos"], d["dstpos"], ofs, d["weights"], scale=s)
+    g = torch.randn_like(out)
+    (out * g).sum().backward()
+    gx, go, gw, gs = ffdas_amd.das_backward(d["x"], d["srcpos"], d["dstpos"], d["offsets"],
+                                            d["weights"], g, scale=1.01)
+    assert torch.allclose(x.grad, gx, rtol=1e-6, atol=1e-6)
+    assert torch.allclose(ofs.grad, go, rtol=1e-6, atol=1e-6)
+    assert abs(float(s.grad) - float(gs)) <= 1e-4 * max(1.0, abs(float(gs)))
