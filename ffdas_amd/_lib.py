@@ -108,6 +108,9 @@ def lib():
         "ffdas_event_record": (i32, [vp, u]),
         "ffdas_event_synchronize": (i32, [u]),
         "ffdas_event_elapsed_time": (i32, [u, u, P(f32)]),
+        "ffdas_profile_enable": (i32, [i32]),
+        "ffdas_profile_reset": (i32, []),
+        "ffdas_profile_query": (i32, [ctypes.c_char_p, P(ctypes.c_double), P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -34,9 +34,25 @@ void read_beta(const void *beta, const DasProblem &p, double &re, double &im) {
     }
 }
 
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+size_t cast_bytes(const DasProblem &p) {
+    if (p.fi_frames > 0) return align_up(fi_bytes(p));
+    if (!p.x_needs_copy) return 0;
+    const size_t cw = p.cplx ? 2 : 1;
+    return align_up((size_t)p.B * p.C * p.T * p.S * cw * storage_bytes(p.k_storage));
+}
+
 // Produce the view of x the kernels gather from: the caller's tensor when it already has
-// the kernel storage type, otherwise a contiguous converted copy in the handle workspace.
-ffdas_error_t prepare_x(ffdas_context &h, const DasProblem &p, const void *x, XView &xv) {
+// the kernel storage type, otherwise a contiguous converted copy in workspace `ws`
+// (cast_bytes(p) bytes).
+ffdas_error_t prepare_x(ffdas_context &h, const DasProblem &p, const void *x, void *ws,
+                        XView &xv) {
+    if (p.fi_frames > 0) {
+        FFDAS_TRY(launch_relayout_fi(h, p, x, ws));
+        xv.ptr = ws;
+        return FFDAS_SUCCESS;
+    }
     if (!p.x_needs_copy) {
         xv.ptr = x;
         xv.sb = p.xs_b;
@@ -44,10 +60,6 @@ ffdas_error_t prepare_x(ffdas_context &h, const DasProblem &p, const void *x, XV
         xv.st = p.xs_t;
         return FFDAS_SUCCESS;
     }
-    const size_t cw = p.cplx ? 2 : 1;
-    const size_t bytes = (size_t)p.B * p.C * p.T * p.S * cw * storage_bytes(p.k_storage);
-    void *ws = nullptr;
-    FFDAS_TRY(h.workspace(bytes, &ws));
     FFDAS_TRY(launch_cast(h, p, x, ws));
     xv.ptr = ws;
     xv.st = p.S;
@@ -70,12 +82,15 @@ ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, con
     DasProblem p;
     FFDAS_TRY(plan_das(xd, od, srcpos, dstpos, offsets, weights, sparse_mode, sparse_count,
                        sparse_indices, compute_type, alg, p));
+    p.fi_frames = fi_frames_for(p, alg);
     if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
     if (!(scale > 0.0f) || !std::isfinite(scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
     double br, bi;
     read_beta(beta, p, br, bi);
+    void *ws = nullptr;
+    FFDAS_TRY(handle->workspace(cast_bytes(p), &ws));
     XView xv;
-    FFDAS_TRY(prepare_x(*handle, p, x, xv));
+    FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
     DasGeomArgs a;
     a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
     a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
@@ -138,6 +153,7 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
     DasProblem p;
     FFDAS_TRY(plan_das(xd, gd, srcpos, dstpos, offsets, weights, sparse_indices != nullptr,
                        sparse_count, sparse_indices, compute_type, alg, p));
+    p.fi_frames = fi_frames_for(p, alg);
     if (!x || !grad_out) return FFDAS_ERROR_NULL_POINTER;
     if (!(scale > 0.0f) || !std::isfinite(scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
     if (grad_x) {
@@ -153,13 +169,19 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
             s *= xg.dims[i];
         }
     }
+    const size_t xb = cast_bytes(p);
+    const size_t gb = grad_x ? align_up(backward_workspace(p)) : 0;
+    if (grad_x && gb == 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
+    void *ws = nullptr;
+    FFDAS_TRY(handle->workspace(xb + gb, &ws));
     XView xv;
-    FFDAS_TRY(prepare_x(*handle, p, x, xv));
+    FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
+    void *gt = grad_x ? static_cast<char *>(ws) + xb : nullptr;
     DasGeomArgs a;
     a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
     a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
     return launch_backward(*handle, p, a, xv, grad_out, p.os_b, grad_x, grad_offsets,
-                           grad_weights, grad_scale);
+                           grad_weights, grad_scale, gt);
 }
 
 ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,

@@ -15,6 +15,9 @@
 // Adjoint w.r.t. x: a workgroup owns one channel-data row (channel m, transmit s) for a
 // chunk of FS frames, scatters every voxel's two interpolation weights into that row in
 // LDS with LDS float atomics, then writes the row once (no global atomics).
+#include <cstdlib>
+#include <cstring>
+
 #include "das_device.h"
 
 namespace ffdas {
@@ -22,41 +25,105 @@ namespace dev {
 
 template <bool ACC64> using AccT = typename std::conditional<ACC64, double, float>::type;
 
+__device__ __forceinline__ float fmaA(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaA(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <class D> __device__ __forceinline__ D convert_to(float v);
+template <> __device__ __forceinline__ float convert_to<float>(float v) { return v; }
+template <> __device__ __forceinline__ _Float16 convert_to<_Float16>(float v) { return (_Float16)v; }
+template <> __device__ __forceinline__ uint16_t convert_to<uint16_t>(float v) {
+    // round to nearest even; NaN stays NaN
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------------------------------
+// channel-data access policies
+// ---------------------------------------------------------------------------------------
+// Natural layout ([B,]C,T,S with S contiguous): FT frames per lane, one pointer per frame,
+// x[k] and x[k+1] of each frame read separately.
+template <Storage ST, int FT, int CW>
+struct NaturalRows {
+    using raw = typename Elem<ST>::raw;
+    const raw *fb[FT];
+    int64_t sc, st;
+    __device__ __forceinline__ NaturalRows(const void *xv, int64_t xs_b, int64_t xs_c,
+                                           int64_t xs_t, int b0, int B) {
+        const raw *x = static_cast<const raw *>(xv);
+#pragma unroll
+        for (int f = 0; f < FT; f++) {
+            const int b = min(b0 + f, B - 1);   // frames past B re-read the last one
+            fb[f] = x + (int64_t)b * xs_b * CW;
+        }
+        sc = xs_c;
+        st = xs_t;
+    }
+    __device__ __forceinline__ uint32_t offset(int m, int s, int k) const {
+        return (uint32_t)((int64_t)m * sc) + (uint32_t)((int64_t)s * st) + (uint32_t)k;
+    }
+    // v[0][f][c] = x[k], v[1][f][c] = x[k+1]
+    template <class A>
+    __device__ __forceinline__ void load(uint32_t off, A (&v)[2][FT][CW]) const {
+#pragma unroll
+        for (int f = 0; f < FT; f++) {
+            const raw *p = fb[f] + (size_t)off * CW;
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                v[0][f][c] = (A)Elem<ST>::real(p[c]);
+                v[1][f][c] = (A)Elem<ST>::real(p[CW + c]);
+            }
+        }
+    }
+};
+
+// Frame-inner layout [chunk][C][T][S][FT] (written by relayout_fi_kernel): the FT frames of
+// rows k and k+1 are 2*FT*CW contiguous elements, fetched with 16-byte vector loads.
+template <Storage ST, int FT, int CW>
+struct FrameInnerRows {
+    using raw = typename Elem<ST>::raw;
+    static constexpr int NB = 2 * FT * CW * (int)sizeof(raw);
+    static_assert((FT * CW * sizeof(raw)) % 16 == 0, "frame-inner rows must be 16-byte multiples");
+    const raw *base;
+    int T, S;
+    __device__ __forceinline__ FrameInnerRows(const void *xv, int chunk, int C, int T_, int S_)
+        : T(T_), S(S_) {
+        base = static_cast<const raw *>(xv) + (int64_t)chunk * C * T_ * S_ * FT * CW;
+    }
+    __device__ __forceinline__ uint32_t offset(int m, int s, int k) const {
+        return (uint32_t)((m * T + s) * S + k);
+    }
+    template <class A>
+    __device__ __forceinline__ void load(uint32_t off, A (&v)[2][FT][CW]) const {
+        const uint4 *q = reinterpret_cast<const uint4 *>(base + (size_t)off * FT * CW);
+        uint4 tmp[NB / 16];
+#pragma unroll
+        for (int i = 0; i < NB / 16; i++) tmp[i] = q[i];
+        raw r[2 * FT * CW];
+        __builtin_memcpy(r, tmp, NB);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int f = 0; f < FT; f++)
+#pragma unroll
+                for (int c = 0; c < CW; c++) v[h][f][c] = (A)Elem<ST>::real(r[(h * FT + f) * CW + c]);
+    }
+};
+
 // ---------------------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------------------
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
-__global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
-                                                      int64_t xs_b, int64_t xs_c, int64_t xs_t,
-                                                      void *__restrict__ outv, int64_t os_b,
-                                                      double beta_re, double beta_im) {
+template <class Rows, Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n,
+                                             AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1]) {
     using A = AccT<ACC64>;
-    using raw = typename Elem<ST>::raw;
     constexpr int CW = CPLX ? 2 : 1;
-    const raw *__restrict__ x = static_cast<const raw *>(xv);
-
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    bool valid;
-    const int n = voxel_of(tl, tile, lane, valid);
-    if (!valid) return;
-
-    const int b0 = blockIdx.y * FT;
     const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
-
-    A acc[FT][CW];
 #pragma unroll
     for (int f = 0; f < FT; f++)
 #pragma unroll
         for (int c = 0; c < CW; c++) acc[f][c] = A(0);
-
-    // frame base pointers (uniform); frames past B re-read the last frame (never stored)
-    const raw *fb[FT];
-#pragma unroll
-    for (int f = 0; f < FT; f++) {
-        const int b = min(b0 + f, g.B - 1);
-        fb[f] = x + (int64_t)b * xs_b * CW;
-    }
     const bool unit = g.unit_scale != 0;
 
     for (int kc = 0; kc < g.K; kc += KC) {
@@ -84,7 +151,6 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
                 const float *sd = g.srcdir + 4 * m;
                 dok = (R.dx * sd[0] + R.dy * sd[1] + R.dz * sd[2]) * R.rinv > sd[3];
             }
-            const uint32_t rowc = (uint32_t)((int64_t)m * xs_c);
 #pragma unroll
             for (int j = 0; j < KC; j++) {
                 if (kc + j >= g.K) break;
@@ -92,42 +158,45 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
                 const bool in = dok && (uint32_t)tp.k < (uint32_t)(g.S - 1) &&
                                 (uint32_t)sq[j] < (uint32_t)g.T;
                 if (in) {
-                    const uint32_t off = rowc + (uint32_t)((int64_t)sq[j] * xs_t) + (uint32_t)tp.k;
+                    A v[2][FT][CW];
+                    xr.load(xr.offset(m, sq[j], tp.k), v);
                     if constexpr (!CPLX) {
                         const A c = (A)wt[j] * (A)tp.t;
                         const A a = (A)wt[j] - c;
 #pragma unroll
-                        for (int f = 0; f < FT; f++) {
-                            A x0, x1;
-                            load_pair<ST>(fb[f] + off, x0, x1);
-                            acc[f][0] = __builtin_fma(c, x1, __builtin_fma(a, x0, acc[f][0]));
-                        }
+                        for (int f = 0; f < FT; f++)
+                            acc[f][0] = fmaA(c, v[1][f][0], fmaA(a, v[0][f][0], acc[f][0]));
                     } else {
                         float sn, cs;
                         phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
-                        const float wr = wt[j] * cs, wi = wt[j] * sn;
+                        const A wr = (A)(wt[j] * cs), wi = (A)(wt[j] * sn);
                         const A t = (A)tp.t;
 #pragma unroll
                         for (int f = 0; f < FT; f++) {
-                            A x0r, x0i, x1r, x1i;
-                            load_pair_c<ST>(fb[f] + 2 * (size_t)off, x0r, x0i, x1r, x1i);
-                            const A lr = x0r + t * (x1r - x0r);
-                            const A li = x0i + t * (x1i - x0i);
-                            acc[f][0] += (A)wr * lr - (A)wi * li;
-                            acc[f][1] += (A)wr * li + (A)wi * lr;
+                            const A lr = v[0][f][0] + t * (v[1][f][0] - v[0][f][0]);
+                            const A li = v[0][f][1] + t * (v[1][f][1] - v[0][f][1]);
+                            acc[f][0] += wr * lr - wi * li;
+                            acc[f][1] += wr * li + wi * lr;
                         }
                     }
                 }
             }
         }
     }
+}
 
+template <bool ACC64, bool CPLX, int FT>
+__device__ __forceinline__ void store_out(const AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1], void *outv,
+                                          int64_t os_b, int n, int b0, int B, double beta_re,
+                                          double beta_im) {
+    using A = AccT<ACC64>;
+    constexpr int CW = CPLX ? 2 : 1;
     A *out = static_cast<A *>(outv);
     const bool zero_beta = beta_re == 0.0 && beta_im == 0.0;
 #pragma unroll
     for (int f = 0; f < FT; f++) {
         const int b = b0 + f;
-        if (b >= g.B) break;
+        if (b >= B) break;
         A *o = out + ((int64_t)b * os_b + n) * CW;
         if constexpr (!CPLX) {
             o[0] = zero_beta ? acc[f][0] : (A)beta_re * o[0] + acc[f][0];
@@ -144,6 +213,43 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
     }
 }
 
+// natural layout (plain kernel; any strides)
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
+                                                      int64_t xs_b, int64_t xs_c, int64_t xs_t,
+                                                      void *__restrict__ outv, int64_t os_b,
+                                                      double beta_re, double beta_im) {
+    constexpr int CW = CPLX ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    bool valid;
+    const int n = voxel_of(tl, tile, lane, valid);
+    if (!valid) return;
+    const int b0 = blockIdx.y * FT;
+    const NaturalRows<ST, FT, CW> xr(xv, xs_b, xs_c, xs_t, b0, g.B);
+    AccT<ACC64> acc[FT][CW];
+    das_fwd_body<NaturalRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, acc);
+    store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, b0, g.B, beta_re, beta_im);
+}
+
+// frame-inner layout (grid.y = frame chunk)
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__global__ __launch_bounds__(256) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
+                                                         void *__restrict__ outv, int64_t os_b,
+                                                         double beta_re, double beta_im) {
+    constexpr int CW = CPLX ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    bool valid;
+    const int n = voxel_of(tl, tile, lane, valid);
+    if (!valid) return;
+    const int chunk = blockIdx.y;
+    const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
+    AccT<ACC64> acc[FT][CW];
+    das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, acc);
+    store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, chunk * FT, g.B, beta_re, beta_im);
+}
+
 // ---------------------------------------------------------------------------------------
 // adjoint: offsets / weights / scale
 // ---------------------------------------------------------------------------------------
@@ -154,179 +260,300 @@ __device__ __forceinline__ AccT<ACC64> wave_sum(AccT<ACC64> v) {
     return v;
 }
 
+template <class Rows, Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Rows &xr, int n,
+                                                          int b0, const void *goutv, int64_t gs_b,
+                                                          float *gofs, float *gwt, int accumulate) {
+    using A = AccT<ACC64>;
+    constexpr int CW = CPLX ? 2 : 1;
+    const A *__restrict__ gout = static_cast<const A *>(goutv);
+    A scale_part = A(0);
+    const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
+    A gv[FT][CW];
+#pragma unroll
+    for (int f = 0; f < FT; f++) {
+        const int b = b0 + f;
+#pragma unroll
+        for (int c = 0; c < CW; c++)
+            gv[f][c] = (b < g.B) ? gout[((int64_t)b * gs_b + n) * CW + c] : A(0);
+    }
+    const bool unit = g.unit_scale != 0;
+
+    for (int kc = 0; kc < g.K; kc += KC) {
+        float ofs[KC], wt[KC];
+        int sq[KC];
+        A d_tau[KC], d_w[KC];
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            const int jj = kc + j;
+            d_tau[j] = A(0);
+            d_w[j] = A(0);
+            if (jj < g.K) {
+                const int64_t o = (int64_t)jj * g.N + n;
+                ofs[j] = g.offsets[o];
+                wt[j] = g.weights[o];
+                sq[j] = g.sparse ? g.sparse[o] : jj;
+            } else {
+                ofs[j] = 0.0f;
+                wt[j] = 0.0f;
+                sq[j] = -1;
+            }
+        }
+        for (int m = 0; m < g.C; m++) {
+            const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
+            const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
+            bool dok = true;
+            if (g.srcdir) {
+                const float *sd = g.srcdir + 4 * m;
+                dok = (R.dx * sd[0] + R.dy * sd[1] + R.dz * sd[2]) * R.rinv > sd[3];
+            }
+#pragma unroll
+            for (int j = 0; j < KC; j++) {
+                if (kc + j >= g.K) break;
+                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                const bool in = dok && (uint32_t)tp.k < (uint32_t)(g.S - 1) &&
+                                (uint32_t)sq[j] < (uint32_t)g.T;
+                if (in) {
+                    A v[2][FT][CW];
+                    xr.load(xr.offset(m, sq[j], tp.k), v);
+                    const A t = (A)tp.t;
+                    if constexpr (!CPLX) {
+                        A sd_ = A(0), sl_ = A(0);
+#pragma unroll
+                        for (int f = 0; f < FT; f++) {
+                            const A d = v[1][f][0] - v[0][f][0];
+                            sd_ += gv[f][0] * d;
+                            sl_ += gv[f][0] * (v[0][f][0] + t * d);
+                        }
+                        const A dt = (A)wt[j] * sd_;
+                        d_tau[j] += dt;
+                        d_w[j] += sl_;
+                        scale_part += dt * ((A)ofs[j] + (A)R.rh + (A)R.rl);
+                    } else {
+                        // Q = sum conj(g) (x1 - x0), L = sum conj(g) lerp
+                        A qr = 0, qi = 0, lr = 0, li = 0;
+#pragma unroll
+                        for (int f = 0; f < FT; f++) {
+                            const A dr = v[1][f][0] - v[0][f][0], di = v[1][f][1] - v[0][f][1];
+                            const A pr = v[0][f][0] + t * dr, pi = v[0][f][1] + t * di;
+                            const A gr = gv[f][0], gi = gv[f][1];
+                            qr += gr * dr + gi * di;
+                            qi += gr * di - gi * dr;
+                            lr += gr * pr + gi * pi;
+                            li += gr * pi - gi * pr;
+                        }
+                        float sn, cs;
+                        phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
+                        const A er = (A)cs, ei = (A)sn, kap = (A)g.wavenum;
+                        // Re(e (Q + i kap L)), Re(e L)
+                        const A zr = qr - kap * li, zi = qi + kap * lr;
+                        const A dt = (A)wt[j] * (er * zr - ei * zi);
+                        d_tau[j] += dt;
+                        d_w[j] += er * lr - ei * li;
+                        scale_part += dt * ((A)ofs[j] + (A)R.rh + (A)R.rl);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            const int jj = kc + j;
+            if (jj >= g.K) break;
+            const int64_t o = (int64_t)jj * g.N + n;
+            const float vo = (float)(d_tau[j] * (A)g.scale);
+            const float vw = (float)d_w[j];
+            if (accumulate) {
+                if (gofs) atomicAdd(gofs + o, vo);
+                if (gwt) atomicAdd(gwt + o, vw);
+            } else {
+                if (gofs) gofs[o] = vo;
+                if (gwt) gwt[o] = vw;
+            }
+        }
+    }
+    return scale_part;
+}
+
+template <bool ACC64>
+__device__ __forceinline__ void block_scale_sum(AccT<ACC64> part, float *gscale) {
+    using A = AccT<ACC64>;
+    __shared__ A red[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    A v = wave_sum<ACC64>(part);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        A s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+        atomicAdd(gscale, (float)s);
+    }
+}
+
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
 __global__ __launch_bounds__(256) void das_bwd_param_kernel(
     Geo g, Tiling tl, const void *__restrict__ xv, int64_t xs_b, int64_t xs_c, int64_t xs_t,
     const void *__restrict__ goutv, int64_t gs_b, float *__restrict__ gofs,
     float *__restrict__ gwt, float *__restrict__ gscale, int accumulate) {
-    using A = AccT<ACC64>;
-    using raw = typename Elem<ST>::raw;
     constexpr int CW = CPLX ? 2 : 1;
-    const raw *__restrict__ x = static_cast<const raw *>(xv);
-    const A *__restrict__ gout = static_cast<const A *>(goutv);
-
-    __shared__ A red[8];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int tile = blockIdx.x * (blockDim.x >> 6) + wave;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     bool valid;
     const int n = voxel_of(tl, tile, lane, valid);
-    A scale_part = A(0);
-
+    AccT<ACC64> part = 0;
     if (valid) {
         const int b0 = blockIdx.y * FT;
-        const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
-        A gv[FT][CW];
-        const raw *fb[FT];
-#pragma unroll
-        for (int f = 0; f < FT; f++) {
-            const int b = b0 + f;
-            const int bc = min(b, g.B - 1);
-            fb[f] = x + (int64_t)bc * xs_b * CW;
-#pragma unroll
-            for (int c = 0; c < CW; c++)
-                gv[f][c] = (b < g.B) ? gout[((int64_t)b * gs_b + n) * CW + c] : A(0);
-        }
-        const bool unit = g.unit_scale != 0;
-
-        for (int kc = 0; kc < g.K; kc += KC) {
-            float ofs[KC], wt[KC];
-            int sq[KC];
-            A d_tau[KC], d_w[KC];
-#pragma unroll
-            for (int j = 0; j < KC; j++) {
-                const int jj = kc + j;
-                d_tau[j] = A(0);
-                d_w[j] = A(0);
-                if (jj < g.K) {
-                    const int64_t o = (int64_t)jj * g.N + n;
-                    ofs[j] = g.offsets[o];
-                    wt[j] = g.weights[o];
-                    sq[j] = g.sparse ? g.sparse[o] : jj;
-                } else {
-                    ofs[j] = 0.0f;
-                    wt[j] = 0.0f;
-                    sq[j] = -1;
-                }
-            }
-            for (int m = 0; m < g.C; m++) {
-                const SplitPt pp =
-                    split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
-                const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
-                bool dok = true;
-                if (g.srcdir) {
-                    const float *sd = g.srcdir + 4 * m;
-                    dok = (R.dx * sd[0] + R.dy * sd[1] + R.dz * sd[2]) * R.rinv > sd[3];
-                }
-                const uint32_t rowc = (uint32_t)((int64_t)m * xs_c);
-#pragma unroll
-                for (int j = 0; j < KC; j++) {
-                    if (kc + j >= g.K) break;
-                    const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
-                    const bool in = dok && (uint32_t)tp.k < (uint32_t)(g.S - 1) &&
-                                    (uint32_t)sq[j] < (uint32_t)g.T;
-                    if (in) {
-                        const uint32_t off =
-                            rowc + (uint32_t)((int64_t)sq[j] * xs_t) + (uint32_t)tp.k;
-                        const A t = (A)tp.t;
-                        if constexpr (!CPLX) {
-                            A sd_ = A(0), sl_ = A(0);
-#pragma unroll
-                            for (int f = 0; f < FT; f++) {
-                                A x0, x1;
-                                load_pair<ST>(fb[f] + off, x0, x1);
-                                const A d = x1 - x0;
-                                sd_ += gv[f][0] * d;
-                                sl_ += gv[f][0] * (x0 + t * d);
-                            }
-                            const A dt = (A)wt[j] * sd_;
-                            d_tau[j] += dt;
-                            d_w[j] += sl_;
-                            scale_part += dt * ((A)ofs[j] + (A)R.rh + (A)R.rl);
-                        } else {
-                            // Q = sum conj(g) (x1 - x0), L = sum conj(g) lerp
-                            A qr = 0, qi = 0, lr = 0, li = 0;
-#pragma unroll
-                            for (int f = 0; f < FT; f++) {
-                                A x0r, x0i, x1r, x1i;
-                                load_pair_c<ST>(fb[f] + 2 * (size_t)off, x0r, x0i, x1r, x1i);
-                                const A dr = x1r - x0r, di = x1i - x0i;
-                                const A pr = x0r + t * dr, pi = x0i + t * di;
-                                const A gr = gv[f][0], gi = gv[f][1];
-                                qr += gr * dr + gi * di;
-                                qi += gr * di - gi * dr;
-                                lr += gr * pr + gi * pi;
-                                li += gr * pi - gi * pr;
-                            }
-                            float sn, cs;
-                            phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
-                            const A er = (A)cs, ei = (A)sn, kap = (A)g.wavenum;
-                            // Re(e (Q + i kap L)), Re(e L)
-                            const A zr = qr - kap * li, zi = qi + kap * lr;
-                            const A dt = (A)wt[j] * (er * zr - ei * zi);
-                            d_tau[j] += dt;
-                            d_w[j] += er * lr - ei * li;
-                            scale_part += dt * ((A)ofs[j] + (A)R.rh + (A)R.rl);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < KC; j++) {
-                const int jj = kc + j;
-                if (jj >= g.K) break;
-                const int64_t o = (int64_t)jj * g.N + n;
-                const float vo = (float)(d_tau[j] * (A)g.scale);
-                const float vw = (float)d_w[j];
-                if (accumulate) {
-                    if (gofs) atomicAdd(gofs + o, vo);
-                    if (gwt) atomicAdd(gwt + o, vw);
-                } else {
-                    if (gofs) gofs[o] = vo;
-                    if (gwt) gwt[o] = vw;
-                }
-            }
-        }
+        const NaturalRows<ST, FT, CW> xr(xv, xs_b, xs_c, xs_t, b0, g.B);
+        part = das_bwd_param_body<NaturalRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
+            g, xr, n, b0, goutv, gs_b, gofs, gwt, accumulate);
     }
-    if (gscale) {
-        A v = wave_sum<ACC64>(scale_part);
-        if (lane == 0) red[wave] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            A s = 0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
-            atomicAdd(gscale, (float)s);
+    if (gscale) block_scale_sum<ACC64>(part, gscale);
+}
+
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__global__ __launch_bounds__(256) void das_bwd_param_fi_kernel(
+    Geo g, Tiling tl, const void *__restrict__ xv, const void *__restrict__ goutv, int64_t gs_b,
+    float *__restrict__ gofs, float *__restrict__ gwt, float *__restrict__ gscale, int accumulate) {
+    constexpr int CW = CPLX ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    bool valid;
+    const int n = voxel_of(tl, tile, lane, valid);
+    AccT<ACC64> part = 0;
+    if (valid) {
+        const int chunk = blockIdx.y;
+        const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
+        part = das_bwd_param_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
+            g, xr, n, chunk * FT, goutv, gs_b, gofs, gwt, accumulate);
+    }
+    if (gscale) block_scale_sum<ACC64>(part, gscale);
+}
+
+// Relayout + cast into the frame-inner layout: dst[chunk][c][t][k][f] = src[chunk*FT+f][c][t][k]
+// (zero for frames past B).  One block handles a 64-sample segment of one (chunk, c, t) row
+// for all FT frames, transposing through LDS so reads and writes are both contiguous.
+template <Storage SS, class DST, int FT, int CW>
+__global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS>::raw *__restrict__ src, int64_t sb,
+                                                          int64_t sc, int64_t st, int B, int C,
+                                                          int T, int S, DST *__restrict__ dst) {
+    __shared__ float tile[FT][64 * CW + 1];
+    const int seg = blockIdx.x;                 // 64-sample segment
+    const int row = blockIdx.y;                 // c * T + t
+    const int chunk = blockIdx.z;
+    const int c = row / T, t = row - c * T;
+    const int k0 = seg * 64;
+    for (int i = threadIdx.x; i < FT * 64 * CW; i += blockDim.x) {
+        const int f = i / (64 * CW), e = i - f * 64 * CW;
+        const int k = k0 + e / CW;
+        const int b = chunk * FT + f;
+        float v = 0.0f;
+        if (b < B && k < S) {
+            const typename Elem<SS>::raw *p =
+                src + ((int64_t)b * sb + (int64_t)c * sc + (int64_t)t * st) * CW;
+            v = (float)Elem<SS>::real(p[(int64_t)k * CW + (e % CW)]);
         }
+        tile[f][e] = v;
+    }
+    __syncthreads();
+    DST *d = dst + ((((int64_t)chunk * C + c) * T + t) * S) * FT * CW;
+    for (int i = threadIdx.x; i < 64 * FT * CW; i += blockDim.x) {
+        const int kk = i / (FT * CW), r = i - kk * FT * CW;
+        const int f = r / CW, cc = r - f * CW;
+        const int k = k0 + kk;
+        if (k < S) d[((int64_t)k * FT + f) * CW + cc] = convert_to<DST>(tile[f][kk * CW + cc]);
     }
 }
 
 // ---------------------------------------------------------------------------------------
 // adjoint: x (channel-row scatter through LDS)
 // ---------------------------------------------------------------------------------------
-// grid.x = C*T rows (m = row / T, s = row % T), grid.y = frame chunks of FS.
-// LDS holds the row as [k][f] with a padded frame stride fsp = FS|1 (odd: conflict-free
-// row write-back).  Voxels are visited in tile order (tiling of the forward) so that lanes
-// of one wave fall on different samples.
-template <bool ACC64, bool CPLX, bool PRECISE>
-__global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const void *__restrict__ goutv,
-                                                        int64_t gs_b, void *__restrict__ gxv,
-                                                        int64_t gx_b, int64_t gx_c, int64_t gx_t,
-                                                        int FS) {
+// grad_out transposed to frame-inner chunks: gt[chunk][n][f] (f < FS), zero past B.
+template <bool ACC64, bool CPLX, int FS>
+__global__ void transpose_gout_kernel(const void *__restrict__ goutv, int64_t gs_b, int B, int N,
+                                      void *__restrict__ gtv) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
-    const int FSP = FS | 1;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    A *lds = reinterpret_cast<A *>(lds_raw);
     const A *__restrict__ gout = static_cast<const A *>(goutv);
+    A *__restrict__ gt = static_cast<A *>(gtv);
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (n >= N) return;
+    A v[FS * CW];
+#pragma unroll
+    for (int f = 0; f < FS; f++) {
+        const int b = chunk * FS + f;
+#pragma unroll
+        for (int c = 0; c < CW; c++)
+            v[f * CW + c] = b < B ? gout[((int64_t)b * gs_b + n) * CW + c] : A(0);
+    }
+    A *dst = gt + ((int64_t)chunk * N + n) * FS * CW;
+#pragma unroll
+    for (int i = 0; i < FS * CW; i++) dst[i] = v[i];
+}
+
+// max |v| over n scalars, folded into *out as float bits (non-negative floats order like
+// their bit patterns, so an integer atomicMax suffices).
+template <class T>
+__global__ void absmax_kernel(const T *__restrict__ a, int64_t n, unsigned *__restrict__ out) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, (float)fabs((double)a[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+// Fixed-point accumulation: a contribution v becomes the 64-bit integer round-down of
+// v * 2^E * 2^32 (hi word: floor(v 2^E), lo word: the fraction times 2^32), so sums are
+// exact, order-independent integer additions (LDS integer atomics run ~40x faster than
+// LDS float atomics on gfx950).  E is chosen so every |v| 2^E < 2^30.
+template <class A>
+__device__ __forceinline__ void fx_add(unsigned long long *p, A v, A scale) {
+    const A y = v * scale;                       // exact: scale is a power of two
+    const A hi = floor(y);
+    const A fr = y - hi;                         // exact, in [0, 1)
+    const unsigned lo = (unsigned)(fr * (A)4294967296.0);
+    const unsigned hw = (unsigned)(int)hi;
+    atomicAdd(p, ((unsigned long long)hw << 32) | (unsigned long long)lo);
+}
+
+// Exponent E with maxw * maxg * 2 * 2^E < 2^30 (the factor 2 bounds complex products).
+__device__ __forceinline__ int fx_exponent(const unsigned *maxes) {
+    const double bound = (double)__uint_as_float(maxes[0]) * (double)__uint_as_float(maxes[1]);
+    if (!(bound > 0.0)) return 0;
+    int e;
+    frexp(bound, &e);                            // bound < 2^e
+    int E = 28 - e;
+    return E < -120 ? -120 : (E > 120 ? 120 : E);
+}
+
+// grid.x = C*T rows (m = row / T, s = row % T), grid.y = frame chunks of FS frames.
+// The row lives in LDS as 64-bit fixed point [k][f] with an odd frame stride FSP
+// (conflict-free write-back); every lane owns one voxel, loads its FS grad_out values with
+// vector loads, then adds its two interpolation weights times grad_out into the row.
+template <bool ACC64, bool CPLX, bool PRECISE, int FS>
+__global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const void *__restrict__ gtv,
+                                                        const unsigned *__restrict__ maxes,
+                                                        void *__restrict__ gxv, int64_t gx_b,
+                                                        int64_t gx_c, int64_t gx_t) {
+    using A = AccT<ACC64>;
+    using U64 = unsigned long long;
+    constexpr int CW = CPLX ? 2 : 1;
+    constexpr int FSP = (FS > 1) ? (FS | 1) : 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    U64 *lds = reinterpret_cast<U64 *>(lds_raw);
+    const A *__restrict__ gt = static_cast<const A *>(gtv);
     A *__restrict__ gx = static_cast<A *>(gxv);
 
     const int row = blockIdx.x;
     const int m = row / g.T;
     const int s = row - m * g.T;
-    const int b0 = blockIdx.y * FS;
+    const int chunk = blockIdx.y;
+    const int b0 = chunk * FS;
     const int nf = min(FS, g.B - b0);
     const int nlds = g.S * FSP * CW;
-    for (int i = threadIdx.x; i < nlds; i += blockDim.x) lds[i] = A(0);
+    for (int i = threadIdx.x; i < nlds; i += blockDim.x) lds[i] = 0ull;
+    const int E = fx_exponent(maxes);
+    const A scale = (A)ldexp(1.0, E);
     __syncthreads();
 
     const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
@@ -339,10 +566,16 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
     const int lane = threadIdx.x & 63;
     const int waves = blockDim.x >> 6;
     const int ntiles = tl.nta * tl.nth * tl.ntw;
+    const A *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
     for (int tile = threadIdx.x >> 6; tile < ntiles; tile += waves) {
         bool valid;
         const int n = voxel_of(tl, tile, lane, valid);
         if (!valid) continue;
+        // grad_out of this voxel first: its latency overlaps the geometry below
+        A gv[FS * CW];
+        const A *gp = gchunk + (int64_t)n * FS * CW;
+#pragma unroll
+        for (int i = 0; i < FS * CW; i++) gv[i] = gp[i];
         const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
         const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
         if (g.srcdir && !((R.dx * sdx + R.dy * sdy + R.dz * sdz) * R.rinv > sdw)) continue;
@@ -354,40 +587,45 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
             const Tap tp = make_tap<PRECISE>(g.offsets[o], R, g.scale, unit);
             if ((uint32_t)tp.k >= (uint32_t)(g.S - 1)) continue;
             const float w = g.weights[o];
-            A *l0 = lds + (size_t)tp.k * FSP * CW;
-            A *l1 = l0 + FSP * CW;
-            const A t = (A)tp.t, omt = A(1) - (A)tp.t;
+            U64 *l0 = lds + (size_t)tp.k * FSP * CW;
+            U64 *l1 = l0 + FSP * CW;
+            const A t = (A)tp.t;
             if constexpr (!CPLX) {
-                const A w0 = (A)w * omt, w1 = (A)w * t;
-                for (int f = 0; f < nf; f++) {
-                    const A gvv = gout[(int64_t)(b0 + f) * gs_b + n];
-                    atomicAdd(l0 + f, w0 * gvv);
-                    atomicAdd(l1 + f, w1 * gvv);
+                const A w1 = (A)w * t, w0 = (A)w - w1;
+#pragma unroll
+                for (int f = 0; f < FS; f++) {
+                    fx_add<A>(l0 + f, w0 * gv[f], scale);
+                    fx_add<A>(l1 + f, w1 * gv[f], scale);
                 }
             } else {
                 float sn, cs;
                 phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
                 const A cr = (A)(w * cs), ci = (A)(w * sn);
-                for (int f = 0; f < nf; f++) {
-                    const A *gp = gout + ((int64_t)(b0 + f) * gs_b + n) * 2;
-                    const A gr = gp[0], gi = gp[1];
+                const A omt = A(1) - t;
+#pragma unroll
+                for (int f = 0; f < FS; f++) {
+                    const A gr = gv[2 * f], gi = gv[2 * f + 1];
                     // conj(coef) * g
                     const A vr = cr * gr + ci * gi, vi = cr * gi - ci * gr;
-                    atomicAdd(l0 + 2 * f, vr * omt);
-                    atomicAdd(l0 + 2 * f + 1, vi * omt);
-                    atomicAdd(l1 + 2 * f, vr * t);
-                    atomicAdd(l1 + 2 * f + 1, vi * t);
+                    fx_add<A>(l0 + 2 * f, vr * omt, scale);
+                    fx_add<A>(l0 + 2 * f + 1, vi * omt, scale);
+                    fx_add<A>(l1 + 2 * f, vr * t, scale);
+                    fx_add<A>(l1 + 2 * f + 1, vi * t, scale);
                 }
             }
         }
     }
     __syncthreads();
     // write back: consecutive threads -> consecutive samples
+    const double inv = ldexp(1.0, -E - 32);
     for (int f = 0; f < nf; f++) {
         A *dst = gx + ((int64_t)(b0 + f) * gx_b + (int64_t)m * gx_c + (int64_t)s * gx_t) * CW;
         for (int k = threadIdx.x; k < g.S; k += blockDim.x) {
 #pragma unroll
-            for (int c = 0; c < CW; c++) dst[(size_t)k * CW + c] = lds[((size_t)k * FSP + f) * CW + c];
+            for (int c = 0; c < CW; c++) {
+                const long long q = (long long)lds[((size_t)k * FSP + f) * CW + c];
+                dst[(size_t)k * CW + c] = (A)((double)q * inv);
+            }
         }
     }
 }
@@ -404,17 +642,6 @@ __global__ void fill_zero_kernel(float *p, int64_t n) {
 // ---------------------------------------------------------------------------------------
 // cast / relayout: strided ([B,]C,T,S) of type SRC -> contiguous (B,C,T,S) of type DST
 // ---------------------------------------------------------------------------------------
-template <class D> __device__ __forceinline__ D convert_to(float v);
-template <> __device__ __forceinline__ float convert_to<float>(float v) { return v; }
-template <> __device__ __forceinline__ _Float16 convert_to<_Float16>(float v) { return (_Float16)v; }
-template <> __device__ __forceinline__ uint16_t convert_to<uint16_t>(float v) {
-    // round to nearest even; NaN stays NaN
-    uint32_t u = __float_as_uint(v);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
 template <class SRC, class DST>
 __global__ void cast_rows_kernel(const SRC *__restrict__ src, int64_t sb, int64_t sc, int64_t st,
                                  DST *__restrict__ dst, int C, int T, int S, int cw) {
@@ -495,6 +722,7 @@ hipError_t launch_fwd_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &t
     constexpr bool PRECISE = (ST == Storage::F32 || ST == Storage::F64);
     const int ntiles = tl.nta * tl.nth * tl.ntw;
     dim3 grid((ntiles + 3) / 4, (g.B + FT - 1) / FT);
+    ProfScope ps(st, "das_fwd");
     dev::das_fwd_kernel<ST, ACC64, CPLX, PRECISE, FT, kKC><<<grid, 256, 0, st>>>(
         g, tl, xv.ptr, xv.sb, xv.sc, xv.st, out, os_b, br, bi);
     return hipGetLastError();
@@ -519,6 +747,7 @@ hipError_t launch_bwdp_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &
     constexpr bool PRECISE = (ST == Storage::F32 || ST == Storage::F64);
     const int ntiles = tl.nta * tl.nth * tl.ntw;
     dim3 grid((ntiles + 3) / 4, (g.B + FT - 1) / FT);
+    ProfScope ps(st, "das_bwd_param");
     dev::das_bwd_param_kernel<ST, ACC64, CPLX, PRECISE, FT, kKC><<<grid, 256, 0, st>>>(
         g, tl, xv.ptr, xv.sb, xv.sc, xv.st, gout, gs_b, gofs, gwt, gscale, grid.y > 1 ? 1 : 0);
     return hipGetLastError();
@@ -537,18 +766,131 @@ hipError_t launch_bwdp_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &t
     }
 }
 
+template <bool ACC64, bool CPLX, bool PRECISE, int FS>
+hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+                          const void *gout, int64_t gs_b, void *gt, void *gx) {
+    using A = typename std::conditional<ACC64, double, float>::type;
+    const size_t esz = 8 * (CPLX ? 2 : 1);
+    const int fsp = FS > 1 ? (FS | 1) : 1;
+    const size_t lds = (size_t)g.S * (size_t)fsp * esz;
+    const int nchunks = (g.B + FS - 1) / FS;
+    // workspace: [maxes: 2 words, padded to 256 B][transposed grad_out]
+    unsigned *maxes = static_cast<unsigned *>(gt);
+    void *gtr = static_cast<char *>(gt) + 256;
+    {
+    ProfScope ps(st, "absmax");
+    dev::fill_zero_kernel<<<1, 64, 0, st>>>(reinterpret_cast<float *>(maxes), 2);
+    dev::absmax_kernel<float><<<256, 256, 0, st>>>(g.weights, (int64_t)g.K * g.N, maxes);
+    const int64_t gspan = ((int64_t)(g.B - 1) * gs_b + g.N) * (CPLX ? 2 : 1);
+    dev::absmax_kernel<A><<<512, 256, 0, st>>>(static_cast<const A *>(gout), gspan, maxes + 1);
+    }
+    {
+    ProfScope pt(st, "transpose_gout");
+    dev::transpose_gout_kernel<ACC64, CPLX, FS><<<dim3((g.N + 255) / 256, nchunks), 256, 0, st>>>(
+        gout, gs_b, g.B, g.N, gtr);
+    }
+    dim3 grid(g.C * g.T, nchunks);
+    const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
+    hipFuncSetAttribute((const void *)dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    ProfScope ps(st, "das_bwd_x");
+    dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, 512, lds, st>>>(
+        g, tl, gtr, maxes, gx, gx_b, gx_c, gx_t);
+    return hipGetLastError();
+}
+
 template <bool ACC64, bool CPLX, bool PRECISE>
 hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
-                         const void *gout, int64_t gs_b, void *gx, int FS) {
-    const size_t esz = (ACC64 ? 8 : 4) * (CPLX ? 2 : 1);
-    const size_t lds = (size_t)g.S * (size_t)(FS | 1) * esz;
-    dim3 grid(g.C * g.T, (g.B + FS - 1) / FS);
-    const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
-    hipFuncSetAttribute((const void *)dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE><<<grid, 512, lds, st>>>(
-        g, tl, gout, gs_b, gx, gx_b, gx_c, gx_t, FS);
+                         const void *gout, int64_t gs_b, void *gt, void *gx, int FS) {
+    switch (FS) {
+        case 16: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 16>(st, g, tl, gout, gs_b, gt, gx);
+        case 8: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 8>(st, g, tl, gout, gs_b, gt, gx);
+        case 4: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 4>(st, g, tl, gout, gs_b, gt, gx);
+        case 2: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 2>(st, g, tl, gout, gs_b, gt, gx);
+        default: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 1>(st, g, tl, gout, gs_b, gt, gx);
+    }
+}
+
+template <Storage ST, bool CPLX, int F>
+constexpr bool fi_ok() {
+    return (F * (CPLX ? 2 : 1) * (ST == Storage::F32 ? 4 : 2)) % 16 == 0;
+}
+
+template <Storage ST, bool CPLX, int F>
+hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+                           const void *x, void *out, int64_t os_b, double br, double bi) {
+    if constexpr (!fi_ok<ST, CPLX, F>()) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr bool PRECISE = (ST == Storage::F32);
+        const int ntiles = tl.nta * tl.nth * tl.ntw;
+        dim3 grid((ntiles + 3) / 4, (g.B + F - 1) / F);
+        ProfScope ps(st, "das_fwd_fi");
+        dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 256, 0, st>>>(
+            g, tl, x, out, os_b, br, bi);
+        return hipGetLastError();
+    }
+}
+
+template <Storage ST, bool CPLX>
+hipError_t launch_fwd_fi_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int F,
+                           const void *x, void *out, int64_t os_b, double br, double bi) {
+    switch (F) {
+        case 16: return launch_fwd_fi_f<ST, CPLX, 16>(st, g, tl, x, out, os_b, br, bi);
+        case 8: return launch_fwd_fi_f<ST, CPLX, 8>(st, g, tl, x, out, os_b, br, bi);
+        case 4: return launch_fwd_fi_f<ST, CPLX, 4>(st, g, tl, x, out, os_b, br, bi);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <Storage ST, bool CPLX, int F>
+hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+                            const void *x, const void *gout, int64_t gs_b, float *gofs,
+                            float *gwt, float *gscale) {
+    if constexpr (!fi_ok<ST, CPLX, F>()) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr bool PRECISE = (ST == Storage::F32);
+        const int ntiles = tl.nta * tl.nth * tl.ntw;
+        dim3 grid((ntiles + 3) / 4, (g.B + F - 1) / F);
+        ProfScope ps(st, "das_bwd_param_fi");
+        dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 256, 0, st>>>(
+            g, tl, x, gout, gs_b, gofs, gwt, gscale, grid.y > 1 ? 1 : 0);
+        return hipGetLastError();
+    }
+}
+
+template <Storage ST, bool CPLX>
+hipError_t launch_bwdp_fi_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int F,
+                            const void *x, const void *gout, int64_t gs_b, float *gofs,
+                            float *gwt, float *gscale) {
+    switch (F) {
+        case 16: return launch_bwdp_fi_f<ST, CPLX, 16>(st, g, tl, x, gout, gs_b, gofs, gwt, gscale);
+        case 8: return launch_bwdp_fi_f<ST, CPLX, 8>(st, g, tl, x, gout, gs_b, gofs, gwt, gscale);
+        case 4: return launch_bwdp_fi_f<ST, CPLX, 4>(st, g, tl, x, gout, gs_b, gofs, gwt, gscale);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <Storage SS, class DST, bool CPLX, int F>
+hipError_t launch_relayout_f(hipStream_t st, const DasProblem &p, const void *x, void *dst) {
+    constexpr int CW = CPLX ? 2 : 1;
+    dim3 grid((p.S + 63) / 64, p.C * p.T, (p.B + F - 1) / F);
+    ProfScope ps(st, "relayout_fi");
+    dev::relayout_fi_kernel<SS, DST, F, CW><<<grid, 256, 0, st>>>(
+        static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B, p.C,
+        p.T, p.S, static_cast<DST *>(dst));
     return hipGetLastError();
+}
+
+template <Storage SS, class DST, bool CPLX>
+hipError_t launch_relayout_t(hipStream_t st, const DasProblem &p, const void *x, void *dst) {
+    switch (p.fi_frames) {
+        case 16: return launch_relayout_f<SS, DST, CPLX, 16>(st, p, x, dst);
+        case 8: return launch_relayout_f<SS, DST, CPLX, 8>(st, p, x, dst);
+        case 4: return launch_relayout_f<SS, DST, CPLX, 4>(st, p, x, dst);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -560,6 +902,25 @@ ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeo
     hipError_t e = hipSuccess;
     hipStream_t st = h.stream;
     const int64_t os_b = p.os_b;
+    if (p.fi_frames > 0) {
+        const int F = p.fi_frames;
+        switch (p.k_storage) {
+            case Storage::F16:
+                e = p.cplx ? launch_fwd_fi_t<Storage::F16, true>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im)
+                           : launch_fwd_fi_t<Storage::F16, false>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im);
+                break;
+            case Storage::BF16:
+                e = p.cplx ? launch_fwd_fi_t<Storage::BF16, true>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im)
+                           : launch_fwd_fi_t<Storage::BF16, false>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im);
+                break;
+            case Storage::F32:
+                e = p.cplx ? launch_fwd_fi_t<Storage::F32, true>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im)
+                           : launch_fwd_fi_t<Storage::F32, false>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im);
+                break;
+            default: return FFDAS_ERROR_INTERNAL;
+        }
+        return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+    }
     if (!p.cplx) {
         switch (p.k_storage) {
             case Storage::F16: e = launch_fwd_t<Storage::F16, false, false>(st, g, tl, xv, out, os_b, beta_re, beta_im); break;
@@ -592,9 +953,35 @@ ffdas_error_t launch_fill_zero(ffdas_context &h, float *p, int64_t n) {
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
 }
 
+// Frames per workgroup of the grad_x scatter: the largest of 16/8/4/2/1 whose LDS row fits
+// twice per CU (two workgroups resident), else once; FFDAS_BWDX_FRAMES overrides.
+int bwdx_frames(const DasProblem &p) {
+    const size_t esz = 8 * (p.cplx ? 2 : 1);   // 64-bit fixed-point LDS row
+    auto lds = [&](int fs) { return (size_t)p.S * (size_t)(fs > 1 ? (fs | 1) : 1) * esz; };
+    if (const char *env = std::getenv("FFDAS_BWDX_FRAMES")) {
+        const int fs = std::atoi(env);
+        if ((fs == 1 || fs == 2 || fs == 4 || fs == 8 || fs == 16) && lds(fs) <= (size_t)kLdsLimit)
+            return fs;
+    }
+    const int cap = p.B >= 16 ? 16 : p.B >= 8 ? 8 : p.B >= 4 ? 4 : p.B >= 2 ? 2 : 1;
+    for (int fs = cap; fs >= 1; fs /= 2)
+        if (lds(fs) <= (size_t)kLdsLimit / 2) return fs;
+    for (int fs = cap; fs >= 1; fs /= 2)
+        if (lds(fs) <= (size_t)kLdsLimit) return fs;
+    return 0;
+}
+
+size_t backward_workspace(const DasProblem &p) {
+    const int fs = bwdx_frames(p);
+    if (fs <= 0) return 0;
+    const size_t esz = (p.acc64 ? 8 : 4) * (p.cplx ? 2 : 1);
+    const size_t nchunks = (size_t)(p.B + fs - 1) / fs;
+    return 256 + nchunks * (size_t)fs * (size_t)p.N * esz;
+}
+
 ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
                               const XView &xv, const void *gout, int64_t gs_b, void *gx,
-                              float *gofs, float *gwt, float *gscale) {
+                              float *gofs, float *gwt, float *gscale, void *gt) {
     const dev::Geo g = make_geo(a, p);
     const dev::Tiling tl = make_tiling(p.spatial);
     hipStream_t st = h.stream;
@@ -602,7 +989,7 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     const bool precise = p.k_storage == Storage::F32 || p.k_storage == Storage::F64;
 
     if (gofs || gwt || gscale) {
-        const int FT = frame_tile_for(p.B);
+        const int FT = p.fi_frames > 0 ? p.fi_frames : frame_tile_for(p.B);
         const bool multi = (p.B + FT - 1) / FT > 1;
         if (multi) {
             FFDAS_TRY(launch_fill_zero(h, gofs, (int64_t)p.K * p.N));
@@ -610,7 +997,15 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
         }
         FFDAS_TRY(launch_fill_zero(h, gscale, 1));
 #define BWDP(ST, A64, CX) e = launch_bwdp_t<ST, A64, CX>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale)
-        if (!p.cplx) {
+#define BWDPFI(ST, CX) e = launch_bwdp_fi_t<ST, CX>(st, g, tl, p.fi_frames, xv.ptr, gout, gs_b, gofs, gwt, gscale)
+        if (p.fi_frames > 0) {
+            switch (p.k_storage) {
+                case Storage::F16: if (p.cplx) BWDPFI(Storage::F16, true); else BWDPFI(Storage::F16, false); break;
+                case Storage::BF16: if (p.cplx) BWDPFI(Storage::BF16, true); else BWDPFI(Storage::BF16, false); break;
+                case Storage::F32: if (p.cplx) BWDPFI(Storage::F32, true); else BWDPFI(Storage::F32, false); break;
+                default: return FFDAS_ERROR_INTERNAL;
+            }
+        } else if (!p.cplx) {
             switch (p.k_storage) {
                 case Storage::F16: BWDP(Storage::F16, false, false); break;
                 case Storage::BF16: BWDP(Storage::BF16, false, false); break;
@@ -626,22 +1021,21 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
             }
         }
 #undef BWDP
+#undef BWDPFI
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }
     if (gx) {
-        const size_t esz = (p.acc64 ? 8 : 4) * (p.cplx ? 2 : 1);
-        int FS = std::min(p.B, 16);
-        while (FS > 0 && (size_t)p.S * (size_t)(FS | 1) * esz > (size_t)kLdsLimit) FS--;
+        const int FS = bwdx_frames(p);
         if (FS <= 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
         if (p.acc64) {
-            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, tl, gout, gs_b, gx, FS)
-                       : launch_bwdx_t<true, false, true>(st, g, tl, gout, gs_b, gx, FS);
+            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, tl, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<true, false, true>(st, g, tl, gout, gs_b, gt, gx, FS);
         } else if (precise) {
-            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, tl, gout, gs_b, gx, FS)
-                       : launch_bwdx_t<false, false, true>(st, g, tl, gout, gs_b, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, tl, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, true>(st, g, tl, gout, gs_b, gt, gx, FS);
         } else {
-            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, tl, gout, gs_b, gx, FS)
-                       : launch_bwdx_t<false, false, false>(st, g, tl, gout, gs_b, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, tl, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, false>(st, g, tl, gout, gs_b, gt, gx, FS);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }
@@ -656,6 +1050,7 @@ ffdas_error_t launch_cast(ffdas_context &h, const DasProblem &p, const void *x, 
     dim3 grid((unsigned)rows);
     const int threads = 256;
     hipStream_t st = h.stream;
+    ProfScope ps(st, "cast_rows");
 #define CAST(SRC, DST) \
     dev::cast_rows_kernel<SRC, DST><<<grid, threads, 0, st>>>((const SRC *)x, p.xs_b, p.xs_c, p.xs_t, (DST *)dst, p.C, p.T, p.S, cw)
     using bf = uint16_t;
@@ -686,6 +1081,62 @@ ffdas_error_t launch_cast(ffdas_context &h, const DasProblem &p, const void *x, 
     }
 #undef CAST
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+// Frame-inner layout choice: frames per chunk F (16/8/4) for the gather kernels, or 0 for the
+// natural layout.  ALG1 keeps the natural-layout (plain) kernels; double accumulation and
+// double storage keep them too.  Padding may add at most as many frames as there are.
+int fi_frames_for(const DasProblem &p, ffdas_alg_t alg) {
+    if (alg == FFDAS_ALG1 || p.acc64 || p.k_storage == Storage::F64 || p.B < 4) return 0;
+    if (const char *env = std::getenv("FFDAS_LAYOUT"))
+        if (std::strcmp(env, "natural") == 0) return 0;
+    const int esz = (int)storage_bytes(p.k_storage) * (p.cplx ? 2 : 1);
+    // cost ~ padded frames * (1 + G / F): per-voxel delay arithmetic is worth about G
+    // interpolations and is shared by the F frames of a chunk
+    constexpr double G = 10.0;
+    int best = 0;
+    double best_cost = 0.0;
+    for (int F : {16, 8, 4}) {
+        if ((F * esz) % 16 != 0) continue;
+        const double padded = (double)((p.B + F - 1) / F) * F;
+        if (padded > 2.0 * p.B) continue;
+        const double cost = padded * (1.0 + G / F);
+        if (best == 0 || cost < best_cost) {
+            best = F;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+size_t fi_bytes(const DasProblem &p) {
+    if (p.fi_frames <= 0) return 0;
+    const size_t chunks = (size_t)(p.B + p.fi_frames - 1) / p.fi_frames;
+    return chunks * (size_t)p.fi_frames * (size_t)p.C * p.T * p.S * (p.cplx ? 2 : 1) *
+           storage_bytes(p.k_storage);
+}
+
+ffdas_error_t launch_relayout_fi(ffdas_context &h, const DasProblem &p, const void *x, void *dst) {
+    hipStream_t st = h.stream;
+    hipError_t e = hipErrorInvalidValue;
+    using bf = uint16_t;
+#define RL(SS, DST) e = p.cplx ? launch_relayout_t<SS, DST, true>(st, p, x, dst) : launch_relayout_t<SS, DST, false>(st, p, x, dst)
+    switch (p.x_storage) {
+        case Storage::F32:
+            if (p.k_storage == Storage::F16) RL(Storage::F32, _Float16);
+            else if (p.k_storage == Storage::BF16) RL(Storage::F32, bf);
+            else if (p.k_storage == Storage::F32) RL(Storage::F32, float);
+            break;
+        case Storage::F16:
+            if (p.k_storage == Storage::F16) RL(Storage::F16, _Float16);
+            break;
+        case Storage::BF16:
+            if (p.k_storage == Storage::BF16) RL(Storage::BF16, bf);
+            break;
+        default: break;
+    }
+#undef RL
+    return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
 }
 
 }  // namespace ffdas

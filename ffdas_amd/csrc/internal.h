@@ -59,6 +59,7 @@ struct DasProblem {
     Storage k_storage = Storage::F32;     // storage the kernel gathers from
     bool acc64 = false;                   // double accumulation (double data)
     bool x_needs_copy = false;            // cast / relayout into a contiguous workspace
+    int fi_frames = 0;                    // > 0: frame-inner chunks of this many frames
     int64_t xs_b = 0, xs_c = 0, xs_t = 0; // strides of x as given (elements)
     int64_t os_b = 0;                     // batch stride of out (elements)
     // spatial shape of the output (without batch), used for voxel tiling
@@ -105,10 +106,28 @@ ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeo
                              const XView &xv, void *out, double beta_re, double beta_im);
 ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
                               const XView &xv, const void *gout, int64_t gs_b, void *gx,
-                              float *gofs, float *gwt, float *gscale);
+                              float *gofs, float *gwt, float *gscale, void *gt);
+// bytes of workspace the grad_x scatter needs for the transposed grad_out
+size_t backward_workspace(const DasProblem &p);
+int bwdx_frames(const DasProblem &p);
 ffdas_error_t launch_cast(ffdas_context &h, const DasProblem &p, const void *x, void *dst);
 ffdas_error_t launch_fill_zero(ffdas_context &h, float *p, int64_t n);
 
 size_t storage_bytes(Storage s);
+
+// opt-in per-kernel timing (profile.cpp)
+bool profiling_enabled();
+void *profile_begin(hipStream_t st);
+void profile_end(void *begin, hipStream_t st, const char *name);
+struct ProfScope {
+    void *ev;
+    hipStream_t st;
+    const char *name;
+    ProfScope(hipStream_t s, const char *n) : ev(profile_begin(s)), st(s), name(n) {}
+    ~ProfScope() { profile_end(ev, st, name); }
+};
+int fi_frames_for(const DasProblem &p, ffdas_alg_t alg);
+size_t fi_bytes(const DasProblem &p);
+ffdas_error_t launch_relayout_fi(ffdas_context &h, const DasProblem &p, const void *x, void *dst);
 
 }  // namespace ffdas

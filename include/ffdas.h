@@ -226,6 +226,15 @@ FFDAS_API ffdas_error_t ffdas_das_validate(ffdas_tensor_desc_t x_desc,
                                            const int *sparse_indices,
                                            ffdas_compute_type_t compute_type, ffdas_alg_t alg);
 
+/* Opt-in kernel timing: while enabled, every kernel the library launches is bracketed by
+ * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
+ * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
+ * "das_bwd_x", "relayout_fi", "cast_rows", "transpose_gout", "absmax").  Counterpart of the
+ * reference's NVTX ranges (no reference symbol). */
+FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
+FFDAS_API ffdas_error_t ffdas_profile_reset(void);
+FFDAS_API ffdas_error_t ffdas_profile_query(const char *name, double *total_ms, int64_t *count);
+
 /* Library version string. */
 FFDAS_API const char *ffdas_version(void);
 

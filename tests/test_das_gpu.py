@@ -188,3 +188,71 @@ def test_config3_iq_complex(dev):
     print(f"config3 IQ complex64 rel-L2 = {e32:.3e}, complex32 (half2) rel-L2 = {e16:.3e}")
     assert e32 < 1e-5
     assert e16 < 1e-3
+
+
+# ---- frame-inner (batched) path: every chunking and storage -----------------------------
+
+@pytest.mark.parametrize("B", [4, 5, 16, 20])
+@pytest.mark.parametrize("storage", ["float32", "float16", "bfloat16"])
+def test_batched_layout_parity(dev, B, storage):
+    """DEFAULT runs the frame-inner kernels for B >= 4 (chunks of 4/8/16 frames, padding
+    frames at the tail); ALG1 runs the natural-layout kernel.  Both against the oracle."""
+    import torch
+    import ffdas_amd
+    w = W.small_random(seed=50 + B, B=B, C=7, T=3, S=120, N=(9, 11), srcdir=True)
+    xt = getattr(torch, storage)
+    d = to_dev(w, dev, xt)
+    xq = torch.from_numpy(w.x).to(xt).to(torch.float64).numpy()
+    ref = _ref(w, x=xq)
+    tol = 1e-5 if storage == "float32" else 1e-4
+    for alg in (ffdas_amd.Algorithm.DEFAULT, ffdas_amd.Algorithm.ALG1):
+        got = _das(d, algorithm=alg).cpu().numpy()
+        assert rel_l2(got, ref) < tol, (alg, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("B", [4, 16])
+def test_batched_complex_and_sparse(dev, B):
+    import torch
+    w = W.small_random(seed=70 + B, B=B, C=5, T=4, K=2, S=100, N=(6, 7), complex_iq=True,
+                       sparse=True)
+    got = _das(to_dev(w, dev)).cpu().numpy()
+    assert rel_l2(got, _ref(w)) < 1e-5
+    got16 = _das(to_dev(w, dev, torch.complex32)).cpu().numpy()
+    xq = torch.from_numpy(w.x).to(torch.complex32).to(torch.complex128).numpy()
+    assert rel_l2(got16, _ref(w, x=xq)) < 1e-4
+
+
+def test_use_fp16_on_batched_float_matches_fp16_tensor(dev):
+    import torch
+    w = W.small_random(seed=90, B=16, C=6, T=2, S=96, N=(8, 8))
+    a = _das(to_dev(w, dev), use_fp16=True).cpu().numpy()
+    b = _das(to_dev(w, dev, torch.float16)).cpu().numpy()
+    assert np.array_equal(a, b)
+
+
+def test_channels_trailing_and_strided_input(dev):
+    import torch
+    w = W.small_random(seed=91, B=8, C=5, T=3, S=80, N=(7, 6))
+    d = to_dev(w, dev)
+    ref = _das(d).cpu().numpy()
+    d2 = dict(d)
+    d2["x"] = d["x"].transpose(1, 2).contiguous()
+    got = _das(d2, channels_trailing=True).cpu().numpy()
+    assert np.allclose(got, ref, rtol=1e-6, atol=1e-6)
+    big = torch.zeros((8, 5, 3, 100), device=dev)
+    big[..., :80] = d["x"]
+    d3 = dict(d)
+    d3["x"] = big[..., :80]                     # non-contiguous (row stride 100)
+    got = _das(d3).cpu().numpy()
+    assert np.allclose(got, ref, rtol=1e-6, atol=1e-6)
+
+
+def test_out_and_beta_reuse(dev):
+    import torch
+    import ffdas_amd
+    w = W.small_random(seed=92, B=4)
+    d = to_dev(w, dev)
+    out = torch.empty((4,) + tuple(w.dstpos.shape[:-1]), device=dev)
+    r = _das(d, out=out)
+    assert r is out
+    assert rel_l2(out.cpu().numpy(), _ref(w)) < 1e-5
