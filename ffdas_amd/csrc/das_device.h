@@ -62,6 +62,7 @@ struct SplitPt {
 };
 
 __device__ __forceinline__ SplitPt split_point(float x, float y, float z) {
+#pragma clang fp contract(off)
     SplitPt s;
     s.x = x; s.y = y; s.z = z;
     s.cx = __builtin_rintf(x); s.cy = __builtin_rintf(y); s.cz = __builtin_rintf(z);
@@ -87,11 +88,16 @@ __device__ __forceinline__ Radius radius_fast(const SplitPt &y, const SplitPt &p
     return R;
 }
 
+// The error-free transformations below (TwoSum, exact products via fmaf) rely on every
+// product and sum being rounded exactly where written, so floating-point contraction is
+// switched off inside these functions; intended fused operations are explicit fmaf calls.
+//
 // Compensated FP32: the coordinate differences are formed exactly as integer + fraction,
 // the squared radius as an exact 32-bit integer plus an FP32 remainder, and one Newton
 // correction gives r = rh + rl with an error of a few 1e-8 samples over the whole range
 // of delays (instead of ~1e-4 for a single FP32 value near 2000 samples).
 __device__ __forceinline__ Radius radius_precise(const SplitPt &y, const SplitPt &p) {
+#pragma clang fp contract(off)
     Radius R;
     const float dcx = y.cx - p.cx, dcy = y.cy - p.cy, dcz = y.cz - p.cz;   // exact integers
     const float dfx = y.fx - p.fx, dfy = y.fy - p.fy, dfz = y.fz - p.fz;   // |.| <= 1
@@ -141,6 +147,7 @@ struct Tap {
 // tau = scale * (ofs + r)
 template <bool PRECISE>
 __device__ __forceinline__ Tap make_tap(float ofs, const Radius &R, float scale, bool unit_scale) {
+#pragma clang fp contract(off)
     Tap tp;
     if constexpr (!PRECISE) {
         float tau = ofs + R.rh;
@@ -176,6 +183,7 @@ __device__ __forceinline__ Tap make_tap(float ofs, const Radius &R, float scale,
 // sin/cos of 2*pi*u for |u| <= 0.5 by quadrant reduction and Taylor polynomials on
 // [-pi/4, pi/4] (truncation below 2e-9).
 __device__ __forceinline__ void sincos_rev_poly(float u, float &s, float &c) {
+#pragma clang fp contract(off)
     const float q = __builtin_rintf(4.0f * u);
     const float v = __builtin_fmaf(-0.25f, q, u);
     const float x = v * 6.28318530717958648f;
@@ -197,6 +205,7 @@ __device__ __forceinline__ void sincos_rev_poly(float u, float &s, float &c) {
 // rev_h + rev_l = wavenum / (2 pi) (double-float), tau = kf + t.
 template <bool PRECISE>
 __device__ __forceinline__ void phase(float rev_h, float rev_l, const Tap &tp, float &s, float &c) {
+#pragma clang fp contract(off)
     if constexpr (PRECISE) {
         const float p = rev_h * tp.kf;
         const float pe = __builtin_fmaf(rev_h, tp.kf, -p);

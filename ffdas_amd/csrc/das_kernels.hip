@@ -466,26 +466,42 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
 // ---------------------------------------------------------------------------------------
 // adjoint: x (channel-row scatter through LDS)
 // ---------------------------------------------------------------------------------------
-// grad_out transposed to frame-inner chunks: gt[chunk][n][f] (f < FS), zero past B.
+// Exponent E of the fixed-point scale: with bound = max|w| * max|g| < 2^e, every scaled
+// contribution |w g 2^E| < 2^min(40, 61 - log2cnt) stays inside the +-2^51 range of the FP64
+// round-to-integer trick, and the sum of at most 2^log2cnt contributions per sample fits in
+// 63 bits.
+__device__ __forceinline__ int fx_exponent(const unsigned *maxes, int log2cnt) {
+    const double bound = (double)__uint_as_float(maxes[0]) * (double)__uint_as_float(maxes[1]);
+    if (!(bound > 0.0)) return 0;
+    int e;
+    frexp(bound, &e);                            // bound < 2^e
+    const int top = min(40, 61 - log2cnt);
+    int E = top - e;
+    return E < -900 ? -900 : (E > 900 ? 900 : E);
+}
+
+// grad_out transposed to frame-inner chunks and pre-scaled for fixed-point accumulation:
+// gt[chunk][n][f] = grad_out[chunk*FS+f][n] * 2^E as double (zero past B).
 template <bool ACC64, bool CPLX, int FS>
 __global__ void transpose_gout_kernel(const void *__restrict__ goutv, int64_t gs_b, int B, int N,
-                                      void *__restrict__ gtv) {
+                                      const unsigned *__restrict__ maxes, int log2cnt,
+                                      double *__restrict__ gt) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
     const A *__restrict__ gout = static_cast<const A *>(goutv);
-    A *__restrict__ gt = static_cast<A *>(gtv);
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
     if (n >= N) return;
-    A v[FS * CW];
+    const double scale = ldexp(1.0, fx_exponent(maxes, log2cnt));
+    double v[FS * CW];
 #pragma unroll
     for (int f = 0; f < FS; f++) {
         const int b = chunk * FS + f;
 #pragma unroll
         for (int c = 0; c < CW; c++)
-            v[f * CW + c] = b < B ? gout[((int64_t)b * gs_b + n) * CW + c] : A(0);
+            v[f * CW + c] = b < B ? (double)gout[((int64_t)b * gs_b + n) * CW + c] * scale : 0.0;
     }
-    A *dst = gt + ((int64_t)chunk * N + n) * FS * CW;
+    double *dst = gt + ((int64_t)chunk * N + n) * FS * CW;
 #pragma unroll
     for (int i = 0; i < FS * CW; i++) dst[i] = v[i];
 }
@@ -502,28 +518,24 @@ __global__ void absmax_kernel(const T *__restrict__ a, int64_t n, unsigned *__re
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
 
-// Fixed-point accumulation: a contribution v becomes the 64-bit integer round-down of
-// v * 2^E * 2^32 (hi word: floor(v 2^E), lo word: the fraction times 2^32), so sums are
-// exact, order-independent integer additions (LDS integer atomics run ~40x faster than
-// LDS float atomics on gfx950).  E is chosen so every |v| 2^E < 2^30.
-template <class A>
-__device__ __forceinline__ void fx_add(unsigned long long *p, A v, A scale) {
-    const A y = v * scale;                       // exact: scale is a power of two
-    const A hi = floor(y);
-    const A fr = y - hi;                         // exact, in [0, 1)
-    const unsigned lo = (unsigned)(fr * (A)4294967296.0);
-    const unsigned hw = (unsigned)(int)hi;
-    atomicAdd(p, ((unsigned long long)hw << 32) | (unsigned long long)lo);
+// Fixed-point accumulation: the contribution w * g (g already scaled by 2^E) is rounded to
+// an integer by one FP64 FMA with the magic constant 1.5 * 2^52 (the integer then sits in the
+// low mantissa bits, exact for |w g 2^E| < 2^51) and added to the 64-bit LDS accumulator with
+// an integer atomic: sums are exact and order-independent (LDS integer atomics run ~40x faster
+// than LDS float atomics on gfx950).
+constexpr double kFxMagic = 6755399441055744.0;                 // 1.5 * 2^52
+constexpr unsigned long long kFxMagicBits = 0x4338000000000000ull;
+
+__device__ __forceinline__ void fx_add(unsigned long long *p, double w, double g_scaled) {
+    const double y = __builtin_fma(w, g_scaled, kFxMagic);
+    atomicAdd(p, (unsigned long long)__double_as_longlong(y) - kFxMagicBits);
 }
 
-// Exponent E with maxw * maxg * 2 * 2^E < 2^30 (the factor 2 bounds complex products).
-__device__ __forceinline__ int fx_exponent(const unsigned *maxes) {
-    const double bound = (double)__uint_as_float(maxes[0]) * (double)__uint_as_float(maxes[1]);
-    if (!(bound > 0.0)) return 0;
-    int e;
-    frexp(bound, &e);                            // bound < 2^e
-    int E = 28 - e;
-    return E < -120 ? -120 : (E > 120 ? 120 : E);
+// round(w1 g1 + round(w2 g2)): two products into one accumulator (complex parts)
+__device__ __forceinline__ void fx_add2(unsigned long long *p, double w1, double g1, double w2,
+                                        double g2) {
+    const double y = __builtin_fma(w1, g1, __builtin_fma(w2, g2, kFxMagic));
+    atomicAdd(p, (unsigned long long)__double_as_longlong(y) - kFxMagicBits);
 }
 
 // grid.x = C*T rows (m = row / T, s = row % T), grid.y = frame chunks of FS frames.
@@ -533,15 +545,15 @@ __device__ __forceinline__ int fx_exponent(const unsigned *maxes) {
 template <bool ACC64, bool CPLX, bool PRECISE, int FS>
 __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const void *__restrict__ gtv,
                                                         const unsigned *__restrict__ maxes,
-                                                        void *__restrict__ gxv, int64_t gx_b,
-                                                        int64_t gx_c, int64_t gx_t) {
+                                                        int log2cnt, void *__restrict__ gxv,
+                                                        int64_t gx_b, int64_t gx_c, int64_t gx_t) {
     using A = AccT<ACC64>;
     using U64 = unsigned long long;
     constexpr int CW = CPLX ? 2 : 1;
     constexpr int FSP = (FS > 1) ? (FS | 1) : 1;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     U64 *lds = reinterpret_cast<U64 *>(lds_raw);
-    const A *__restrict__ gt = static_cast<const A *>(gtv);
+    const double *__restrict__ gt = static_cast<const double *>(gtv);
     A *__restrict__ gx = static_cast<A *>(gxv);
 
     const int row = blockIdx.x;
@@ -552,8 +564,7 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
     const int nf = min(FS, g.B - b0);
     const int nlds = g.S * FSP * CW;
     for (int i = threadIdx.x; i < nlds; i += blockDim.x) lds[i] = 0ull;
-    const int E = fx_exponent(maxes);
-    const A scale = (A)ldexp(1.0, E);
+    const int E = fx_exponent(maxes, log2cnt);
     __syncthreads();
 
     const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
@@ -566,14 +577,14 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
     const int lane = threadIdx.x & 63;
     const int waves = blockDim.x >> 6;
     const int ntiles = tl.nta * tl.nth * tl.ntw;
-    const A *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
+    const double *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
     for (int tile = threadIdx.x >> 6; tile < ntiles; tile += waves) {
         bool valid;
         const int n = voxel_of(tl, tile, lane, valid);
         if (!valid) continue;
         // grad_out of this voxel first: its latency overlaps the geometry below
-        A gv[FS * CW];
-        const A *gp = gchunk + (int64_t)n * FS * CW;
+        double gv[FS * CW];
+        const double *gp = gchunk + (int64_t)n * FS * CW;
 #pragma unroll
         for (int i = 0; i < FS * CW; i++) gv[i] = gp[i];
         const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
@@ -589,35 +600,35 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
             const float w = g.weights[o];
             U64 *l0 = lds + (size_t)tp.k * FSP * CW;
             U64 *l1 = l0 + FSP * CW;
-            const A t = (A)tp.t;
+            const double t = (double)tp.t;
             if constexpr (!CPLX) {
-                const A w1 = (A)w * t, w0 = (A)w - w1;
+                const double w1 = (double)w * t, w0 = (double)w - w1;
 #pragma unroll
                 for (int f = 0; f < FS; f++) {
-                    fx_add<A>(l0 + f, w0 * gv[f], scale);
-                    fx_add<A>(l1 + f, w1 * gv[f], scale);
+                    fx_add(l0 + f, w0, gv[f]);
+                    fx_add(l1 + f, w1, gv[f]);
                 }
             } else {
                 float sn, cs;
                 phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
-                const A cr = (A)(w * cs), ci = (A)(w * sn);
-                const A omt = A(1) - t;
+                const double cr = (double)w * (double)cs, ci = (double)w * (double)sn;
+                const double omt = 1.0 - t;
+                // conj(coef) * g = (cr gr + ci gi) + i (cr gi - ci gr), split by (1-t) and t
+                const double a_r = cr * omt, a_i = ci * omt, b_r = cr * t, b_i = ci * t;
 #pragma unroll
                 for (int f = 0; f < FS; f++) {
-                    const A gr = gv[2 * f], gi = gv[2 * f + 1];
-                    // conj(coef) * g
-                    const A vr = cr * gr + ci * gi, vi = cr * gi - ci * gr;
-                    fx_add<A>(l0 + 2 * f, vr * omt, scale);
-                    fx_add<A>(l0 + 2 * f + 1, vi * omt, scale);
-                    fx_add<A>(l1 + 2 * f, vr * t, scale);
-                    fx_add<A>(l1 + 2 * f + 1, vi * t, scale);
+                    const double gr = gv[2 * f], gi = gv[2 * f + 1];
+                    fx_add2(l0 + 2 * f, a_r, gr, a_i, gi);
+                    fx_add2(l0 + 2 * f + 1, a_r, gi, -a_i, gr);
+                    fx_add2(l1 + 2 * f, b_r, gr, b_i, gi);
+                    fx_add2(l1 + 2 * f + 1, b_r, gi, -b_i, gr);
                 }
             }
         }
     }
     __syncthreads();
     // write back: consecutive threads -> consecutive samples
-    const double inv = ldexp(1.0, -E - 32);
+    const double inv = ldexp(1.0, -E);
     for (int f = 0; f < nf; f++) {
         A *dst = gx + ((int64_t)(b0 + f) * gx_b + (int64_t)m * gx_c + (int64_t)s * gx_t) * CW;
         for (int k = threadIdx.x; k < g.S; k += blockDim.x) {
@@ -628,6 +639,266 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// adjoint: x by bucketing voxels per sample (gather form of the scatter)
+// ---------------------------------------------------------------------------------------
+// One workgroup (1024 threads) owns one channel-data row (channel m, transmit s) for a chunk
+// of F frames.  Voxels are processed in chunks of nc: every contributing (voxel, slot) is
+// counted into its sample bucket k with an LDS integer atomic (whose return value is the
+// record's slot in the bucket), the bucket starts come from a block-wide exclusive scan, and
+// the records {voxel, (1-t) w, t w} are placed bucket-ordered in LDS.  Thread i owns samples
+// [KPT*i, KPT*i + KPT) for all F frames and sums, in FP32 registers, w0 * g into sample k and
+// w1 * g into sample k+1 for every record of its buckets: one FMA per contribution and no
+// atomics on the data.  The sample just past a thread's range is handed to the next thread
+// (shuffle within a wave, LDS across waves).  Summation order within a bucket follows the
+// atomic ranks, so results can differ from run to run in the last bits; the fixed-point
+// kernel above is the bitwise-deterministic alternative.
+template <bool CPLX> struct BwdRec;
+template <> struct BwdRec<false> { int n; float w0, w1; };
+template <> struct BwdRec<true> { int n; float ar, ai, br, bi; };
+
+template <int NT>
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned *wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        unsigned t = lane < NT / 64 ? wsum[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < NT / 64) wsum[lane] = t;   // inclusive wave totals
+    }
+    __syncthreads();
+    const unsigned before = wave > 0 ? wsum[wave - 1] : 0u;
+    return before + x - v;
+}
+
+constexpr int kSortNT = 1024;                   // threads per workgroup
+template <bool CPLX> constexpr int sort_vpt() { return 2; }   // records per thread per chunk
+
+// Records of bucket k0+A add (1-t) w g to acc[A] and t w g to acc[A+1]; A is a template
+// parameter so every accumulator index is static (registers, not scratch).
+template <int A, int KPT, int F, bool CPLX>
+__device__ __forceinline__ void sorted_bucket_pass(float (&acc)[KPT + 1][F][CPLX ? 2 : 1],
+                                                   const BwdRec<CPLX> *rec, const unsigned *cnt,
+                                                   const float *gchunk, int k0, int S) {
+    constexpr int CW = CPLX ? 2 : 1;
+    const int k = k0 + A;
+    if (k >= S) return;
+    const unsigned r0 = cnt[k], r1 = cnt[k + 1];
+    for (unsigned r = r0; r < r1; r++) {
+        const BwdRec<CPLX> rc_ = rec[r];
+        const float *gp = gchunk + (int64_t)rc_.n * F * CW;
+        if constexpr (!CPLX) {
+            constexpr int H = F > 8 ? 8 : F;      // frames per load group
+#pragma unroll
+            for (int h = 0; h < F; h += H) {
+                float gv[H];
+#pragma unroll
+                for (int i = 0; i < H; i++) gv[i] = gp[h + i];
+#pragma unroll
+                for (int i = 0; i < H; i++) {
+                    acc[A][h + i][0] = __builtin_fmaf(rc_.w0, gv[i], acc[A][h + i][0]);
+                    acc[A + 1][h + i][0] = __builtin_fmaf(rc_.w1, gv[i], acc[A + 1][h + i][0]);
+                }
+            }
+        } else {
+            float gv[F * CW];
+#pragma unroll
+            for (int i = 0; i < F * CW; i++) gv[i] = gp[i];
+#pragma unroll
+            for (int f = 0; f < F; f++) {
+                const float gr = gv[2 * f], gi = gv[2 * f + 1];
+                // conj(a) g and conj(b) g
+                acc[A][f][0] += rc_.ar * gr + rc_.ai * gi;
+                acc[A][f][1] += rc_.ar * gi - rc_.ai * gr;
+                acc[A + 1][f][0] += rc_.br * gr + rc_.bi * gi;
+                acc[A + 1][f][1] += rc_.br * gi - rc_.bi * gr;
+            }
+        }
+    }
+    if constexpr (A + 1 < KPT) sorted_bucket_pass<A + 1, KPT, F, CPLX>(acc, rec, cnt, gchunk, k0, S);
+}
+
+template <bool CPLX, bool PRECISE, int KPT, int F>
+__global__ __launch_bounds__(1024) void das_bwd_x_sorted_kernel(
+    Geo g, const float *__restrict__ gtf, float *__restrict__ gx, int64_t gx_b, int64_t gx_c,
+    int64_t gx_t, int nc) {
+    constexpr int NT = kSortNT;
+    constexpr int CW = CPLX ? 2 : 1;
+    constexpr int VPT = sort_vpt<CPLX>();
+    using Rec = BwdRec<CPLX>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    unsigned *cnt = reinterpret_cast<unsigned *>(lds_raw);            // S+1 bucket starts
+    unsigned *wsum = cnt + (g.S + 1);                                  // NT/64 scan partials
+    float *xwave = reinterpret_cast<float *>(wsum + NT / 64);          // NT/64 * F * CW carries
+    Rec *rec = reinterpret_cast<Rec *>(xwave + (NT / 64) * F * CW);
+
+    const int row = blockIdx.x;
+    const int m = row / g.T;
+    const int s = row - m * g.T;
+    const int chunk = blockIdx.y;
+    const int b0 = chunk * F;
+    const int nf = min(F, g.B - b0);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int k0 = tid * KPT;
+
+    float acc[KPT + 1][F][CW];
+#pragma unroll
+    for (int a = 0; a <= KPT; a++)
+#pragma unroll
+        for (int f = 0; f < F; f++)
+#pragma unroll
+            for (int c = 0; c < CW; c++) acc[a][f][c] = 0.0f;
+
+    const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
+    float sdx = 0, sdy = 0, sdz = 0, sdw = 0;
+    if (g.srcdir) {
+        sdx = g.srcdir[4 * m]; sdy = g.srcdir[4 * m + 1]; sdz = g.srcdir[4 * m + 2];
+        sdw = g.srcdir[4 * m + 3];
+    }
+    const bool unit = g.unit_scale != 0;
+    const int kslots = g.sparse ? g.K : 1;
+    const float *gchunk = gtf + (int64_t)chunk * g.N * F * CW;
+
+    for (int v0 = 0; v0 < g.N; v0 += nc) {
+        for (int i = tid; i <= g.S; i += NT) cnt[i] = 0u;
+        __syncthreads();
+        // phase A: taps of this chunk's voxels -> bucket counts (+ rank within the bucket)
+        int rk[VPT], rn[VPT];
+        unsigned rr[VPT];
+        float ra[VPT], rb[VPT], rc[VPT], rd[VPT];
+#pragma unroll
+        for (int q = 0; q < VPT; q++) {
+            rk[q] = -1;
+            const int li = q * NT + tid;            // candidate (voxel, slot) in the chunk
+            const int vi = li / kslots;
+            const int j = g.sparse ? li - vi * kslots : s;
+            const int n = v0 + vi;
+            if (vi >= nc || n >= g.N) continue;
+            const int64_t o = (int64_t)j * g.N + n;
+            if (g.sparse && g.sparse[o] != s) continue;
+            const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
+            const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
+            if (g.srcdir && !((R.dx * sdx + R.dy * sdy + R.dz * sdz) * R.rinv > sdw)) continue;
+            const Tap tp = make_tap<PRECISE>(g.offsets[o], R, g.scale, unit);
+            if ((uint32_t)tp.k >= (uint32_t)(g.S - 1)) continue;
+            const float w = g.weights[o];
+            rk[q] = tp.k;
+            rn[q] = n;
+            if constexpr (!CPLX) {
+                rb[q] = w * tp.t;
+                ra[q] = w - rb[q];
+            } else {
+                float sn, cs;
+                phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
+                const float omt = 1.0f - tp.t;
+                ra[q] = w * cs * omt; rb[q] = w * sn * omt;
+                rc[q] = w * cs * tp.t; rd[q] = w * sn * tp.t;
+            }
+            rr[q] = atomicAdd(&cnt[tp.k], 1u);
+        }
+        __syncthreads();
+        // exclusive scan of the S+1 counts, each thread over a contiguous run of <= 8
+        {
+            constexpr int maxrun = 8;
+            const int run = (g.S + 1 + NT - 1) / NT;
+            unsigned loc[maxrun];
+            unsigned tot = 0;
+#pragma unroll
+            for (int r = 0; r < maxrun; r++) {
+                const int i = tid * run + r;
+                loc[r] = (r < run && i <= g.S) ? cnt[i] : 0u;
+                tot += loc[r];
+            }
+            const unsigned base = block_exclusive_scan<NT>(tot, wsum);
+            unsigned acc_s = base;
+#pragma unroll
+            for (int r = 0; r < maxrun; r++) {
+                const int i = tid * run + r;
+                if (r < run && i <= g.S) cnt[i] = acc_s;
+                acc_s += loc[r];
+            }
+        }
+        __syncthreads();
+        // placement
+#pragma unroll
+        for (int q = 0; q < VPT; q++) {
+            if (rk[q] < 0) continue;
+            Rec &r = rec[cnt[rk[q]] + rr[q]];
+            r.n = rn[q];
+            if constexpr (!CPLX) {
+                r.w0 = ra[q]; r.w1 = rb[q];
+            } else {
+                r.ar = ra[q]; r.ai = rb[q]; r.br = rc[q]; r.bi = rd[q];
+            }
+        }
+        __syncthreads();
+        // phase D: accumulate the records of the owned buckets
+        sorted_bucket_pass<0, KPT, F, CPLX>(acc, rec, cnt, gchunk, k0, g.S);
+        __syncthreads();
+    }
+    // hand the sample past each thread's range to the next thread, then write
+#pragma unroll
+    for (int f = 0; f < F; f++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            const float up = __shfl_up(acc[KPT][f][c], 1, 64);
+            if (lane == 63) xwave[(wave * F + f) * CW + c] = acc[KPT][f][c];
+            if (lane > 0) acc[0][f][c] += up;
+        }
+    __syncthreads();
+    if (lane == 0 && wave > 0) {
+#pragma unroll
+        for (int f = 0; f < F; f++)
+#pragma unroll
+            for (int c = 0; c < CW; c++) acc[0][f][c] += xwave[((wave - 1) * F + f) * CW + c];
+    }
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+        if (f >= nf) break;
+        float *dst = gx + ((int64_t)(b0 + f) * gx_b + (int64_t)m * gx_c + (int64_t)s * gx_t) * CW;
+#pragma unroll
+        for (int a = 0; a < KPT; a++) {
+            const int k = k0 + a;
+            if (k >= g.S) break;
+#pragma unroll
+            for (int c = 0; c < CW; c++) dst[(size_t)k * CW + c] = acc[a][f][c];
+        }
+    }
+}
+
+// grad_out transposed to frame-inner float chunks: gt[chunk][n][f] (zero past B).
+template <bool CPLX, int F>
+__global__ void transpose_gout_f_kernel(const float *__restrict__ gout, int64_t gs_b, int B, int N,
+                                        float *__restrict__ gt) {
+    constexpr int CW = CPLX ? 2 : 1;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (n >= N) return;
+    float v[F * CW];
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+        const int b = chunk * F + f;
+#pragma unroll
+        for (int c = 0; c < CW; c++)
+            v[f * CW + c] = b < B ? gout[((int64_t)b * gs_b + n) * CW + c] : 0.0f;
+    }
+    float *dst = gt + ((int64_t)chunk * N + n) * F * CW;
+#pragma unroll
+    for (int i = 0; i < F * CW; i++) dst[i] = v[i];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -777,6 +1048,11 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Tiling &
     // workspace: [maxes: 2 words, padded to 256 B][transposed grad_out]
     unsigned *maxes = static_cast<unsigned *>(gt);
     void *gtr = static_cast<char *>(gt) + 256;
+    // contributions per LDS sample at most N (dense) or N*K (sparse), doubled for complex
+    const double cnt = (double)g.N * (g.sparse ? g.K : 1) * (CPLX ? 2 : 1);
+    int log2cnt = 0;
+    while ((double)(1ll << log2cnt) < cnt && log2cnt < 62) log2cnt++;
+
     {
     ProfScope ps(st, "absmax");
     dev::fill_zero_kernel<<<1, 64, 0, st>>>(reinterpret_cast<float *>(maxes), 2);
@@ -787,15 +1063,15 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Tiling &
     {
     ProfScope pt(st, "transpose_gout");
     dev::transpose_gout_kernel<ACC64, CPLX, FS><<<dim3((g.N + 255) / 256, nchunks), 256, 0, st>>>(
-        gout, gs_b, g.B, g.N, gtr);
+        gout, gs_b, g.B, g.N, maxes, log2cnt, static_cast<double *>(gtr));
     }
     dim3 grid(g.C * g.T, nchunks);
     const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
-    hipFuncSetAttribute((const void *)dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS>,
+    (void)hipFuncSetAttribute((const void *)dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     ProfScope ps(st, "das_bwd_x");
     dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, 512, lds, st>>>(
-        g, tl, gtr, maxes, gx, gx_b, gx_c, gx_t);
+        g, tl, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t);
     return hipGetLastError();
 }
 
@@ -893,6 +1169,73 @@ hipError_t launch_relayout_t(hipStream_t st, const DasProblem &p, const void *x,
     }
 }
 
+// ---- grad_x by bucketing ----
+// Frames per workgroup for the bucketed grad_x kernel, limited by the accumulator registers
+// ((KPT+1) * F * CW floats per thread).
+int sorted_frames(int KPT, bool cplx, int B) {
+    // largest frame count whose accumulators stay in registers (no scratch) at 1024 threads
+    int fmax = cplx ? (KPT >= 4 ? 4 : 8) : (KPT >= 2 ? 8 : 16);
+    int want = B >= 16 ? 16 : B >= 8 ? 8 : 4;
+    return want < fmax ? want : fmax;
+}
+
+size_t sorted_lds_bytes(int S, int F, bool cplx) {
+    const int CW = cplx ? 2 : 1;
+    const size_t rec = cplx ? 20 : 12;
+    return (size_t)(S + 1) * 4 + (size_t)(dev::kSortNT / 64) * 4 +
+           (size_t)(dev::kSortNT / 64) * F * CW * 4 +
+           (size_t)(cplx ? dev::sort_vpt<true>() : dev::sort_vpt<false>()) * dev::kSortNT * rec;
+}
+
+template <bool CPLX, bool PRECISE, int KPT, int F>
+hipError_t launch_bwdx_sorted_kf(hipStream_t st, const dev::Geo &g, const void *gout, int64_t gs_b,
+                                 void *ws, void *gx) {
+    if constexpr ((CPLX && F * (KPT + 1) > 24) || (!CPLX && F * (KPT + 1) > 48)) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr int CW = CPLX ? 2 : 1;
+        const int nchunks = (g.B + F - 1) / F;
+        float *gt = static_cast<float *>(ws);
+        {
+            ProfScope pt(st, "transpose_gout");
+            dev::transpose_gout_f_kernel<CPLX, F><<<dim3((g.N + 255) / 256, nchunks), 256, 0, st>>>(
+                static_cast<const float *>(gout), gs_b, g.B, g.N, gt);
+        }
+        const int kslots = g.sparse ? g.K : 1;
+        const int nc = std::max(1, dev::sort_vpt<CPLX>() * dev::kSortNT / kslots);
+        const size_t lds = sorted_lds_bytes(g.S, F, CPLX);
+        dim3 grid(g.C * g.T, nchunks);
+        const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
+        (void)hipFuncSetAttribute((const void *)dev::das_bwd_x_sorted_kernel<CPLX, PRECISE, KPT, F>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        ProfScope ps(st, "das_bwd_x");
+        dev::das_bwd_x_sorted_kernel<CPLX, PRECISE, KPT, F><<<grid, dev::kSortNT, lds, st>>>(
+            g, gt, static_cast<float *>(gx), gx_b, gx_c, gx_t, nc);
+        (void)CW;
+        return hipGetLastError();
+    }
+}
+
+template <bool CPLX, bool PRECISE, int KPT>
+hipError_t launch_bwdx_sorted_k(hipStream_t st, const dev::Geo &g, const void *gout, int64_t gs_b,
+                                void *ws, void *gx, int F) {
+    switch (F) {
+        case 16: return launch_bwdx_sorted_kf<CPLX, PRECISE, KPT, 16>(st, g, gout, gs_b, ws, gx);
+        case 8: return launch_bwdx_sorted_kf<CPLX, PRECISE, KPT, 8>(st, g, gout, gs_b, ws, gx);
+        default: return launch_bwdx_sorted_kf<CPLX, PRECISE, KPT, 4>(st, g, gout, gs_b, ws, gx);
+    }
+}
+
+template <bool CPLX, bool PRECISE>
+hipError_t launch_bwdx_sorted_t(hipStream_t st, const dev::Geo &g, const void *gout, int64_t gs_b,
+                                void *ws, void *gx, int KPT, int F) {
+    switch (KPT) {
+        case 1: return launch_bwdx_sorted_k<CPLX, PRECISE, 1>(st, g, gout, gs_b, ws, gx, F);
+        case 2: return launch_bwdx_sorted_k<CPLX, PRECISE, 2>(st, g, gout, gs_b, ws, gx, F);
+        default: return launch_bwdx_sorted_k<CPLX, PRECISE, 4>(st, g, gout, gs_b, ws, gx, F);
+    }
+}
+
 }  // namespace
 
 ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
@@ -971,10 +1314,26 @@ int bwdx_frames(const DasProblem &p) {
     return 0;
 }
 
+// The bucketed grad_x kernel (FFDAS_BWDX=sorted; float data, S <= 4096) is selectable; the
+// default is the fixed-point LDS-atomic kernel, which is faster on the measured workloads and
+// bitwise reproducible.
+bool use_sorted_bwdx(const DasProblem &p) {
+    if (p.acc64 || p.S > 4096) return false;
+    const char *env = std::getenv("FFDAS_BWDX");
+    return env && std::strcmp(env, "sorted") == 0;
+}
+
+int sorted_kpt(const DasProblem &p) { return p.S <= 1024 ? 1 : (p.S <= 2048 ? 2 : 4); }
+
 size_t backward_workspace(const DasProblem &p) {
+    if (use_sorted_bwdx(p)) {
+        const int F = sorted_frames(sorted_kpt(p), p.cplx, p.B);
+        const size_t chunks = (size_t)(p.B + F - 1) / F;
+        return chunks * (size_t)F * (size_t)p.N * (p.cplx ? 2 : 1) * sizeof(float);
+    }
     const int fs = bwdx_frames(p);
     if (fs <= 0) return 0;
-    const size_t esz = (p.acc64 ? 8 : 4) * (p.cplx ? 2 : 1);
+    const size_t esz = 8 * (p.cplx ? 2 : 1);   // pre-scaled double grad_out
     const size_t nchunks = (size_t)(p.B + fs - 1) / fs;
     return 256 + nchunks * (size_t)fs * (size_t)p.N * esz;
 }
@@ -1024,7 +1383,17 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
 #undef BWDPFI
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }
-    if (gx) {
+    if (gx && use_sorted_bwdx(p)) {
+        const int KPT = sorted_kpt(p);
+        const int F = sorted_frames(KPT, p.cplx, p.B);
+        if (p.cplx)
+            e = precise ? launch_bwdx_sorted_t<true, true>(st, g, gout, gs_b, gt, gx, KPT, F)
+                        : launch_bwdx_sorted_t<true, false>(st, g, gout, gs_b, gt, gx, KPT, F);
+        else
+            e = precise ? launch_bwdx_sorted_t<false, true>(st, g, gout, gs_b, gt, gx, KPT, F)
+                        : launch_bwdx_sorted_t<false, false>(st, g, gout, gs_b, gt, gx, KPT, F);
+        if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+    } else if (gx) {
         const int FS = bwdx_frames(p);
         if (FS <= 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
         if (p.acc64) {

@@ -109,15 +109,39 @@ def test_batched_adjoint(dev, B, storage):
     assert abs(float(gs) - ref["grad_scale"]) <= max(tol, 1e-4) * max(1.0, abs(ref["grad_scale"]))
 
 
-def test_grad_x_is_deterministic(dev):
-    """Fixed-point LDS accumulation: repeated adjoints are bitwise identical."""
+def test_grad_x_fixed_point_path_is_deterministic(dev, monkeypatch):
+    """The default grad_x kernel accumulates in 64-bit fixed point: repeated adjoints are
+    bitwise identical.  FFDAS_BWDX=sorted selects the bucketed FP32 kernel, which agrees to
+    float rounding."""
     import torch
     import ffdas_amd
     w = W.config1()
     d = to_dev(w, dev)
     g = torch.randn((1,) + tuple(w.dstpos.shape[:-1]), device=dev)
-    a = ffdas_amd.das_backward(d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"], g,
-                               need_offsets=False, need_weights=False, need_scale=False)[0]
-    b = ffdas_amd.das_backward(d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"], g,
-                               need_offsets=False, need_weights=False, need_scale=False)[0]
+    args = (d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"], g)
+    kw = dict(need_offsets=False, need_weights=False, need_scale=False)
+    a = ffdas_amd.das_backward(*args, **kw)[0]
+    b = ffdas_amd.das_backward(*args, **kw)[0]
     assert torch.equal(a, b)
+    monkeypatch.setenv("FFDAS_BWDX", "sorted")
+    sorted_gx = ffdas_amd.das_backward(*args, **kw)[0]
+    assert rel_l2(sorted_gx.cpu().numpy(), a.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("S", [700, 1500, 4000, 5000])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_grad_x_row_lengths(dev, S, cplx):
+    """Rows of several lengths (LDS row sizes, frames per workgroup) for the grad_x kernel."""
+    import torch
+    import ffdas_amd
+    w = W.small_random(seed=S, B=3, C=3, T=2, S=S, N=(12, 10), complex_iq=cplx)
+    w.offsets[:] = np.random.default_rng(S).uniform(-10, S * 0.9, w.offsets.shape).astype(np.float32)
+    rng = np.random.default_rng(9)
+    gout = rng.standard_normal((3,) + w.dstpos.shape[:-1])
+    if cplx:
+        gout = gout + 1j * rng.standard_normal(gout.shape)
+    gout = gout.astype(np.complex64 if cplx else np.float32)
+    ref = oracle.das_adjoint(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             wavenum=w.wavenum)
+    gx, _, _, _ = _grads(w, dev, gout, need_offsets=False, need_weights=False, need_scale=False)
+    assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
