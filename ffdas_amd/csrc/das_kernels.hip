@@ -15,6 +15,7 @@
 // Adjoint w.r.t. x: a workgroup owns one channel-data row (channel m, transmit s) for a
 // chunk of FS frames, scatters every voxel's two interpolation weights into that row in
 // LDS with LDS float atomics, then writes the row once (no global atomics).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -115,8 +116,8 @@ struct FrameInnerRows {
 // forward
 // ---------------------------------------------------------------------------------------
 template <class Rows, Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
-__device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n,
-                                             AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1]) {
+__device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n, int m0,
+                                             int mstep, AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1]) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
     const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
@@ -143,7 +144,7 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
                 sq[j] = -1;
             }
         }
-        for (int m = 0; m < g.C; m++) {
+        for (int m = m0; m < g.C; m += mstep) {
             const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
             const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
             bool dok = true;
@@ -228,26 +229,67 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
     const int b0 = blockIdx.y * FT;
     const NaturalRows<ST, FT, CW> xr(xv, xs_b, xs_c, xs_t, b0, g.B);
     AccT<ACC64> acc[FT][CW];
-    das_fwd_body<NaturalRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, acc);
+    das_fwd_body<NaturalRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, 0, 1, acc);
     store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, b0, g.B, beta_re, beta_im);
 }
 
-// frame-inner layout (grid.y = frame chunk)
+// frame-inner layout.  A workgroup of G waves shares one voxel tile (blockIdx.x); wave w
+// sums channels w, w+G, ... and the G partial sums are combined through LDS, so a launch
+// has G times more waves in flight than tiles (latency hiding for the gathers) without
+// repeating any delay arithmetic.  grid.y = frame chunk.
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
-__global__ __launch_bounds__(256) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
+__global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
                                                          void *__restrict__ outv, int64_t os_b,
                                                          double beta_re, double beta_im) {
+    using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
+    extern __shared__ unsigned char fwd_lds[];
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wave = threadIdx.x >> 6;
+    const int G = blockDim.x >> 6;
     bool valid;
-    const int n = voxel_of(tl, tile, lane, valid);
-    if (!valid) return;
+    int n = voxel_of(tl, blockIdx.x, lane, valid);
+    if (!valid) n = 0;          // computed, never stored: keeps every lane at the barrier
     const int chunk = blockIdx.y;
     const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
-    AccT<ACC64> acc[FT][CW];
-    das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, acc);
-    store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, chunk * FT, g.B, beta_re, beta_im);
+    A acc[FT][CW];
+    das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, wave, G, acc);
+    if (G == 1) {
+        if (valid) store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, chunk * FT, g.B, beta_re, beta_im);
+        return;
+    }
+    // red[w][i][lane], i = f * CW + c
+    A *red = reinterpret_cast<A *>(fwd_lds);
+#pragma unroll
+    for (int f = 0; f < FT; f++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) red[(wave * FT * CW + f * CW + c) * 64 + lane] = acc[f][c];
+    __syncthreads();
+    if (!valid) return;
+    A *out = static_cast<A *>(outv);
+    const bool zero_beta = beta_re == 0.0 && beta_im == 0.0;
+    for (int f = wave; f < FT; f += G) {
+        const int b = chunk * FT + f;
+        if (b >= g.B) break;
+        A v[CW];
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            A sum = A(0);
+            for (int w = 0; w < G; w++) sum += red[(w * FT * CW + f * CW + c) * 64 + lane];
+            v[c] = sum;
+        }
+        A *o = out + ((int64_t)b * os_b + n) * CW;
+        if constexpr (!CPLX) {
+            o[0] = zero_beta ? v[0] : (A)beta_re * o[0] + v[0];
+        } else if (zero_beta) {
+            o[0] = v[0];
+            o[1] = v[1];
+        } else {
+            const A orr = o[0], oi = o[1];
+            o[0] = (A)beta_re * orr - (A)beta_im * oi + v[0];
+            o[1] = (A)beta_re * oi + (A)beta_im * orr + v[1];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -263,7 +305,9 @@ __device__ __forceinline__ AccT<ACC64> wave_sum(AccT<ACC64> v) {
 template <class Rows, Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
 __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Rows &xr, int n,
                                                           int b0, const void *goutv, int64_t gs_b,
-                                                          float *gofs, float *gwt, int accumulate) {
+                                                          float *gofs, float *gwt, int accumulate,
+                                                          int m0, int mstep, bool valid,
+                                                          AccT<ACC64> *red) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
     const A *__restrict__ gout = static_cast<const A *>(goutv);
@@ -299,7 +343,7 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
                 sq[j] = -1;
             }
         }
-        for (int m = 0; m < g.C; m++) {
+        for (int m = m0; m < g.C; m += mstep) {
             const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
             const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
             bool dok = true;
@@ -355,10 +399,33 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
                 }
             }
         }
+        if (mstep > 1) {
+            // combine the channel groups' partial sums: red[w][i][lane], i = 2 j + {0: tau, 1: w}
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+            for (int j = 0; j < KC; j++) {
+                red[(wave * 2 * KC + 2 * j) * 64 + lane] = d_tau[j];
+                red[(wave * 2 * KC + 2 * j + 1) * 64 + lane] = d_w[j];
+            }
+            __syncthreads();
+            for (int i = wave; i < 2 * KC; i += mstep) {
+                const int jj = kc + (i >> 1);
+                A sum = A(0);
+                for (int w = 0; w < mstep; w++) sum += red[(w * 2 * KC + i) * 64 + lane];
+                float *dst = (i & 1) ? gwt : gofs;
+                if (!valid || jj >= g.K || !dst) continue;
+                const int64_t o = (int64_t)jj * g.N + n;
+                const float v = (i & 1) ? (float)sum : (float)(sum * (A)g.scale);
+                if (accumulate) atomicAdd(dst + o, v);
+                else dst[o] = v;
+            }
+            __syncthreads();
+            continue;
+        }
 #pragma unroll
         for (int j = 0; j < KC; j++) {
             const int jj = kc + j;
-            if (jj >= g.K) break;
+            if (jj >= g.K || !valid) break;
             const int64_t o = (int64_t)jj * g.N + n;
             const float vo = (float)(d_tau[j] * (A)g.scale);
             const float vw = (float)d_w[j];
@@ -404,27 +471,30 @@ __global__ __launch_bounds__(256) void das_bwd_param_kernel(
         const int b0 = blockIdx.y * FT;
         const NaturalRows<ST, FT, CW> xr(xv, xs_b, xs_c, xs_t, b0, g.B);
         part = das_bwd_param_body<NaturalRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
-            g, xr, n, b0, goutv, gs_b, gofs, gwt, accumulate);
+            g, xr, n, b0, goutv, gs_b, gofs, gwt, accumulate, 0, 1, true, nullptr);
     }
     if (gscale) block_scale_sum<ACC64>(part, gscale);
 }
 
+// Frame-inner layout, G waves per voxel tile splitting the channels (as the forward).
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
-__global__ __launch_bounds__(256) void das_bwd_param_fi_kernel(
+__global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     Geo g, Tiling tl, const void *__restrict__ xv, const void *__restrict__ goutv, int64_t gs_b,
     float *__restrict__ gofs, float *__restrict__ gwt, float *__restrict__ gscale, int accumulate) {
     constexpr int CW = CPLX ? 2 : 1;
+    extern __shared__ unsigned char param_lds[];
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wave = threadIdx.x >> 6;
+    const int G = blockDim.x >> 6;
     bool valid;
-    const int n = voxel_of(tl, tile, lane, valid);
-    AccT<ACC64> part = 0;
-    if (valid) {
-        const int chunk = blockIdx.y;
-        const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
-        part = das_bwd_param_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
-            g, xr, n, chunk * FT, goutv, gs_b, gofs, gwt, accumulate);
-    }
+    int n = voxel_of(tl, blockIdx.x, lane, valid);
+    if (!valid) n = 0;          // computed, never stored: keeps every lane at the barriers
+    const int chunk = blockIdx.y;
+    const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
+    AccT<ACC64> part = das_bwd_param_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
+        g, xr, n, chunk * FT, goutv, gs_b, gofs, gwt, accumulate, wave, G, valid,
+        reinterpret_cast<AccT<ACC64> *>(param_lds));
+    if (!valid) part = 0;
     if (gscale) block_scale_sum<ACC64>(part, gscale);
 }
 
@@ -973,6 +1043,20 @@ dev::Tiling make_tiling(const std::vector<int64_t> &spatial) {
     return t;
 }
 
+// Waves per voxel tile for the frame-inner gathers: enough to put about 32 waves on each
+// CU (8 per SIMD), at most 8 and at most one per channel.  FFDAS_CGROUPS overrides.
+int channel_groups(const dev::Geo &g, int64_t tiles) {
+    int G = 1;
+    if (const char *env = std::getenv("FFDAS_CGROUPS")) {
+        G = std::atoi(env);
+    } else {
+        while (G < 8 && tiles * G < 8192) G *= 2;
+    }
+    G = std::max(1, std::min(G, 8));
+    while (G > 1 && G > g.C) G >>= 1;
+    return G;
+}
+
 dev::Geo make_geo(const DasGeomArgs &a, const DasProblem &p) {
     dev::Geo g{};
     g.srcpos = a.srcpos; g.srcdir = a.srcdir; g.dstpos = a.dstpos;
@@ -1100,9 +1184,14 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
     } else {
         constexpr bool PRECISE = (ST == Storage::F32);
         const int ntiles = tl.nta * tl.nth * tl.ntw;
-        dim3 grid((ntiles + 3) / 4, (g.B + F - 1) / F);
+        dim3 grid(ntiles, (g.B + F - 1) / F);
+        const int G = channel_groups(g, (int64_t)grid.x * grid.y);
+        const size_t lds = G > 1 ? (size_t)G * F * (CPLX ? 2 : 1) * 64 * sizeof(float) : 0;
+        if (lds > 0)
+            (void)hipFuncSetAttribute((const void *)dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         ProfScope ps(st, "das_fwd_fi");
-        dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 256, 0, st>>>(
+        dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 64 * G, lds, st>>>(
             g, tl, x, out, os_b, br, bi);
         return hipGetLastError();
     }
@@ -1128,9 +1217,14 @@ hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling
     } else {
         constexpr bool PRECISE = (ST == Storage::F32);
         const int ntiles = tl.nta * tl.nth * tl.ntw;
-        dim3 grid((ntiles + 3) / 4, (g.B + F - 1) / F);
+        dim3 grid(ntiles, (g.B + F - 1) / F);
+        const int G = channel_groups(g, (int64_t)grid.x * grid.y);
+        const size_t lds = G > 1 ? (size_t)G * 2 * kKC * 64 * sizeof(float) : 0;
+        if (lds > 0)
+            (void)hipFuncSetAttribute((const void *)dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         ProfScope ps(st, "das_bwd_param_fi");
-        dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 256, 0, st>>>(
+        dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 64 * G, lds, st>>>(
             g, tl, x, gout, gs_b, gofs, gwt, gscale, grid.y > 1 ? 1 : 0);
         return hipGetLastError();
     }
