@@ -10,6 +10,11 @@ Metric: Gvoxel*ch*tx/s, one unit U = one (voxel, channel, transmit, frame) inter
 counted once per forward+adjoint step.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--storage fp16|bf16|fp32]
+  python bench.py --profile-only N   # N steps + a 256 MiB copy, for rocprofv3 --pmc runs
+
+roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
+profiles/das_config4_<storage>_traffic.json, written by ffdas_amd.utils.pmc from two
+rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --profile-only`.
 """
 from __future__ import annotations
 
@@ -42,6 +47,8 @@ def parse():
                     help="voxel rows (of 256) in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--profile-only", type=int, default=0, metavar="N",
+                    help="run N steps and a 256 MiB calibration copy, print no JSON")
     return ap.parse_args()
 
 
@@ -106,14 +113,46 @@ def kernel_times(p, x, steps):
     return {k: v["avg_ms"] for k, v in profiling.kernel_stats().items()}
 
 
-def accuracy(p, storage):
-    """rel-L2 of the forward output against the FP64 oracle on frame 0."""
+def accuracy(p):
+    """rel-L2 against the FP64 oracle on frame 0: the forward output and the adjoint's
+    grad_x for the loss gradient of that frame."""
     import oracle
     import ffdas_amd
     w = p.w
-    out = ffdas_amd.das(p.x[:1], p.srcpos, p.dstpos, p.offsets, p.weights).cpu().numpy()
+    x0 = p.x[:1].contiguous()
+    out = ffdas_amd.das(x0, p.srcpos, p.dstpos, p.offsets, p.weights)
     ref = oracle.das(w.x[:1], w.srcpos, w.dstpos, w.offsets, w.weights, bits=64)
-    return oracle.rel_l2(out, ref)
+    g = 2.0 * (out - p.target[:1])
+    gx = ffdas_amd.das_backward(x0, p.srcpos, p.dstpos, p.offsets, p.weights, g,
+                                need_offsets=False, need_weights=False, need_scale=False)[0]
+    gref = oracle.das_adjoint(w.x[:1], w.srcpos, w.dstpos, w.offsets, w.weights,
+                              g.cpu().numpy(), bits=64)["grad_x"]
+    return (oracle.rel_l2(out.cpu().numpy(), ref),
+            oracle.rel_l2(gx.float().cpu().numpy(), gref))
+
+
+def traffic_for(storage, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"das_config4_{storage}_traffic.json")
+    try:
+        with open(path) as fh:
+            k = json.load(fh)["kernels"].get(kernel + "_kernel")
+        return None if k is None else round(k["traffic_bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def profile_only(p, x, n):
+    """A few steps and a 256 MiB device copy (the FETCH/WRITE_SIZE calibration)."""
+    import torch
+    for _ in range(n):
+        step(p, x)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=p.dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"profile-only: {n} steps and the calibration copy done")
 
 
 def cpu_baseline(w, rows):
@@ -160,6 +199,9 @@ def main():
         f"{w.dstpos.shape[:-1]}, U/step {w.units:.4g}")
     p = Problem(w, dev, a.storage)
     U = w.units
+    if a.profile_only:
+        profile_only(p, p.x, a.profile_only)
+        return
 
     ms, wall = time_steps(lambda: step(p, p.x), a.steps, a.warmup)
     value = U / (ms * 1e-3) / 1e9
@@ -180,18 +222,19 @@ def main():
     # algorithmic bytes per unit: the two gathered samples (forward, parameter adjoint) or
     # the two FP32 scatter-adds (grad_x adjoint)
     bytes_per_u = {"das_fwd_fi": 2 * esz, "das_fwd": 2 * esz, "das_bwd_param_fi": 2 * esz,
-                   "das_bwd_param": 2 * esz, "das_bwd_x": 2 * 4}
+                   "das_bwd_param": 2 * esz, "das_bwd_x": 2 * 4, "das_bwd_x_plain": 2 * 4}
     dom = max((k for k in kt if k in bytes_per_u), key=kt.get)
     achieved = U * bytes_per_u[dom] / (kt[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_unit": bytes_per_u[dom], "units_per_launch": U,
+            "traffic": traffic_for(a.storage, dom), "bytes_per_unit": bytes_per_u[dom],
+            "units_per_launch": U,
             "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
 
-    acc = None
+    acc = (None, None)
     if not a.no_accuracy:
-        acc = accuracy(p, a.storage)
-        log(f"rel-L2 vs FP64 oracle (frame 0): {acc:.3e}")
+        acc = accuracy(p)
+        log(f"rel-L2 vs FP64 oracle (frame 0): out {acc[0]:.3e}, grad_x {acc[1]:.3e}")
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(w, a.cpu_sample_rows)
@@ -207,8 +250,10 @@ def main():
                    "storage": a.storage, "units_per_step": U,
                    "fp32_plain_value": round(v32, 2),
                    "fp32_tuned_value": round(v32o, 2),
-                   "ratio_vs_fp32": round(value / v32, 3),
-                   "rel_l2_vs_fp64": acc, "wall_s": round(wall, 3)},
+                   "ratio_vs_fp32_plain": round(value / v32, 3),
+                   "ratio_vs_fp32_tuned": round(value / v32o, 3),
+                   "rel_l2_vs_fp64": acc[0], "rel_l2_grad_x_vs_fp64": acc[1],
+                   "wall_s": round(wall, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

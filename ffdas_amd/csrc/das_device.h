@@ -245,6 +245,17 @@ __device__ __forceinline__ int voxel_of(const Tiling &tl, int tile, int lane, bo
     return (a * tl.H + h) * tl.W + w;
 }
 
+// Axial voxel order for the channel-row scatter: the spatial grid is viewed as (D0, R) with
+// D0 its first non-unit dimension (depth, in the (z, y, x) grids the reference's examples
+// build) and R the product of the others.  Voxel n = d0 * R + r is visited as
+// q = r * D0 + d0, and a wave takes 2^lg consecutive d0 for 64 >> lg values of r: along depth
+// the delay grows by several samples per voxel, so the lanes of one scatter instruction hit
+// distinct LDS samples (no same-address serialisation).
+struct Axial {
+    int D0, R;
+    int lg;   // log2 of the lanes along d0
+};
+
 // Problem geometry passed to every kernel by value.
 struct Geo {
     const float *srcpos;   // (C,3)

@@ -550,18 +550,22 @@ __device__ __forceinline__ int fx_exponent(const unsigned *maxes, int log2cnt) {
     return E < -900 ? -900 : (E > 900 ? 900 : E);
 }
 
-// grad_out transposed to frame-inner chunks and pre-scaled for fixed-point accumulation:
-// gt[chunk][n][f] = grad_out[chunk*FS+f][n] * 2^E as double (zero past B).
+// grad_out transposed to frame-inner chunks in axial voxel order and pre-scaled for
+// fixed-point accumulation: gt[chunk][q][f] = grad_out[chunk*FS+f][n(q)] * 2^E as double
+// (zero past B).  Chunk 0 also writes the geometry in axial order: pos[q], and ofs / wts /
+// sp [j][q] for every slot j.
 template <bool ACC64, bool CPLX, int FS>
 __global__ void transpose_gout_kernel(const void *__restrict__ goutv, int64_t gs_b, int B, int N,
-                                      const unsigned *__restrict__ maxes, int log2cnt,
-                                      double *__restrict__ gt) {
+                                      Axial ax, const unsigned *__restrict__ maxes, int log2cnt,
+                                      double *__restrict__ gt, Geo g, float *__restrict__ perm) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
     const A *__restrict__ gout = static_cast<const A *>(goutv);
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
-    if (n >= N) return;
+    if (q >= N) return;
+    const int r = q / ax.D0, d0 = q - r * ax.D0;
+    const int n = d0 * ax.R + r;
     const double scale = ldexp(1.0, fx_exponent(maxes, log2cnt));
     double v[FS * CW];
 #pragma unroll
@@ -571,9 +575,22 @@ __global__ void transpose_gout_kernel(const void *__restrict__ goutv, int64_t gs
         for (int c = 0; c < CW; c++)
             v[f * CW + c] = b < B ? (double)gout[((int64_t)b * gs_b + n) * CW + c] * scale : 0.0;
     }
-    double *dst = gt + ((int64_t)chunk * N + n) * FS * CW;
+    double *dst = gt + ((int64_t)chunk * N + q) * FS * CW;
 #pragma unroll
     for (int i = 0; i < FS * CW; i++) dst[i] = v[i];
+    if (chunk != 0) return;
+    float *pos = perm;
+    float *ofs = pos + 3 * (int64_t)N;
+    float *wts = ofs + (int64_t)g.K * N;
+    int *sp = reinterpret_cast<int *>(wts + (int64_t)g.K * N);
+    pos[3 * (int64_t)q] = g.dstpos[3 * (int64_t)n];
+    pos[3 * (int64_t)q + 1] = g.dstpos[3 * (int64_t)n + 1];
+    pos[3 * (int64_t)q + 2] = g.dstpos[3 * (int64_t)n + 2];
+    for (int j = 0; j < g.K; j++) {
+        ofs[(int64_t)j * N + q] = g.offsets[(int64_t)j * N + n];
+        wts[(int64_t)j * N + q] = g.weights[(int64_t)j * N + n];
+        if (g.sparse) sp[(int64_t)j * N + q] = g.sparse[(int64_t)j * N + n];
+    }
 }
 
 // max |v| over n scalars, folded into *out as float bits (non-negative floats order like
@@ -610,10 +627,11 @@ __device__ __forceinline__ void fx_add2(unsigned long long *p, double w1, double
 
 // grid.x = C*T rows (m = row / T, s = row % T), grid.y = frame chunks of FS frames.
 // The row lives in LDS as 64-bit fixed point [k][f] with an odd frame stride FSP
-// (conflict-free write-back); every lane owns one voxel, loads its FS grad_out values with
-// vector loads, then adds its two interpolation weights times grad_out into the row.
+// (conflict-free write-back); every lane owns one voxel (axial order, geometry and grad_out
+// already permuted to it), loads its FS grad_out values with vector loads, then adds its two
+// interpolation weights times grad_out into the row.
 template <bool ACC64, bool CPLX, bool PRECISE, int FS>
-__global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const void *__restrict__ gtv,
+__global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Axial ax, const void *__restrict__ gtv,
                                                         const unsigned *__restrict__ maxes,
                                                         int log2cnt, void *__restrict__ gxv,
                                                         int64_t gx_b, int64_t gx_c, int64_t gx_t) {
@@ -646,12 +664,15 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
     const bool unit = g.unit_scale != 0;
     const int lane = threadIdx.x & 63;
     const int waves = blockDim.x >> 6;
-    const int ntiles = tl.nta * tl.nth * tl.ntw;
+    const int dl = lane & ((1 << ax.lg) - 1);
+    const int RT = 64 >> ax.lg;
+    const int rl = lane >> ax.lg;
     const double *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
-    for (int tile = threadIdx.x >> 6; tile < ntiles; tile += waves) {
-        bool valid;
-        const int n = voxel_of(tl, tile, lane, valid);
-        if (!valid) continue;
+    for (int rb = (threadIdx.x >> 6) * RT; rb < ax.R; rb += waves * RT)
+    for (int d0b = 0; d0b < ax.D0; d0b += (1 << ax.lg)) {
+        const int d0 = d0b + dl, r = rb + rl;
+        if (d0 >= ax.D0 || r >= ax.R) continue;
+        const int n = r * ax.D0 + d0;   // axial index q
         // grad_out of this voxel first: its latency overlaps the geometry below
         double gv[FS * CW];
         const double *gp = gchunk + (int64_t)n * FS * CW;
@@ -706,6 +727,94 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Tiling tl, const 
             for (int c = 0; c < CW; c++) {
                 const long long q = (long long)lds[((size_t)k * FSP + f) * CW + c];
                 dst[(size_t)k * CW + c] = (A)((double)q * inv);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// adjoint: x, plain form (ALG1)
+// ---------------------------------------------------------------------------------------
+// The straightforward transpose of the plain forward kernel: one lane per voxel with the
+// grad_out of FT frames in registers, every in-range (voxel, channel, slot) adds its two
+// interpolation weights times grad_out into grad_x with global float atomics.  grad_x must
+// be zero on entry.  Summation order is not fixed, so results vary in the last bits.
+template <bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__global__ __launch_bounds__(256) void das_bwd_x_plain_kernel(
+    Geo g, Tiling tl, const void *__restrict__ goutv, int64_t gs_b, void *__restrict__ gxv,
+    int64_t gx_b, int64_t gx_c, int64_t gx_t) {
+    using A = AccT<ACC64>;
+    constexpr int CW = CPLX ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    bool valid;
+    const int n = voxel_of(tl, tile, lane, valid);
+    if (!valid) return;
+    const A *__restrict__ gout = static_cast<const A *>(goutv);
+    A *__restrict__ gx = static_cast<A *>(gxv);
+    const int b0 = blockIdx.y * FT;
+    const int nf = min(FT, g.B - b0);
+    A gv[FT][CW];
+#pragma unroll
+    for (int f = 0; f < FT; f++)
+#pragma unroll
+        for (int c = 0; c < CW; c++)
+            gv[f][c] = (b0 + f < g.B) ? gout[((int64_t)(b0 + f) * gs_b + n) * CW + c] : A(0);
+    const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
+    const bool unit = g.unit_scale != 0;
+    for (int kc = 0; kc < g.K; kc += KC) {
+        float ofs[KC], wt[KC];
+        int sq[KC];
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            const int jj = kc + j;
+            if (jj < g.K) {
+                const int64_t o = (int64_t)jj * g.N + n;
+                ofs[j] = g.offsets[o];
+                wt[j] = g.weights[o];
+                sq[j] = g.sparse ? g.sparse[o] : jj;
+            } else {
+                ofs[j] = 0.0f;
+                wt[j] = 0.0f;
+                sq[j] = -1;
+            }
+        }
+        for (int m = 0; m < g.C; m++) {
+            const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
+            const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
+            if (g.srcdir) {
+                const float *sd = g.srcdir + 4 * m;
+                if (!((R.dx * sd[0] + R.dy * sd[1] + R.dz * sd[2]) * R.rinv > sd[3])) continue;
+            }
+#pragma unroll
+            for (int j = 0; j < KC; j++) {
+                if (kc + j >= g.K) break;
+                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                if (!((uint32_t)tp.k < (uint32_t)(g.S - 1) && (uint32_t)sq[j] < (uint32_t)g.T)) continue;
+                A *row = gx + ((int64_t)b0 * gx_b + (int64_t)m * gx_c + (int64_t)sq[j] * gx_t +
+                               tp.k) * CW;
+                const A t = (A)tp.t;
+                if constexpr (!CPLX) {
+                    const A w1 = (A)wt[j] * t, w0 = (A)wt[j] - w1;
+                    for (int f = 0; f < nf; f++) {
+                        atomicAdd(row + f * gx_b, w0 * gv[f][0]);
+                        atomicAdd(row + f * gx_b + 1, w1 * gv[f][0]);
+                    }
+                } else {
+                    float sn, cs;
+                    phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
+                    const A cr = (A)wt[j] * (A)cs, ci = (A)wt[j] * (A)sn;
+                    for (int f = 0; f < nf; f++) {
+                        // conj(w e^{i phi}) g
+                        const A vr = cr * gv[f][0] + ci * gv[f][1];
+                        const A vi = cr * gv[f][1] - ci * gv[f][0];
+                        A *q = row + f * gx_b * CW;
+                        atomicAdd(q, (A(1) - t) * vr);
+                        atomicAdd(q + 1, (A(1) - t) * vi);
+                        atomicAdd(q + 2, t * vr);
+                        atomicAdd(q + 3, t * vi);
+                    }
+                }
             }
         }
     }
@@ -1043,6 +1152,18 @@ dev::Tiling make_tiling(const std::vector<int64_t> &spatial) {
     return t;
 }
 
+dev::Axial make_axial(const DasProblem &p) {
+    dev::Axial ax{};
+    ax.D0 = 1;
+    for (int64_t v : p.spatial)
+        if (v > 1) { ax.D0 = (int)v; break; }
+    ax.R = (int)(p.N / ax.D0);
+    ax.lg = 6;
+    if (const char *env = std::getenv("FFDAS_BWDX_AXIAL")) ax.lg = std::max(0, std::min(6, std::atoi(env)));
+    while (ax.lg > 0 && (1 << (ax.lg - 1)) >= ax.D0) ax.lg--;
+    return ax;
+}
+
 // Waves per voxel tile for the frame-inner gathers: enough to put about 32 waves on each
 // CU (8 per SIMD), at most 8 and at most one per channel.  FFDAS_CGROUPS overrides.
 int channel_groups(const dev::Geo &g, int64_t tiles) {
@@ -1121,17 +1242,43 @@ hipError_t launch_bwdp_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &t
     }
 }
 
+template <bool ACC64, bool CPLX, bool PRECISE, int FT>
+hipError_t launch_bwdx_plain_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+                                const void *gout, int64_t gs_b, void *gx) {
+    const int ntiles = tl.nta * tl.nth * tl.ntw;
+    dim3 grid((ntiles + 3) / 4, (g.B + FT - 1) / FT);
+    const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
+    ProfScope ps(st, "das_bwd_x_plain");
+    dev::das_bwd_x_plain_kernel<ACC64, CPLX, PRECISE, FT, kKC><<<grid, 256, 0, st>>>(
+        g, tl, gout, gs_b, gx, gx_b, gx_c, gx_t);
+    return hipGetLastError();
+}
+
+template <bool ACC64, bool CPLX, bool PRECISE>
+hipError_t launch_bwdx_plain_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+                               const void *gout, int64_t gs_b, void *gx) {
+    switch (frame_tile_for(g.B)) {
+        case 16: return launch_bwdx_plain_ft<ACC64, CPLX, PRECISE, 16>(st, g, tl, gout, gs_b, gx);
+        case 8: return launch_bwdx_plain_ft<ACC64, CPLX, PRECISE, 8>(st, g, tl, gout, gs_b, gx);
+        case 4: return launch_bwdx_plain_ft<ACC64, CPLX, PRECISE, 4>(st, g, tl, gout, gs_b, gx);
+        case 2: return launch_bwdx_plain_ft<ACC64, CPLX, PRECISE, 2>(st, g, tl, gout, gs_b, gx);
+        default: return launch_bwdx_plain_ft<ACC64, CPLX, PRECISE, 1>(st, g, tl, gout, gs_b, gx);
+    }
+}
+
 template <bool ACC64, bool CPLX, bool PRECISE, int FS>
-hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &ax,
                           const void *gout, int64_t gs_b, void *gt, void *gx) {
     using A = typename std::conditional<ACC64, double, float>::type;
     const size_t esz = 8 * (CPLX ? 2 : 1);
     const int fsp = FS > 1 ? (FS | 1) : 1;
     const size_t lds = (size_t)g.S * (size_t)fsp * esz;
     const int nchunks = (g.B + FS - 1) / FS;
-    // workspace: [maxes: 2 words, padded to 256 B][transposed grad_out]
+    // workspace: [maxes: 2 words, padded to 256 B][transposed grad_out][axial geometry]
     unsigned *maxes = static_cast<unsigned *>(gt);
     void *gtr = static_cast<char *>(gt) + 256;
+    float *perm = reinterpret_cast<float *>(static_cast<char *>(gtr) +
+                                            (size_t)nchunks * FS * g.N * esz);
     // contributions per LDS sample at most N (dense) or N*K (sparse), doubled for complex
     const double cnt = (double)g.N * (g.sparse ? g.K : 1) * (CPLX ? 2 : 1);
     int log2cnt = 0;
@@ -1147,20 +1294,25 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Tiling &
     {
     ProfScope pt(st, "transpose_gout");
     dev::transpose_gout_kernel<ACC64, CPLX, FS><<<dim3((g.N + 255) / 256, nchunks), 256, 0, st>>>(
-        gout, gs_b, g.B, g.N, maxes, log2cnt, static_cast<double *>(gtr));
+        gout, gs_b, g.B, g.N, ax, maxes, log2cnt, static_cast<double *>(gtr), g, perm);
     }
     dim3 grid(g.C * g.T, nchunks);
     const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
     (void)hipFuncSetAttribute((const void *)dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     ProfScope ps(st, "das_bwd_x");
+    dev::Geo ga = g;
+    ga.dstpos = perm;
+    ga.offsets = perm + 3 * (int64_t)g.N;
+    ga.weights = ga.offsets + (int64_t)g.K * g.N;
+    ga.sparse = g.sparse ? reinterpret_cast<const int *>(ga.weights + (int64_t)g.K * g.N) : nullptr;
     dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, 512, lds, st>>>(
-        g, tl, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t);
+        ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t);
     return hipGetLastError();
 }
 
 template <bool ACC64, bool CPLX, bool PRECISE>
-hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl,
                          const void *gout, int64_t gs_b, void *gt, void *gx, int FS) {
     switch (FS) {
         case 16: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 16>(st, g, tl, gout, gs_b, gt, gx);
@@ -1420,6 +1572,7 @@ bool use_sorted_bwdx(const DasProblem &p) {
 int sorted_kpt(const DasProblem &p) { return p.S <= 1024 ? 1 : (p.S <= 2048 ? 2 : 4); }
 
 size_t backward_workspace(const DasProblem &p) {
+    if (p.plain) return 0;
     if (use_sorted_bwdx(p)) {
         const int F = sorted_frames(sorted_kpt(p), p.cplx, p.B);
         const size_t chunks = (size_t)(p.B + F - 1) / F;
@@ -1429,7 +1582,8 @@ size_t backward_workspace(const DasProblem &p) {
     if (fs <= 0) return 0;
     const size_t esz = 8 * (p.cplx ? 2 : 1);   // pre-scaled double grad_out
     const size_t nchunks = (size_t)(p.B + fs - 1) / fs;
-    return 256 + nchunks * (size_t)fs * (size_t)p.N * esz;
+    const size_t perm = (size_t)p.N * (3 + 3 * (size_t)p.K) * sizeof(float);   // axial geometry
+    return 256 + nchunks * (size_t)fs * (size_t)p.N * esz + perm;
 }
 
 ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
@@ -1477,7 +1631,21 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
 #undef BWDPFI
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }
-    if (gx && use_sorted_bwdx(p)) {
+    if (gx && p.plain) {
+        const int64_t cnt = (int64_t)p.B * p.C * p.T * p.S * (p.cplx ? 2 : 1) * (p.acc64 ? 2 : 1);
+        FFDAS_TRY(launch_fill_zero(h, static_cast<float *>(gx), cnt));
+        if (p.acc64) {
+            e = p.cplx ? launch_bwdx_plain_t<true, true, true>(st, g, tl, gout, gs_b, gx)
+                       : launch_bwdx_plain_t<true, false, true>(st, g, tl, gout, gs_b, gx);
+        } else if (precise) {
+            e = p.cplx ? launch_bwdx_plain_t<false, true, true>(st, g, tl, gout, gs_b, gx)
+                       : launch_bwdx_plain_t<false, false, true>(st, g, tl, gout, gs_b, gx);
+        } else {
+            e = p.cplx ? launch_bwdx_plain_t<false, true, false>(st, g, tl, gout, gs_b, gx)
+                       : launch_bwdx_plain_t<false, false, false>(st, g, tl, gout, gs_b, gx);
+        }
+        if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+    } else if (gx && use_sorted_bwdx(p)) {
         const int KPT = sorted_kpt(p);
         const int F = sorted_frames(KPT, p.cplx, p.B);
         if (p.cplx)
@@ -1490,15 +1658,16 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     } else if (gx) {
         const int FS = bwdx_frames(p);
         if (FS <= 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
+        const dev::Axial ax = make_axial(p);
         if (p.acc64) {
-            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, tl, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<true, false, true>(st, g, tl, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, ax, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<true, false, true>(st, g, ax, gout, gs_b, gt, gx, FS);
         } else if (precise) {
-            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, tl, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, true>(st, g, tl, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, ax, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, true>(st, g, ax, gout, gs_b, gt, gx, FS);
         } else {
-            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, tl, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, false>(st, g, tl, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, ax, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, false>(st, g, ax, gout, gs_b, gt, gx, FS);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }

@@ -157,6 +157,7 @@ ffdas_error_t plan_das(const ffdas_tensor_desc &x, const ffdas_tensor_desc &out,
     }
 
     if (!srcpos || !dstpos || !offsets || !weights) return FFDAS_ERROR_NULL_POINTER;
+    p.plain = alg == FFDAS_ALG1;
     return FFDAS_SUCCESS;
 }
 

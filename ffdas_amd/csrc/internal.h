@@ -60,6 +60,7 @@ struct DasProblem {
     bool acc64 = false;                   // double accumulation (double data)
     bool x_needs_copy = false;            // cast / relayout into a contiguous workspace
     int fi_frames = 0;                    // > 0: frame-inner chunks of this many frames
+    bool plain = false;                   // ALG1: plain kernels (natural layout, atomic adjoint)
     int64_t xs_b = 0, xs_c = 0, xs_t = 0; // strides of x as given (elements)
     int64_t os_b = 0;                     // batch stride of out (elements)
     // spatial shape of the output (without batch), used for voxel tiling

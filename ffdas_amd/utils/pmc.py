@@ -1,0 +1,114 @@
+"""HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE).
+
+The counters are reported in KiB per dispatch.  Each is scaled by what the same counter
+reads for a device-to-device copy of known size (256 MiB, issued by ``bench.py
+--profile-only``), so the result is in bytes as a plain copy would count them.
+
+  python -m ffdas_amd.utils.pmc <fetch_run_dir> <write_run_dir> --out profiles/<name>
+
+writes ``<name>.json`` (per-kernel bytes, used by bench.py for ``roofline.traffic``) and
+``<name>.md`` (the table and the calibration factors).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import re
+
+COPY_BYTES = 256 * 1024 * 1024
+
+
+def short_name(kernel: str) -> str:
+    m = re.search(r"ffdas::dev::(\w+)", kernel)
+    if m:
+        return m.group(1)
+    m = re.search(r"_ZN5ffdas3dev\d+([A-Za-z_]\w*?)(?:I|E)", kernel)
+    if m:
+        return m.group(1)
+    if "copyBuffer" in kernel:
+        return "copyBuffer"
+    return kernel.split("(")[0][-60:]
+
+
+def read_counter(run_dir: str, counter: str) -> dict:
+    """{short kernel name: [values per dispatch]} for one counter."""
+    files = glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {run_dir}")
+    per = collections.defaultdict(float)
+    names = {}
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                key = (f, int(r["Dispatch_Id"]))
+                per[key] += float(r["Counter_Value"])
+                names[key] = short_name(r["Kernel_Name"])
+    out = collections.defaultdict(list)
+    for key in sorted(per):
+        out[names[key]].append(per[key])
+    return out
+
+
+def summarize(fetch_dir: str, write_dir: str) -> dict:
+    fetch = read_counter(fetch_dir, "FETCH_SIZE")
+    write = read_counter(write_dir, "WRITE_SIZE")
+    if "copyBuffer" not in fetch or "copyBuffer" not in write:
+        raise RuntimeError("the calibration copy (copyBuffer) is missing from the runs")
+    f_copy = max(fetch["copyBuffer"]) * 1024.0
+    w_copy = max(write["copyBuffer"]) * 1024.0
+    kf, kw = COPY_BYTES / f_copy, COPY_BYTES / w_copy
+    kernels = {}
+    for name in sorted(set(fetch) | set(write)):
+        if name == "copyBuffer" or not name.endswith("kernel"):
+            continue
+        # the last dispatch of each kernel (after warm-up)
+        fb = fetch[name][-1] * 1024.0 * kf if name in fetch else None
+        wb = write[name][-1] * 1024.0 * kw if name in write else None
+        kernels[name] = {"fetch_bytes": fb, "write_bytes": wb,
+                         "traffic_bytes": (fb or 0.0) + (wb or 0.0),
+                         "dispatches": len(fetch.get(name, []))}
+    return {"calibration": {"copy_bytes": COPY_BYTES, "fetch_counter_bytes": f_copy,
+                            "write_counter_bytes": w_copy, "fetch_factor": kf,
+                            "write_factor": kw},
+            "kernels": kernels}
+
+
+def to_markdown(s: dict, command: str) -> str:
+    c = s["calibration"]
+    lines = ["# HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE)", "",
+             f"Command: `{command}`", "",
+             f"Calibration: a {c['copy_bytes'] / 2**20:.0f} MiB device copy reads "
+             f"FETCH_SIZE = {c['fetch_counter_bytes'] / 2**20:.1f} MiB and WRITE_SIZE = "
+             f"{c['write_counter_bytes'] / 2**20:.1f} MiB; factors {c['fetch_factor']:.3f} "
+             f"(fetch) and {c['write_factor']:.3f} (write) are applied below.", "",
+             "| kernel | fetch MB | write MB | traffic MB |", "|---|---|---|---|"]
+    for k, v in s["kernels"].items():
+        fb = v["fetch_bytes"] or 0.0
+        wb = v["write_bytes"] or 0.0
+        lines.append(f"| {k} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} |")
+    return "\n".join(lines) + "\n"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("fetch_dir")
+    ap.add_argument("write_dir")
+    ap.add_argument("--out", required=True, help="output path without extension")
+    ap.add_argument("--command", default="", help="the profiled command, for the summary")
+    a = ap.parse_args()
+    s = summarize(a.fetch_dir, a.write_dir)
+    with open(a.out + ".json", "w") as fh:
+        json.dump(s, fh, indent=1)
+    with open(a.out + ".md", "w") as fh:
+        fh.write(to_markdown(s, a.command))
+    print(to_markdown(s, a.command))
+
+
+if __name__ == "__main__":
+    main()

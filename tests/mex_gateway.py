@@ -1,0 +1,155 @@
+"""The MATLAB MEX gateways' argument handling, restated over the C-ABI (no MATLAB here).
+
+A MATLAB gpuArray of size (d1, ..., dn) is column-major; the reference's gateways view it
+as the row-major array of dims (dn, ..., d1) with C-contiguous strides
+(bindings/matlab/+ffdas/+core/src/_mex_ndarray.h:300-311), and MATLAB drops trailing
+singleton dimensions (an array keeps at least two).  Here such an array is a torch tensor
+holding that row-major view; `matlab_squeeze` applies MATLAB's rule to it.
+
+`mex_das` / `mex_das_sparse` follow ffdas_das.cpp:13-151 and ffdas_das_sparse.cpp:13-148:
+the dimension checks and their messages, the re-insertion of a squeezed (batch, sequence)
+pair, the (channels, sequence) swap for channels_trailing=false done as a strided
+descriptor (no copy), the output allocation, the host zero `beta`, the compute type, and
+the 1-based to 0-based index shift of das_sparse.m:52.  They call `ffdas_das` /
+`ffdas_das_sparse` through ctypes exactly as the gateways call them from C++.
+
+Two gateway quirks are not reproduced: after the channels_trailing=false permute the
+gateways read the sequence length from the channel axis (ffdas_das.cpp:59), and the sparse
+gateway passes sparse_indices.shape(0) as the sparse count even when MATLAB has squeezed a
+count of 1 away (ffdas_das_sparse.cpp:131).  Here the sequence length is the transmit
+axis and a squeezed index table means a count of 1.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ffdas_amd._core import element_code, handle_for
+from ffdas_amd._lib import ComputeType, TensorDesc, check, lib
+
+
+class MexError(RuntimeError):
+    """mexErrMsgIdAndTxt."""
+
+
+def matlab_squeeze(t: torch.Tensor) -> torch.Tensor:
+    """Drop MATLAB-trailing (row-major leading) singleton dims, keeping at least two."""
+    while t.dim() > 2 and t.shape[0] == 1:
+        t = t[0]
+    return t
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _x_view(x: torch.Tensor, channels_trailing: bool, name: str) -> torch.Tensor:
+    # ffdas_das.cpp:39-57
+    if x.dim() < 3 and not (not channels_trailing and x.dim() == 2):
+        raise MexError(f"{name}: x must have 3 or 4 dimensions (got {x.dim()})")
+    if x.dim() > 4:
+        raise MexError(f"{name}: x must have 3 or 4 dimensions (got {x.dim()})")
+    if not channels_trailing:
+        if x.dim() == 2:            # batch and sequence both squeezed away by MATLAB
+            x = x.reshape(1, x.shape[0], x.shape[1])
+        x = x.permute(1, 0, 2) if x.dim() == 3 else x.permute(0, 2, 1, 3)
+    return x
+
+
+def _check_geometry(name, x, srcpos, dstpos, srcdir, tables):
+    sequence = x.shape[-2]          # x is ([B,] C, T, S) here
+    if srcpos.dim() != 2 or srcpos.shape[1] != 3:
+        raise MexError(f"{name}: srcpos must have shape (3, channels)")
+    if srcdir is not None and srcdir.numel() > 0:
+        if srcdir.dtype != torch.float32 or srcdir.dim() != 2 or srcdir.shape[1] != 4:
+            raise MexError(f"{name}: srcdir must be a single array of shape (4, channels)")
+    dstnd = dstpos.dim()
+    if dstnd < 2 or dstpos.shape[-1] != 3:
+        raise MexError(f"{name}: dstpos must have shape (3, ...)")
+    first = tables[0]
+    if any(t.shape != first.shape for t in tables[1:]):
+        raise MexError(f"{name}: offsets and weights must have the same shape")
+    return sequence, dstnd, first
+
+
+def _check_tables(name, sequence, dstnd, dstpos, t, lead):
+    # ffdas_das.cpp:75-92 (lead = sequence for das, the sparse count for das_sparse)
+    if lead == 1 and t.dim() != dstnd - 1:
+        raise MexError(f"{name}: offsets and weights must have the same number of dimensions "
+                       f"as dstpos")
+    if lead > 1 and t.dim() - 1 != dstnd - 1:
+        raise MexError(f"{name}: offsets and weights must have the same number of dimensions "
+                       f"as dstpos")
+    if lead > 1 and t.shape[0] != lead:
+        raise MexError(f"{name}: leading dimension of offsets/weights must match the channel "
+                       f"dimension of x")
+    for i in range(dstnd - 1):
+        if t.shape[i + 1 if lead > 1 else i] != dstpos.shape[i]:
+            raise MexError(f"{name}: spatial dimensions of offsets/weights must match dstpos")
+
+
+def _out_dtype(x):
+    return {torch.float16: torch.float32, torch.bfloat16: torch.float32,
+            torch.complex32: torch.complex64}.get(x.dtype, x.dtype)
+
+
+def _compute_type(x, use_fp16):
+    if x.dtype in (torch.float64, torch.complex128):
+        return ComputeType.FP64
+    return ComputeType.FP16 if use_fp16 else ComputeType.FP32
+
+
+def _call(name, x, srcpos, dstpos, offsets, weights, srcdir, wavenum, algorithm, use_fp16,
+          sparse=None, count=0):
+    have_batch = x.dim() == 4
+    spatial = tuple(dstpos.shape[:-1])
+    out_dims = ((x.shape[0],) if have_batch else ()) + spatial
+    out = torch.empty(out_dims, dtype=_out_dtype(x), device=x.device)
+    x_desc = TensorDesc(tuple(x.shape), tuple(x.stride()), element_code(x))
+    o_desc = TensorDesc(tuple(out.shape), tuple(out.stride()), element_code(out))
+    beta = ctypes.create_string_buffer(16)         # calloc(1, type size): zero
+    sd = srcdir if srcdir is not None and srcdir.numel() > 0 else None
+    L = lib()
+    with torch.cuda.device(x.device):
+        h = handle_for(x)
+        if sparse is None:
+            rc = L.ffdas_das(h.h, _ptr(srcpos), _ptr(sd), float(wavenum), x_desc.ptr, _ptr(x),
+                             _ptr(dstpos), _ptr(offsets), _ptr(weights),
+                             ctypes.cast(beta, ctypes.c_void_p), o_desc.ptr, _ptr(out),
+                             int(_compute_type(x, use_fp16)), int(algorithm))
+        else:
+            rc = L.ffdas_das_sparse(h.h, _ptr(srcpos), _ptr(sd), float(wavenum), x_desc.ptr,
+                                    _ptr(x), _ptr(dstpos), _ptr(offsets), _ptr(weights),
+                                    int(count), _ptr(sparse), ctypes.cast(beta, ctypes.c_void_p),
+                                    o_desc.ptr, _ptr(out), int(_compute_type(x, use_fp16)),
+                                    int(algorithm))
+    try:
+        check(rc, name)
+    except RuntimeError as e:
+        raise MexError(f"{name}: returned error {rc}: {e}") from e
+    return out
+
+
+def mex_das(x, srcpos, dstpos, offsets, weights, srcdir=None, wavenum=0.0, algorithm=0,
+            use_fp16=False, channels_trailing=True):
+    """ffdas.das from MATLAB: ffdas_das.cpp:13-151 (das.m supplies the defaults)."""
+    name = "ffdas_das"
+    x = _x_view(x, channels_trailing, name)
+    sequence, dstnd, t = _check_geometry(name, x, srcpos, dstpos, srcdir, [offsets, weights])
+    _check_tables(name, sequence, dstnd, dstpos, t, sequence)
+    return _call(name, x, srcpos, dstpos, offsets, weights, srcdir, wavenum, algorithm, use_fp16)
+
+
+def mex_das_sparse(x, srcpos, dstpos, offsets, weights, sparse_indices_1based, srcdir=None,
+                   wavenum=0.0, algorithm=0, use_fp16=False, channels_trailing=True):
+    """ffdas.das_sparse from MATLAB: das_sparse.m:45-53 (1-based indices, minus 1) and
+    ffdas_das_sparse.cpp:13-148."""
+    name = "ffdas_das_sparse"
+    sparse = (sparse_indices_1based.to(torch.int32) - 1).contiguous()
+    x = _x_view(x, channels_trailing, name)
+    _, dstnd, t = _check_geometry(name, x, srcpos, dstpos, srcdir, [offsets, weights, sparse])
+    count = t.shape[0] if t.dim() == dstnd else 1
+    _check_tables(name, count, dstnd, dstpos, t, count)
+    return _call(name, x, srcpos, dstpos, offsets, weights, srcdir, wavenum, algorithm, use_fp16,
+                 sparse=sparse, count=count)

@@ -145,3 +145,59 @@ def test_grad_x_row_lengths(dev, S, cplx):
                              wavenum=w.wavenum)
     gx, _, _, _ = _grads(w, dev, gout, need_offsets=False, need_weights=False, need_scale=False)
     assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
+
+
+@pytest.mark.parametrize("N", [(5,), (70, 3), (3, 4, 5), (1, 40, 2), (130, 1, 3), (1, 1)])
+def test_grad_x_grid_shapes(dev, N):
+    """Axial voxel order of the grad_x scatter: first dimension shorter and longer than a
+    wave, unit dimensions, 1-D and 3-D grids, a single voxel."""
+    w = W.small_random(seed=sum(N), B=3, C=4, T=2, S=120, N=N)
+    rng = np.random.default_rng(5)
+    gout = rng.standard_normal((3,) + w.dstpos.shape[:-1]).astype(np.float32)
+    ref = oracle.das_adjoint(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, gout)
+    gx, _, _, _ = _grads(w, dev, gout, need_offsets=False, need_weights=False, need_scale=False)
+    assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["dense", "dir", "sparse", "complex", "scaled", "float64", "batched"])
+def test_plain_adjoint_alg1(dev, case):
+    """ALG1 runs the plain kernels: natural-layout parameter gather and the global-atomic
+    grad_x scatter; parity with the FP64 oracle's adjoint as for the tuned path."""
+    import torch
+    import ffdas_amd
+    scale = 1.0
+    xt = None
+    if case == "dense":
+        w = W.small_random(seed=41)
+    elif case == "dir":
+        w = W.small_random(seed=42, srcdir=True)
+    elif case == "sparse":
+        w = W.small_random(seed=43, T=5, K=3, sparse=True)
+    elif case == "complex":
+        w = W.small_random(seed=44, complex_iq=True)
+    elif case == "scaled":
+        w = W.small_random(seed=45)
+        scale = 0.97
+    elif case == "float64":
+        w = W.small_random(seed=46)
+        xt = torch.float64
+    else:
+        w = W.small_random(seed=47, B=20, C=5, T=3, S=100, N=(9, 8))
+    rng = np.random.default_rng(9)
+    shape = (w.x.shape[0],) + w.dstpos.shape[:-1]
+    gout = rng.standard_normal(shape)
+    if np.iscomplexobj(w.x):
+        gout = gout + 1j * rng.standard_normal(shape)
+    gout = gout.astype(np.complex64 if np.iscomplexobj(w.x) else (np.float64 if xt else np.float32))
+    ref = oracle.das_adjoint(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             srcdir=w.srcdir, wavenum=w.wavenum, sparse_indices=w.sparse_indices,
+                             scale=scale)
+    d = to_dev(w, dev, xt)
+    gx, go, gw, gs = ffdas_amd.das_backward(
+        d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"], torch.from_numpy(gout).to(dev),
+        srcdir=d.get("srcdir"), wavenum=d.get("wavenum", 0.0),
+        sparse_indices=d.get("sparse_indices"), scale=scale, algorithm=ffdas_amd.Algorithm.ALG1)
+    assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
+    assert rel_l2(go.cpu().numpy(), ref["grad_offsets"]) < 1e-4
+    assert rel_l2(gw.cpu().numpy(), ref["grad_weights"]) < 1e-5
+    assert abs(float(gs) - ref["grad_scale"]) <= 1e-4 * max(1.0, abs(ref["grad_scale"]))
