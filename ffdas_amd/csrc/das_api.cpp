@@ -171,7 +171,7 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
     }
     const size_t xb = cast_bytes(p);
     const size_t gb = grad_x ? align_up(backward_workspace(p)) : 0;
-    if (grad_x && gb == 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
+    if (grad_x && gb == 0 && !p.plain) return FFDAS_ERROR_DIMS_TOO_LARGE;   // row exceeds LDS
     void *ws = nullptr;
     FFDAS_TRY(handle->workspace(xb + gb, &ws));
     XView xv;

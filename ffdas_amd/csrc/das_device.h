@@ -151,7 +151,8 @@ __device__ __forceinline__ Tap make_tap(float ofs, const Radius &R, float scale,
     Tap tp;
     if constexpr (!PRECISE) {
         float tau = ofs + R.rh;
-        if (!unit_scale) tau *= scale;
+        tau *= scale;                      // exact for scale == 1
+        (void)unit_scale;
         tp.kf = __builtin_floorf(tau);
         tp.t = tau - tp.kf;
     } else {

@@ -536,73 +536,73 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
 // ---------------------------------------------------------------------------------------
 // adjoint: x (channel-row scatter through LDS)
 // ---------------------------------------------------------------------------------------
-// Exponent E of the fixed-point scale: with bound = max|w| * max|g| < 2^e, every scaled
-// contribution |w g 2^E| < 2^min(40, 61 - log2cnt) stays inside the +-2^51 range of the FP64
-// round-to-integer trick, and the sum of at most 2^log2cnt contributions per sample fits in
-// 63 bits.
-__device__ __forceinline__ int fx_exponent(const unsigned *maxes, int log2cnt) {
-    const double bound = (double)__uint_as_float(maxes[0]) * (double)__uint_as_float(maxes[1]);
-    if (!(bound > 0.0)) return 0;
-    int e;
-    frexp(bound, &e);                            // bound < 2^e
-    const int top = min(40, 61 - log2cnt);
-    int E = top - e;
-    return E < -900 ? -900 : (E > 900 ? 900 : E);
-}
-
-// grad_out transposed to frame-inner chunks in axial voxel order and pre-scaled for
-// fixed-point accumulation: gt[chunk][q][f] = grad_out[chunk*FS+f][n(q)] * 2^E as double
-// (zero past B).  Chunk 0 also writes the geometry in axial order: pos[q], and ofs / wts /
-// sp [j][q] for every slot j.
+// grad_out transposed to frame-inner chunks in axial voxel order:
+// gt[chunk][q][f] = grad_out[chunk*FS+f][n(q)] (zero past B).  Chunk 0 also writes the
+// geometry in axial order: pos[q], and ofs / wts / sp [j][q] for every slot j.  The largest
+// |weight| and |grad_out| are folded into maxes[0] and maxes[1] (as float bits: non-negative
+// floats order like their bit patterns), which must be zero on entry.
 template <bool ACC64, bool CPLX, int FS>
-__global__ void transpose_gout_kernel(const void *__restrict__ goutv, int64_t gs_b, int B, int N,
-                                      Axial ax, const unsigned *__restrict__ maxes, int log2cnt,
-                                      double *__restrict__ gt, Geo g, float *__restrict__ perm) {
+__global__ __launch_bounds__(256) void transpose_gout_kernel(
+    const void *__restrict__ goutv, int64_t gs_b, int B, int N, Axial ax,
+    AccT<ACC64> *__restrict__ gt, Geo g, float *__restrict__ perm, unsigned *__restrict__ maxes) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
     const A *__restrict__ gout = static_cast<const A *>(goutv);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
-    if (q >= N) return;
-    const int r = q / ax.D0, d0 = q - r * ax.D0;
-    const int n = d0 * ax.R + r;
-    const double scale = ldexp(1.0, fx_exponent(maxes, log2cnt));
-    double v[FS * CW];
+    float gmax = 0.0f, wmax = 0.0f;
+    if (q < N) {
+        const int r = q / ax.D0, d0 = q - r * ax.D0;
+        const int n = d0 * ax.R + r;
+        A v[FS * CW];
 #pragma unroll
-    for (int f = 0; f < FS; f++) {
-        const int b = chunk * FS + f;
+        for (int f = 0; f < FS; f++) {
+            const int b = chunk * FS + f;
 #pragma unroll
-        for (int c = 0; c < CW; c++)
-            v[f * CW + c] = b < B ? (double)gout[((int64_t)b * gs_b + n) * CW + c] * scale : 0.0;
+            for (int c = 0; c < CW; c++) {
+                v[f * CW + c] = b < B ? gout[((int64_t)b * gs_b + n) * CW + c] : A(0);
+                gmax = fmaxf(gmax, (float)fabs((double)v[f * CW + c]));
+            }
+        }
+        A *dst = gt + ((int64_t)chunk * N + q) * FS * CW;
+#pragma unroll
+        for (int i = 0; i < FS * CW; i++) dst[i] = v[i];
+        if (chunk == 0) {
+            float *pos = perm;
+            float *ofs = pos + 3 * (int64_t)N;
+            float *wts = ofs + (int64_t)g.K * N;
+            int *sp = reinterpret_cast<int *>(wts + (int64_t)g.K * N);
+            pos[3 * (int64_t)q] = g.dstpos[3 * (int64_t)n];
+            pos[3 * (int64_t)q + 1] = g.dstpos[3 * (int64_t)n + 1];
+            pos[3 * (int64_t)q + 2] = g.dstpos[3 * (int64_t)n + 2];
+            for (int j = 0; j < g.K; j++) {
+                ofs[(int64_t)j * N + q] = g.offsets[(int64_t)j * N + n];
+                const float wv = g.weights[(int64_t)j * N + n];
+                wts[(int64_t)j * N + q] = wv;
+                wmax = fmaxf(wmax, fabsf(wv));
+                if (g.sparse) sp[(int64_t)j * N + q] = g.sparse[(int64_t)j * N + n];
+            }
+        }
     }
-    double *dst = gt + ((int64_t)chunk * N + q) * FS * CW;
-#pragma unroll
-    for (int i = 0; i < FS * CW; i++) dst[i] = v[i];
-    if (chunk != 0) return;
-    float *pos = perm;
-    float *ofs = pos + 3 * (int64_t)N;
-    float *wts = ofs + (int64_t)g.K * N;
-    int *sp = reinterpret_cast<int *>(wts + (int64_t)g.K * N);
-    pos[3 * (int64_t)q] = g.dstpos[3 * (int64_t)n];
-    pos[3 * (int64_t)q + 1] = g.dstpos[3 * (int64_t)n + 1];
-    pos[3 * (int64_t)q + 2] = g.dstpos[3 * (int64_t)n + 2];
-    for (int j = 0; j < g.K; j++) {
-        ofs[(int64_t)j * N + q] = g.offsets[(int64_t)j * N + n];
-        wts[(int64_t)j * N + q] = g.weights[(int64_t)j * N + n];
-        if (g.sparse) sp[(int64_t)j * N + q] = g.sparse[(int64_t)j * N + n];
+    // block maximum first: one atomic per block and maximum
+    __shared__ float red[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
+        wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
     }
-}
-
-// max |v| over n scalars, folded into *out as float bits (non-negative floats order like
-// their bit patterns, so an integer atomicMax suffices).
-template <class T>
-__global__ void absmax_kernel(const T *__restrict__ a, int64_t n, unsigned *__restrict__ out) {
-    float m = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, (float)fabs((double)a[i]));
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = gmax;
+        red[1][threadIdx.x >> 6] = wmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            gmax = fmaxf(gmax, red[0][w]);
+            wmax = fmaxf(wmax, red[1][w]);
+        }
+        if (gmax > 0.0f) atomicMax(maxes + 1, __float_as_uint(gmax));
+        if (chunk == 0 && wmax > 0.0f) atomicMax(maxes, __float_as_uint(wmax));
+    }
 }
 
 // Fixed-point accumulation: the contribution w * g (g already scaled by 2^E) is rounded to
@@ -625,23 +625,77 @@ __device__ __forceinline__ void fx_add2(unsigned long long *p, double w1, double
     atomicAdd(p, (unsigned long long)__double_as_longlong(y) - kFxMagicBits);
 }
 
+// LDS slot of sample k: within each 32-sample block the low five bits are XORed with the
+// block number, so samples a multiple of 4 or 8 apart (neighbouring voxels along depth) land
+// in different LDS banks.  A bijection on [0, 32 * ceil(S / 32)).
+__device__ __forceinline__ int lds_slot(int k) { return k ^ ((k >> 5) & 31); }
+
+// Packed fixed point: two frames share one 64-bit LDS word, frame 2p in the low and frame
+// 2p+1 in the high 32 bits, each an integer sum in units of 2^-E.  Adding
+// c0 + c1 * 2^32 (two's complement) for every contribution leaves
+// sum0 + sum1 * 2^32 mod 2^64 in the word; as long as |sum0|, |sum1| < 2^31 the low half
+// sign-extended is sum0 and the rest is sum1.  c0 and c1 come from the FP64
+// round-to-integer FMA (the integer sits in the low mantissa bits of y - 1.5 * 2^52), so the
+// packed word is lo = lo32(y0), hi = hi32(y0) - 0x43380000 + lo32(y1).
+__device__ __forceinline__ void fx_add_pair(unsigned long long *p, double y0, double y1) {
+    const unsigned long long b0 = (unsigned long long)__double_as_longlong(y0);
+    const unsigned long long b1 = (unsigned long long)__double_as_longlong(y1);
+    const unsigned lo = (unsigned)b0;
+    const unsigned hi = (unsigned)(b0 >> 32) + (unsigned)b1 + 0xBCC80000u;
+    atomicAdd(p, ((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ void fx_unpack_pair(unsigned long long w, long long &s0, long long &s1) {
+    s0 = (long long)(int)(unsigned)w;
+    s1 = ((long long)w - s0) >> 32;
+}
+
+// Largest value in the block (all threads take part, all receive it).
+__device__ __forceinline__ unsigned block_max_u32(unsigned v, unsigned *red) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned r = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = max(r, red[w]);
+    __syncthreads();
+    return r;
+}
+
 // grid.x = C*T rows (m = row / T, s = row % T), grid.y = frame chunks of FS frames.
-// The row lives in LDS as 64-bit fixed point [k][f] with an odd frame stride FSP
-// (conflict-free write-back); every lane owns one voxel (axial order, geometry and grad_out
-// already permuted to it), loads its FS grad_out values with vector loads, then adds its two
-// interpolation weights times grad_out into the row.
+// The row lives in LDS as fixed point [word][slot(k)] (word = frame or frame pair, times the
+// complex part), so the lanes of one atomic differ only in slot; each lane owns
+// one voxel per step (axial order: geometry and grad_out were permuted to it, so the
+// per-wave loads are contiguous) and adds its two interpolation weights times grad_out
+// into the row.  The loads of a lane's next voxel are issued before the current voxel's
+// atomics.
+//
+// PACK (float data): two frames per 64-bit word with 32-bit sums.  The scale 2^E is chosen
+// per row from max|w| * max|g| and the largest number of contributions any sample of the
+// row receives: a first pass assumes at most 2^kLogCnt0 and counts the contributions per
+// sample in LDS; a row that exceeded it is accumulated again with the exponent its count
+// allows.  No 32-bit sum can overflow, and the rounding error per contribution is below
+// 2^-(E+1) (about 2^-24 of the largest contribution in the usual case).
+// !PACK (double data, or one frame per chunk): one 64-bit integer per frame, with the
+// global bound of N (or N*K sparse) contributions per sample.
+constexpr int kLogCnt0 = 8;
+
 template <bool ACC64, bool CPLX, bool PRECISE, int FS>
-__global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Axial ax, const void *__restrict__ gtv,
-                                                        const unsigned *__restrict__ maxes,
-                                                        int log2cnt, void *__restrict__ gxv,
-                                                        int64_t gx_b, int64_t gx_c, int64_t gx_t) {
+__global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const void *__restrict__ gtv,
+                                                         const unsigned *__restrict__ maxes,
+                                                         int log2cnt, void *__restrict__ gxv,
+                                                         int64_t gx_b, int64_t gx_c, int64_t gx_t) {
     using A = AccT<ACC64>;
     using U64 = unsigned long long;
+    constexpr bool PACK = !ACC64 && FS >= 2;
     constexpr int CW = CPLX ? 2 : 1;
-    constexpr int FSP = (FS > 1) ? (FS | 1) : 1;
+    constexpr int NW = PACK ? FS / 2 : FS;                 // words per sample and part
+    constexpr int WS = NW * CW;                            // words per sample
     extern __shared__ __align__(16) unsigned char lds_raw[];
     U64 *lds = reinterpret_cast<U64 *>(lds_raw);
-    const double *__restrict__ gt = static_cast<const double *>(gtv);
+    const int S32 = (g.S + 31) & ~31;
+    unsigned *cnt = reinterpret_cast<unsigned *>(lds + (size_t)S32 * WS);   // PACK only
+    __shared__ unsigned red[16];
+    const A *__restrict__ gt = static_cast<const A *>(gtv);
     A *__restrict__ gx = static_cast<A *>(gxv);
 
     const int row = blockIdx.x;
@@ -650,10 +704,21 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Axial ax, const v
     const int chunk = blockIdx.y;
     const int b0 = chunk * FS;
     const int nf = min(FS, g.B - b0);
-    const int nlds = g.S * FSP * CW;
-    for (int i = threadIdx.x; i < nlds; i += blockDim.x) lds[i] = 0ull;
-    const int E = fx_exponent(maxes, log2cnt);
-    __syncthreads();
+
+    // exponent of max|w| * max|g| (bound < 2^eM)
+    int eM = 0;
+    {
+        const double bound = (double)__uint_as_float(maxes[0]) * (double)__uint_as_float(maxes[1]);
+        if (bound > 0.0) frexp(bound, &eM);
+    }
+    auto exponent = [&](int logc) {
+        // PACK: sums < 2^30; !PACK: contributions < 2^min(40, 61 - logc) (FP64 trick range,
+        // 63-bit sums)
+        const int top = PACK ? 30 - logc - (CPLX ? 1 : 0) : min(40, 61 - logc);
+        const int E = top - eM;
+        return E < -900 ? -900 : (E > 900 ? 900 : E);
+    };
+    int E = exponent(PACK ? kLogCnt0 : log2cnt);
 
     const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
     float sdx = 0, sdy = 0, sdz = 0, sdw = 0;
@@ -661,72 +726,184 @@ __global__ __launch_bounds__(512) void das_bwd_x_kernel(Geo g, Axial ax, const v
         sdx = g.srcdir[4 * m]; sdy = g.srcdir[4 * m + 1]; sdz = g.srcdir[4 * m + 2];
         sdw = g.srcdir[4 * m + 3];
     }
-    const bool unit = g.unit_scale != 0;
+    const bool dense = g.sparse == nullptr;
     const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int waves = blockDim.x >> 6;
-    const int dl = lane & ((1 << ax.lg) - 1);
-    const int RT = 64 >> ax.lg;
-    const int rl = lane >> ax.lg;
-    const double *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
-    for (int rb = (threadIdx.x >> 6) * RT; rb < ax.R; rb += waves * RT)
-    for (int d0b = 0; d0b < ax.D0; d0b += (1 << ax.lg)) {
-        const int d0 = d0b + dl, r = rb + rl;
-        if (d0 >= ax.D0 || r >= ax.R) continue;
-        const int n = r * ax.D0 + d0;   // axial index q
-        // grad_out of this voxel first: its latency overlaps the geometry below
-        double gv[FS * CW];
-        const double *gp = gchunk + (int64_t)n * FS * CW;
+    const int D0T = 1 << ax.lg, RT = 64 >> ax.lg;
+    const int dl = lane & (D0T - 1), rl = lane >> ax.lg;
+    const int per_r = (ax.D0 + D0T - 1) >> ax.lg;
+    const int nitems = per_r * ((ax.R + RT - 1) / RT);
+    const A *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
+
+    struct Vox {
+        A gv[FS * CW];
+        float x, y, z, ofs, w;
+        int q;
+        bool valid;
+    };
+    auto fetch = [&](int item, Vox &v) {
+        const int rg = item / per_r;
+        const int d0 = (item - rg * per_r) * D0T + dl;
+        const int r = rg * RT + rl;
+        v.valid = d0 < ax.D0 && r < ax.R;
+        v.q = v.valid ? r * ax.D0 + d0 : 0;     // invalid lanes read voxel 0, never scatter
+        const A *gp = gchunk + (int64_t)v.q * FS * CW;
 #pragma unroll
-        for (int i = 0; i < FS * CW; i++) gv[i] = gp[i];
-        const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
-        const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
-        if (g.srcdir && !((R.dx * sdx + R.dy * sdy + R.dz * sdz) * R.rinv > sdw)) continue;
-        const int jlo = g.sparse ? 0 : s;
-        const int jhi = g.sparse ? g.K : s + 1;
-        for (int j = jlo; j < jhi; j++) {
-            const int64_t o = (int64_t)j * g.N + n;
-            if (g.sparse && g.sparse[o] != s) continue;
-            const Tap tp = make_tap<PRECISE>(g.offsets[o], R, g.scale, unit);
-            if ((uint32_t)tp.k >= (uint32_t)(g.S - 1)) continue;
-            const float w = g.weights[o];
-            U64 *l0 = lds + (size_t)tp.k * FSP * CW;
-            U64 *l1 = l0 + FSP * CW;
+        for (int i = 0; i < FS * CW; i++) v.gv[i] = gp[i];
+        v.x = g.dstpos[3 * (int64_t)v.q];
+        v.y = g.dstpos[3 * (int64_t)v.q + 1];
+        v.z = g.dstpos[3 * (int64_t)v.q + 2];
+        if (dense) {
+            v.ofs = g.offsets[(int64_t)s * g.N + v.q];
+            v.w = g.weights[(int64_t)s * g.N + v.q];
+        }
+    };
+
+    for (int pass = 0; pass < 2; pass++) {
+        const bool counting = PACK && pass == 0;
+        const double sc = ldexp(1.0, E);
+        for (int i = threadIdx.x; i < S32 * WS; i += blockDim.x) lds[i] = 0ull;
+        if (counting)
+            for (int i = threadIdx.x; i < S32; i += blockDim.x) cnt[i] = 0u;
+        __syncthreads();
+
+        auto scatter = [&](const Vox &v, const Radius &R, float ofs, float w) {
+            const Tap tp = make_tap<PRECISE>(ofs, R, g.scale, g.unit_scale != 0);
+            if ((uint32_t)tp.k >= (uint32_t)(g.S - 1)) return;
+            const int s0 = lds_slot(tp.k), s1 = lds_slot(tp.k + 1);
+            U64 *l0 = lds + s0;
+            U64 *l1 = lds + s1;
+            if (counting) {
+                atomicAdd(cnt + s0, 1u);
+                atomicAdd(cnt + s1, 1u);
+            }
             const double t = (double)tp.t;
+            const double wd = (double)w * sc;
             if constexpr (!CPLX) {
-                const double w1 = (double)w * t, w0 = (double)w - w1;
+                const double w1 = wd * t, w0 = wd - w1;
+                if constexpr (PACK) {
 #pragma unroll
-                for (int f = 0; f < FS; f++) {
-                    fx_add(l0 + f, w0, gv[f]);
-                    fx_add(l1 + f, w1, gv[f]);
+                    for (int p = 0; p < NW; p++) {
+                        const double ga = (double)v.gv[2 * p], gb = (double)v.gv[2 * p + 1];
+                        fx_add_pair(l0 + p * S32, __builtin_fma(w0, ga, kFxMagic), __builtin_fma(w0, gb, kFxMagic));
+                        fx_add_pair(l1 + p * S32, __builtin_fma(w1, ga, kFxMagic), __builtin_fma(w1, gb, kFxMagic));
+                    }
+                } else {
+#pragma unroll
+                    for (int f = 0; f < FS; f++) {
+                        const double gd = (double)v.gv[f];
+                        fx_add(l0 + f * S32, w0, gd);
+                        fx_add(l1 + f * S32, w1, gd);
+                    }
                 }
             } else {
                 float sn, cs;
                 phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
-                const double cr = (double)w * (double)cs, ci = (double)w * (double)sn;
+                const double cr = wd * (double)cs, ci = wd * (double)sn;
                 const double omt = 1.0 - t;
                 // conj(coef) * g = (cr gr + ci gi) + i (cr gi - ci gr), split by (1-t) and t
                 const double a_r = cr * omt, a_i = ci * omt, b_r = cr * t, b_i = ci * t;
+                if constexpr (PACK) {
+                    // words [p][re, im], each packing frames 2p, 2p+1
 #pragma unroll
-                for (int f = 0; f < FS; f++) {
-                    const double gr = gv[2 * f], gi = gv[2 * f + 1];
-                    fx_add2(l0 + 2 * f, a_r, gr, a_i, gi);
-                    fx_add2(l0 + 2 * f + 1, a_r, gi, -a_i, gr);
-                    fx_add2(l1 + 2 * f, b_r, gr, b_i, gi);
-                    fx_add2(l1 + 2 * f + 1, b_r, gi, -b_i, gr);
+                    for (int p = 0; p < NW; p++) {
+                        const double gr0 = (double)v.gv[4 * p], gi0 = (double)v.gv[4 * p + 1];
+                        const double gr1 = (double)v.gv[4 * p + 2], gi1 = (double)v.gv[4 * p + 3];
+                        fx_add_pair(l0 + (2 * p) * S32,
+                                    __builtin_fma(a_r, gr0, __builtin_fma(a_i, gi0, kFxMagic)),
+                                    __builtin_fma(a_r, gr1, __builtin_fma(a_i, gi1, kFxMagic)));
+                        fx_add_pair(l0 + (2 * p + 1) * S32,
+                                    __builtin_fma(a_r, gi0, __builtin_fma(-a_i, gr0, kFxMagic)),
+                                    __builtin_fma(a_r, gi1, __builtin_fma(-a_i, gr1, kFxMagic)));
+                        fx_add_pair(l1 + (2 * p) * S32,
+                                    __builtin_fma(b_r, gr0, __builtin_fma(b_i, gi0, kFxMagic)),
+                                    __builtin_fma(b_r, gr1, __builtin_fma(b_i, gi1, kFxMagic)));
+                        fx_add_pair(l1 + (2 * p + 1) * S32,
+                                    __builtin_fma(b_r, gi0, __builtin_fma(-b_i, gr0, kFxMagic)),
+                                    __builtin_fma(b_r, gi1, __builtin_fma(-b_i, gr1, kFxMagic)));
+                    }
+                } else {
+#pragma unroll
+                    for (int f = 0; f < FS; f++) {
+                        const double gr = (double)v.gv[2 * f], gi = (double)v.gv[2 * f + 1];
+                        fx_add2(l0 + (2 * f) * S32, a_r, gr, a_i, gi);
+                        fx_add2(l0 + (2 * f + 1) * S32, a_r, gi, -a_i, gr);
+                        fx_add2(l1 + (2 * f) * S32, b_r, gr, b_i, gi);
+                        fx_add2(l1 + (2 * f + 1) * S32, b_r, gi, -b_i, gr);
+                    }
                 }
             }
+        };
+        auto process = [&](const Vox &v) {
+            if (!v.valid) return;
+            const SplitPt yp = split_point(v.x, v.y, v.z);
+            const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
+            if (g.srcdir && !((R.dx * sdx + R.dy * sdy + R.dz * sdz) * R.rinv > sdw)) return;
+            if (dense) {
+                scatter(v, R, v.ofs, v.w);
+            } else {
+                for (int j = 0; j < g.K; j++) {
+                    const int64_t o = (int64_t)j * g.N + v.q;
+                    if (g.sparse[o] != s) continue;
+                    scatter(v, R, g.offsets[o], g.weights[o]);
+                }
+            }
+        };
+
+        // two voxel buffers in turn: the next voxel's loads overlap this one's scatter
+        Vox va, vb;
+        va.valid = false;
+        vb.valid = false;
+        int item = wave;
+        if (item < nitems) fetch(item, va);
+        while (item < nitems) {
+            vb.valid = false;
+            if (item + waves < nitems) fetch(item + waves, vb);
+            process(va);
+            item += waves;
+            if (item >= nitems) break;
+            va.valid = false;
+            if (item + waves < nitems) fetch(item + waves, va);
+            process(vb);
+            item += waves;
         }
+        __syncthreads();
+        if (!counting) break;
+        // the row's largest contribution count decides whether pass 0's sums are valid
+        unsigned cmax = 0;
+        for (int i = threadIdx.x; i < S32; i += blockDim.x) cmax = max(cmax, cnt[i]);
+        cmax = block_max_u32(cmax, red);
+        if (cmax <= (1u << kLogCnt0)) break;
+        int logc = kLogCnt0;
+        while ((1ull << logc) < cmax) logc++;
+        E = exponent(logc);
     }
-    __syncthreads();
+
     // write back: consecutive threads -> consecutive samples
     const double inv = ldexp(1.0, -E);
-    for (int f = 0; f < nf; f++) {
-        A *dst = gx + ((int64_t)(b0 + f) * gx_b + (int64_t)m * gx_c + (int64_t)s * gx_t) * CW;
-        for (int k = threadIdx.x; k < g.S; k += blockDim.x) {
+    for (int k = threadIdx.x; k < g.S; k += blockDim.x) {
+        const U64 *wp = lds + lds_slot(k);
+        A *dst = gx + ((int64_t)b0 * gx_b + (int64_t)m * gx_c + (int64_t)s * gx_t + k) * CW;
+        if constexpr (PACK) {
 #pragma unroll
-            for (int c = 0; c < CW; c++) {
-                const long long q = (long long)lds[((size_t)k * FSP + f) * CW + c];
-                dst[(size_t)k * CW + c] = (A)((double)q * inv);
+            for (int p = 0; p < NW; p++) {
+#pragma unroll
+                for (int c = 0; c < CW; c++) {
+                    long long v0, v1;
+                    fx_unpack_pair(wp[(size_t)(p * CW + c) * S32], v0, v1);
+                    if (2 * p < nf) dst[(int64_t)(2 * p) * gx_b * CW + c] = (A)((double)v0 * inv);
+                    if (2 * p + 1 < nf) dst[(int64_t)(2 * p + 1) * gx_b * CW + c] = (A)((double)v1 * inv);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < FS; f++) {
+                if (f >= nf) break;
+#pragma unroll
+                for (int c = 0; c < CW; c++)
+                    dst[(int64_t)f * gx_b * CW + c] =
+                        (A)((double)(long long)wp[(size_t)(f * CW + c) * S32] * inv);
             }
         }
     }
@@ -1158,7 +1335,7 @@ dev::Axial make_axial(const DasProblem &p) {
     for (int64_t v : p.spatial)
         if (v > 1) { ax.D0 = (int)v; break; }
     ax.R = (int)(p.N / ax.D0);
-    ax.lg = 6;
+    ax.lg = 4;
     if (const char *env = std::getenv("FFDAS_BWDX_AXIAL")) ax.lg = std::max(0, std::min(6, std::atoi(env)));
     while (ax.lg > 0 && (1 << (ax.lg - 1)) >= ax.D0) ax.lg--;
     return ax;
@@ -1242,6 +1419,15 @@ hipError_t launch_bwdp_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &t
     }
 }
 
+// LDS of the grad_x kernel: the fixed-point row (two frames per 64-bit word for float data,
+// one for double data or a single frame) plus, packed, the per-sample contribution counts.
+size_t bwdx_lds_bytes(int S, int FS, bool cplx, bool acc64) {
+    const size_t S32 = (size_t)((S + 31) & ~31);
+    const bool pack = !acc64 && FS >= 2;
+    const size_t ws = (size_t)(pack ? FS / 2 : FS) * (cplx ? 2 : 1);
+    return S32 * ws * 8 + (pack ? S32 * 4 : 0);
+}
+
 template <bool ACC64, bool CPLX, bool PRECISE, int FT>
 hipError_t launch_bwdx_plain_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
                                 const void *gout, int64_t gs_b, void *gx) {
@@ -1271,8 +1457,7 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
                           const void *gout, int64_t gs_b, void *gt, void *gx) {
     using A = typename std::conditional<ACC64, double, float>::type;
     const size_t esz = 8 * (CPLX ? 2 : 1);
-    const int fsp = FS > 1 ? (FS | 1) : 1;
-    const size_t lds = (size_t)g.S * (size_t)fsp * esz;
+    const size_t lds = bwdx_lds_bytes(g.S, FS, CPLX, ACC64);
     const int nchunks = (g.B + FS - 1) / FS;
     // workspace: [maxes: 2 words, padded to 256 B][transposed grad_out][axial geometry]
     unsigned *maxes = static_cast<unsigned *>(gt);
@@ -1285,16 +1470,10 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
     while ((double)(1ll << log2cnt) < cnt && log2cnt < 62) log2cnt++;
 
     {
-    ProfScope ps(st, "absmax");
-    dev::fill_zero_kernel<<<1, 64, 0, st>>>(reinterpret_cast<float *>(maxes), 2);
-    dev::absmax_kernel<float><<<256, 256, 0, st>>>(g.weights, (int64_t)g.K * g.N, maxes);
-    const int64_t gspan = ((int64_t)(g.B - 1) * gs_b + g.N) * (CPLX ? 2 : 1);
-    dev::absmax_kernel<A><<<512, 256, 0, st>>>(static_cast<const A *>(gout), gspan, maxes + 1);
-    }
-    {
     ProfScope pt(st, "transpose_gout");
+    dev::fill_zero_kernel<<<1, 64, 0, st>>>(reinterpret_cast<float *>(maxes), 2);
     dev::transpose_gout_kernel<ACC64, CPLX, FS><<<dim3((g.N + 255) / 256, nchunks), 256, 0, st>>>(
-        gout, gs_b, g.B, g.N, ax, maxes, log2cnt, static_cast<double *>(gtr), g, perm);
+        gout, gs_b, g.B, g.N, ax, static_cast<A *>(gtr), g, perm, maxes);
     }
     dim3 grid(g.C * g.T, nchunks);
     const int64_t gx_t = g.S, gx_c = (int64_t)g.T * g.S, gx_b = (int64_t)g.C * g.T * g.S;
@@ -1306,7 +1485,12 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
     ga.offsets = perm + 3 * (int64_t)g.N;
     ga.weights = ga.offsets + (int64_t)g.K * g.N;
     ga.sparse = g.sparse ? reinterpret_cast<const int *>(ga.weights + (int64_t)g.K * g.N) : nullptr;
-    dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, 512, lds, st>>>(
+    int threads = 1024;
+    if (const char *env = std::getenv("FFDAS_BWDX_THREADS")) {
+        const int v = std::atoi(env);
+        if (v == 256 || v == 512 || v == 1024) threads = v;
+    }
+    dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, threads, lds, st>>>(
         ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t);
     return hipGetLastError();
 }
@@ -1543,18 +1727,18 @@ ffdas_error_t launch_fill_zero(ffdas_context &h, float *p, int64_t n) {
 }
 
 // Frames per workgroup of the grad_x scatter: the largest of 16/8/4/2/1 whose LDS row fits
-// twice per CU (two workgroups resident), else once; FFDAS_BWDX_FRAMES overrides.
+// in a CU's LDS.  Measured on gfx950, one workgroup of 16 frames per CU beats two of 8: the
+// per-voxel delay arithmetic and loads are shared by more frames.  FFDAS_BWDX_FRAMES
+// overrides.
 int bwdx_frames(const DasProblem &p) {
-    const size_t esz = 8 * (p.cplx ? 2 : 1);   // 64-bit fixed-point LDS row
-    auto lds = [&](int fs) { return (size_t)p.S * (size_t)(fs > 1 ? (fs | 1) : 1) * esz; };
+    auto lds = [&](int fs) { return bwdx_lds_bytes(p.S, fs, p.cplx, p.acc64); };
     if (const char *env = std::getenv("FFDAS_BWDX_FRAMES")) {
         const int fs = std::atoi(env);
         if ((fs == 1 || fs == 2 || fs == 4 || fs == 8 || fs == 16) && lds(fs) <= (size_t)kLdsLimit)
             return fs;
     }
-    const int cap = p.B >= 16 ? 16 : p.B >= 8 ? 8 : p.B >= 4 ? 4 : p.B >= 2 ? 2 : 1;
-    for (int fs = cap; fs >= 1; fs /= 2)
-        if (lds(fs) <= (size_t)kLdsLimit / 2) return fs;
+    int cap = p.B >= 16 ? 16 : p.B >= 8 ? 8 : p.B >= 4 ? 4 : p.B >= 2 ? 2 : 1;
+    if (p.cplx && cap > 8) cap = 8;   // 16 complex frames exceed the register budget
     for (int fs = cap; fs >= 1; fs /= 2)
         if (lds(fs) <= (size_t)kLdsLimit) return fs;
     return 0;

@@ -1,13 +1,18 @@
-"""HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE).
+"""Summaries of rocprofv3 runs for profiles/.
 
-The counters are reported in KiB per dispatch.  Each is scaled by what the same counter
-reads for a device-to-device copy of known size (256 MiB, issued by ``bench.py
---profile-only``), so the result is in bytes as a plain copy would count them.
+traffic: HBM bytes per kernel launch from two counter runs (FETCH_SIZE, WRITE_SIZE).  The
+counters are reported in KiB per dispatch; each is scaled by what the same counter reads
+for a device-to-device copy of known size (256 MiB, issued by ``bench.py --profile-only``),
+so the result is in bytes as a plain copy would count them.
 
-  python -m ffdas_amd.utils.pmc <fetch_run_dir> <write_run_dir> --out profiles/<name>
+  python -m ffdas_amd.utils.pmc traffic <fetch_run_dir> <write_run_dir> --out profiles/<name>
 
 writes ``<name>.json`` (per-kernel bytes, used by bench.py for ``roofline.traffic``) and
-``<name>.md`` (the table and the calibration factors).
+``<name>.md``.
+
+stats: the kernel table of a ``rocprofv3 --kernel-trace --stats --output-format csv`` run.
+
+  python -m ffdas_amd.utils.pmc stats <run_dir> --out profiles/<name> --command "..."
 """
 from __future__ import annotations
 
@@ -95,19 +100,48 @@ def to_markdown(s: dict, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def kernel_stats(run_dir: str, top: int = 20) -> str:
+    """Markdown table of *kernel_stats.csv (calls, average and total time, share)."""
+    files = glob.glob(os.path.join(run_dir, "**", "*kernel_stats.csv"), recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no kernel_stats.csv under {run_dir}")
+    with open(files[0]) as fh:
+        rows = list(csv.DictReader(fh))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = ["| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
+    for r in rows[:top]:
+        name = r["Name"]
+        name = name.split("(")[0] if "ffdas::" in name else name[:90]
+        out.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                   f"{float(r['Percentage']):.2f} |")
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("fetch_dir")
-    ap.add_argument("write_dir")
-    ap.add_argument("--out", required=True, help="output path without extension")
-    ap.add_argument("--command", default="", help="the profiled command, for the summary")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--out", required=True, help="output path without extension")
+    t.add_argument("--command", default="", help="the profiled command, for the summary")
+    k = sub.add_parser("stats")
+    k.add_argument("run_dir")
+    k.add_argument("--out", required=True, help="output path without extension")
+    k.add_argument("--command", default="", help="the profiled command, for the summary")
+    k.add_argument("--title", default="rocprofv3 kernel statistics")
     a = ap.parse_args()
-    s = summarize(a.fetch_dir, a.write_dir)
-    with open(a.out + ".json", "w") as fh:
-        json.dump(s, fh, indent=1)
+    if a.cmd == "traffic":
+        s = summarize(a.fetch_dir, a.write_dir)
+        with open(a.out + ".json", "w") as fh:
+            json.dump(s, fh, indent=1)
+        text = to_markdown(s, a.command)
+    else:
+        text = f"# {a.title}\n\nCommand: `{a.command}`\n\n" + kernel_stats(a.run_dir)
     with open(a.out + ".md", "w") as fh:
-        fh.write(to_markdown(s, a.command))
-    print(to_markdown(s, a.command))
+        fh.write(text)
+    print(text)
 
 
 if __name__ == "__main__":
