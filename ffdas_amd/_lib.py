@@ -59,6 +59,20 @@ class Algorithm(enum.IntEnum):
     ALG4 = 4
 
 
+class DasTuning(ctypes.Structure):
+    """ffdas_das_tuning_t (include/ffdas.h)."""
+    _fields_ = [("groups", ctypes.c_int), ("fi_frames", ctypes.c_int),
+                ("tile_h", ctypes.c_int), ("tile_w", ctypes.c_int),
+                ("param_groups", ctypes.c_int), ("bwdx_frames", ctypes.c_int),
+                ("bwdx_depth_lanes", ctypes.c_int), ("bwdx_threads", ctypes.c_int),
+                ("forward_ms", ctypes.c_float), ("backward_ms", ctypes.c_float),
+                ("default_forward_ms", ctypes.c_float), ("default_backward_ms", ctypes.c_float),
+                ("candidates", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def _lib_path() -> Path:
     env = os.environ.get("FFDAS_LIB_DIR")
     return Path(env) / LIB_NAME if env else _PKG / LIB_NAME
@@ -108,6 +122,12 @@ def lib():
         "ffdas_event_record": (i32, [vp, u]),
         "ffdas_event_synchronize": (i32, [u]),
         "ffdas_event_elapsed_time": (i32, [u, u, P(f32)]),
+        "ffdas_das_autotune": (i32, [vp, vp, vp, f32, f32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,
+                                     vp, i32, i32, i32, P(DasTuning)]),
+        "ffdas_tune_save": (i32, [ctypes.c_char_p]),
+        "ffdas_tune_load": (i32, [ctypes.c_char_p]),
+        "ffdas_tune_clear": (i32, []),
+        "ffdas_tune_count": (i32, []),
         "ffdas_profile_enable": (i32, [i32]),
         "ffdas_profile_reset": (i32, []),
         "ffdas_profile_query": (i32, [ctypes.c_char_p, P(ctypes.c_double), P(i64)]),

@@ -3,6 +3,7 @@
 // the adjoint ffdas_das_backward and ffdas_contiguous_copy.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "internal.h"
 
@@ -82,6 +83,7 @@ ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, con
     DasProblem p;
     FFDAS_TRY(plan_das(xd, od, srcpos, dstpos, offsets, weights, sparse_mode, sparse_count,
                        sparse_indices, compute_type, alg, p));
+    resolve_tune(*handle, p, srcdir != nullptr);
     p.fi_frames = fi_frames_for(p, alg);
     if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
     if (!(scale > 0.0f) || !std::isfinite(scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
@@ -153,6 +155,7 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
     DasProblem p;
     FFDAS_TRY(plan_das(xd, gd, srcpos, dstpos, offsets, weights, sparse_indices != nullptr,
                        sparse_count, sparse_indices, compute_type, alg, p));
+    resolve_tune(*handle, p, srcdir != nullptr);
     p.fi_frames = fi_frames_for(p, alg);
     if (!x || !grad_out) return FFDAS_ERROR_NULL_POINTER;
     if (!(scale > 0.0f) || !std::isfinite(scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
@@ -229,3 +232,205 @@ ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------------------
+namespace ffdas {
+namespace {
+
+// Average duration [ms] of `reps` runs of fn on the handle's stream, after one warm-up run.
+template <class Fn>
+ffdas_error_t time_runs(ffdas_context &h, int reps, Fn fn, float &ms) {
+    FFDAS_TRY(fn());
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return FFDAS_ERROR_CUDA_RUNTIME;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return FFDAS_ERROR_CUDA_RUNTIME;
+    }
+    ffdas_error_t err = FFDAS_SUCCESS;
+    hipError_t he = hipEventRecord(a, h.stream);
+    for (int i = 0; i < reps && err == FFDAS_SUCCESS && he == hipSuccess; i++) err = fn();
+    if (he == hipSuccess) he = hipEventRecord(b, h.stream);
+    if (he == hipSuccess) he = hipEventSynchronize(b);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (err != FFDAS_SUCCESS) return err;
+    if (he != hipSuccess) return hip_status(he);
+    ms /= (float)reps;
+    return FFDAS_SUCCESS;
+}
+
+// Restores the handle's tuning override on scope exit.
+struct ForceGuard {
+    ffdas_context *h;
+    explicit ForceGuard(ffdas_context *hh) : h(hh) {}
+    ~ForceGuard() { h->force_tune = nullptr; }
+};
+
+}  // namespace
+}  // namespace ffdas
+
+extern "C" ffdas_error_t ffdas_das_autotune(
+    ffdas_handle_t handle, const float *srcpos, const float *srcdir, float wavenum, float scale,
+    ffdas_tensor_desc_t x_desc, const void *x, const float *dstpos, const float *offsets,
+    const float *weights, int sparse_count, const int *sparse_indices,
+    ffdas_tensor_desc_t out_desc, void *out, ffdas_tensor_desc_t grad_x_desc, void *grad_x,
+    ffdas_compute_type_t compute_type, ffdas_alg_t alg, int reps, ffdas_das_tuning_t *result) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    FFDAS_TRY(handle->check_device());
+    if (!x_desc || !out_desc) return FFDAS_ERROR_NULL_POINTER;
+    if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
+    if (!sparse_indices && sparse_count != 0) return FFDAS_ERROR_INVALID_ARGUMENT;
+    reps = reps < 1 ? 1 : (reps > 100 ? 100 : reps);
+    const bool sparse = sparse_indices != nullptr;
+    DasProblem p;
+    FFDAS_TRY(plan_das(*x_desc, *out_desc, srcpos, dstpos, offsets, weights, sparse,
+                       sparse_count, sparse_indices, compute_type, alg, p));
+    const std::string key = tune_key(*handle, p, srcdir != nullptr);
+    ForceGuard guard(handle);
+    const double zero[2] = {0.0, 0.0};   // host beta of any output type
+    auto forward = [&]() {
+        return das_forward_common(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                  offsets, weights, sparse, sparse_count, sparse_indices, zero,
+                                  out_desc, out, compute_type, alg);
+    };
+    int candidates = 0;
+    auto measure = [&](const DasTune &t, auto fn, float &ms) {
+        handle->force_tune = &t;
+        candidates++;
+        return time_runs(*handle, reps, fn, ms);
+    };
+
+    // forward: frames per frame-inner chunk x channel groups x 2-D wave tile
+    const DasTune heuristic;
+    float fwd_default = 0.0f;
+    FFDAS_TRY(measure(heuristic, forward, fwd_default));
+    DasTune best = heuristic;
+    float fwd_best = fwd_default;
+    std::vector<int> frames;
+    for (int F : {16, 8, 4}) {
+        DasProblem q = p;
+        q.tune.fi_frames = F;
+        if (fi_frames_for(q, alg) == F) frames.push_back(F);
+    }
+    int dims2 = 0;
+    for (int64_t v : p.spatial) dims2 += v > 1 ? 1 : 0;
+    std::vector<std::pair<int, int>> tiles = {{0, 0}};
+    if (dims2 == 2) tiles = {{8, 8}, {16, 4}, {4, 16}, {32, 2}};
+    std::vector<int> groups = {0};
+    if (!frames.empty()) {
+        groups.clear();
+        for (int G : {1, 2, 4, 8})
+            if (G <= p.C) groups.push_back(G);
+    } else {
+        frames.push_back(0);
+    }
+    for (int F : frames)
+        for (int G : groups)
+            for (auto tt : tiles) {
+                DasTune t = heuristic;
+                t.fi_frames = F;
+                t.groups = G;
+                t.tile_h = tt.first;
+                t.tile_w = tt.second;
+                float ms = 0.0f;
+                FFDAS_TRY(measure(t, forward, ms));
+                if (ms < fwd_best) {
+                    fwd_best = ms;
+                    best = t;
+                }
+            }
+
+    // adjoint: parameter kernel channel groups, grad_x frames / depth lanes / workgroup size
+    float bwd_default = 0.0f, bwd_best = 0.0f;
+    if (grad_x) {
+        handle->force_tune = &best;
+        FFDAS_TRY(forward());   // out now holds a forward result: used as grad_out below
+        float *gofs = nullptr;
+        const size_t gbytes = (size_t)p.K * (size_t)p.N * sizeof(float);
+        if (hipMalloc(reinterpret_cast<void **>(&gofs), gbytes) != hipSuccess)
+            return FFDAS_ERROR_ALLOCATION_FAILED;
+        auto param = [&]() {
+            return ffdas_das_backward(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                      offsets, weights, sparse_count, sparse_indices, out_desc,
+                                      out, nullptr, nullptr, gofs, nullptr, nullptr,
+                                      compute_type, alg);
+        };
+        auto bwdx = [&]() {
+            return ffdas_das_backward(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                      offsets, weights, sparse_count, sparse_indices, out_desc,
+                                      out, grad_x_desc, grad_x, nullptr, nullptr, nullptr,
+                                      compute_type, alg);
+        };
+        ffdas_error_t err = FFDAS_SUCCESS;
+        float pd = 0.0f, xd = 0.0f;
+        DasTune t0 = best;
+        t0.param_groups = 0;
+        t0.bwdx_frames = 0;
+        t0.bwdx_lg = -1;
+        t0.bwdx_threads = 0;
+        err = measure(t0, param, pd);
+        if (err == FFDAS_SUCCESS) err = measure(t0, bwdx, xd);
+        bwd_default = pd + xd;
+        float pb = pd, xb = xd;
+        DasTune bp = t0, bx = t0;
+        if (err == FFDAS_SUCCESS && best.fi_frames >= 0)
+            for (int G : {1, 2, 4, 8}) {
+                if (G > p.C || err != FFDAS_SUCCESS) break;
+                DasTune t = t0;
+                t.param_groups = G;
+                float ms = 0.0f;
+                err = measure(t, param, ms);
+                if (err == FFDAS_SUCCESS && ms < pb) {
+                    pb = ms;
+                    bp = t;
+                }
+            }
+        if (!p.plain)
+            for (int F : {16, 8, 4, 2})
+                for (int lg : {2, 3, 4, 5, 6})
+                    for (int th : {512, 1024}) {
+                        if (err != FFDAS_SUCCESS) break;
+                        DasTune t = t0;
+                        t.bwdx_frames = F;
+                        t.bwdx_lg = lg;
+                        t.bwdx_threads = th;
+                        DasProblem q = p;
+                        q.tune = t;
+                        if (bwdx_frames(q) != F) continue;
+                        float ms = 0.0f;
+                        err = measure(t, bwdx, ms);
+                        if (err == FFDAS_SUCCESS && ms < xb) {
+                            xb = ms;
+                            bx = t;
+                        }
+                    }
+        (void)hipFree(gofs);
+        if (err != FFDAS_SUCCESS) return err;
+        best.param_groups = bp.param_groups;
+        best.bwdx_frames = bx.bwdx_frames;
+        best.bwdx_lg = bx.bwdx_lg;
+        best.bwdx_threads = bx.bwdx_threads;
+        bwd_best = pb + xb;
+    }
+    tune_store(key, best);
+    if (result) {
+        result->groups = best.groups;
+        result->fi_frames = best.fi_frames;
+        result->tile_h = best.tile_h;
+        result->tile_w = best.tile_w;
+        result->param_groups = best.param_groups;
+        result->bwdx_frames = best.bwdx_frames;
+        result->bwdx_depth_lanes = best.bwdx_lg >= 0 ? (1 << best.bwdx_lg) : 0;
+        result->bwdx_threads = best.bwdx_threads;
+        result->forward_ms = fwd_best;
+        result->backward_ms = bwd_best;
+        result->default_forward_ms = fwd_default;
+        result->default_backward_ms = bwd_default;
+        result->candidates = candidates;
+    }
+    return FFDAS_SUCCESS;
+}

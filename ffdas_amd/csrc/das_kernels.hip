@@ -1311,7 +1311,7 @@ int frame_tile_for(int B) {
     return 1;
 }
 
-dev::Tiling make_tiling(const std::vector<int64_t> &spatial) {
+dev::Tiling make_tiling(const std::vector<int64_t> &spatial, const DasTune &tune = DasTune()) {
     std::vector<int64_t> d;
     for (int64_t v : spatial)
         if (v > 1) d.push_back(v);
@@ -1321,7 +1321,13 @@ dev::Tiling make_tiling(const std::vector<int64_t> &spatial) {
     if (d.size() >= 2) t.H = (int)d[d.size() - 2];
     for (size_t i = 0; i + 2 < d.size(); i++) t.A *= (int)d[i];
     if (d.size() >= 3) { t.ta = 4; t.th = 4; t.tw = 4; }
-    else if (d.size() == 2) { t.ta = 1; t.th = 8; t.tw = 8; }
+    else if (d.size() == 2) {
+        t.ta = 1; t.th = 8; t.tw = 8;
+        if (tune.tile_h > 0 && tune.tile_w > 0 && tune.tile_h * tune.tile_w == 64) {
+            t.th = tune.tile_h;
+            t.tw = tune.tile_w;
+        }
+    }
     else { t.ta = 1; t.th = 1; t.tw = 64; }
     t.nta = (t.A + t.ta - 1) / t.ta;
     t.nth = (t.H + t.th - 1) / t.th;
@@ -1335,7 +1341,7 @@ dev::Axial make_axial(const DasProblem &p) {
     for (int64_t v : p.spatial)
         if (v > 1) { ax.D0 = (int)v; break; }
     ax.R = (int)(p.N / ax.D0);
-    ax.lg = 4;
+    ax.lg = (p.tune.bwdx_lg >= 0 && p.tune.bwdx_lg <= 6) ? p.tune.bwdx_lg : 4;
     if (const char *env = std::getenv("FFDAS_BWDX_AXIAL")) ax.lg = std::max(0, std::min(6, std::atoi(env)));
     while (ax.lg > 0 && (1 << (ax.lg - 1)) >= ax.D0) ax.lg--;
     return ax;
@@ -1343,10 +1349,12 @@ dev::Axial make_axial(const DasProblem &p) {
 
 // Waves per voxel tile for the frame-inner gathers: enough to put about 32 waves on each
 // CU (8 per SIMD), at most 8 and at most one per channel.  FFDAS_CGROUPS overrides.
-int channel_groups(const dev::Geo &g, int64_t tiles) {
+int channel_groups(const dev::Geo &g, int64_t tiles, int want) {
     int G = 1;
     if (const char *env = std::getenv("FFDAS_CGROUPS")) {
         G = std::atoi(env);
+    } else if (want > 0) {
+        G = want;
     } else {
         while (G < 8 && tiles * G < 8192) G *= 2;
     }
@@ -1453,7 +1461,7 @@ hipError_t launch_bwdx_plain_t(hipStream_t st, const dev::Geo &g, const dev::Til
 }
 
 template <bool ACC64, bool CPLX, bool PRECISE, int FS>
-hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &ax,
+hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &ax, int threads,
                           const void *gout, int64_t gs_b, void *gt, void *gx) {
     using A = typename std::conditional<ACC64, double, float>::type;
     const size_t esz = 8 * (CPLX ? 2 : 1);
@@ -1485,25 +1493,22 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
     ga.offsets = perm + 3 * (int64_t)g.N;
     ga.weights = ga.offsets + (int64_t)g.K * g.N;
     ga.sparse = g.sparse ? reinterpret_cast<const int *>(ga.weights + (int64_t)g.K * g.N) : nullptr;
-    int threads = 1024;
-    if (const char *env = std::getenv("FFDAS_BWDX_THREADS")) {
-        const int v = std::atoi(env);
-        if (v == 256 || v == 512 || v == 1024) threads = v;
-    }
+    if (const char *env = std::getenv("FFDAS_BWDX_THREADS")) threads = std::atoi(env);
+    if (threads != 256 && threads != 512 && threads != 1024) threads = 1024;
     dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, threads, lds, st>>>(
         ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t);
     return hipGetLastError();
 }
 
 template <bool ACC64, bool CPLX, bool PRECISE>
-hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl,
+hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl, int threads,
                          const void *gout, int64_t gs_b, void *gt, void *gx, int FS) {
     switch (FS) {
-        case 16: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 16>(st, g, tl, gout, gs_b, gt, gx);
-        case 8: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 8>(st, g, tl, gout, gs_b, gt, gx);
-        case 4: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 4>(st, g, tl, gout, gs_b, gt, gx);
-        case 2: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 2>(st, g, tl, gout, gs_b, gt, gx);
-        default: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 1>(st, g, tl, gout, gs_b, gt, gx);
+        case 16: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 16>(st, g, tl, threads, gout, gs_b, gt, gx);
+        case 8: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 8>(st, g, tl, threads, gout, gs_b, gt, gx);
+        case 4: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 4>(st, g, tl, threads, gout, gs_b, gt, gx);
+        case 2: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 2>(st, g, tl, threads, gout, gs_b, gt, gx);
+        default: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 1>(st, g, tl, threads, gout, gs_b, gt, gx);
     }
 }
 
@@ -1513,7 +1518,7 @@ constexpr bool fi_ok() {
 }
 
 template <Storage ST, bool CPLX, int F>
-hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int groups,
                            const void *x, void *out, int64_t os_b, double br, double bi) {
     if constexpr (!fi_ok<ST, CPLX, F>()) {
         return hipErrorInvalidValue;
@@ -1521,7 +1526,7 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
         constexpr bool PRECISE = (ST == Storage::F32);
         const int ntiles = tl.nta * tl.nth * tl.ntw;
         dim3 grid(ntiles, (g.B + F - 1) / F);
-        const int G = channel_groups(g, (int64_t)grid.x * grid.y);
+        const int G = channel_groups(g, (int64_t)grid.x * grid.y, groups);
         const size_t lds = G > 1 ? (size_t)G * F * (CPLX ? 2 : 1) * 64 * sizeof(float) : 0;
         if (lds > 0)
             (void)hipFuncSetAttribute((const void *)dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC>,
@@ -1535,17 +1540,18 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
 
 template <Storage ST, bool CPLX>
 hipError_t launch_fwd_fi_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int F,
-                           const void *x, void *out, int64_t os_b, double br, double bi) {
+                           int groups, const void *x, void *out, int64_t os_b, double br,
+                           double bi) {
     switch (F) {
-        case 16: return launch_fwd_fi_f<ST, CPLX, 16>(st, g, tl, x, out, os_b, br, bi);
-        case 8: return launch_fwd_fi_f<ST, CPLX, 8>(st, g, tl, x, out, os_b, br, bi);
-        case 4: return launch_fwd_fi_f<ST, CPLX, 4>(st, g, tl, x, out, os_b, br, bi);
+        case 16: return launch_fwd_fi_f<ST, CPLX, 16>(st, g, tl, groups, x, out, os_b, br, bi);
+        case 8: return launch_fwd_fi_f<ST, CPLX, 8>(st, g, tl, groups, x, out, os_b, br, bi);
+        case 4: return launch_fwd_fi_f<ST, CPLX, 4>(st, g, tl, groups, x, out, os_b, br, bi);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <Storage ST, bool CPLX, int F>
-hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
+hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int groups,
                             const void *x, const void *gout, int64_t gs_b, float *gofs,
                             float *gwt, float *gscale) {
     if constexpr (!fi_ok<ST, CPLX, F>()) {
@@ -1554,7 +1560,7 @@ hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling
         constexpr bool PRECISE = (ST == Storage::F32);
         const int ntiles = tl.nta * tl.nth * tl.ntw;
         dim3 grid(ntiles, (g.B + F - 1) / F);
-        const int G = channel_groups(g, (int64_t)grid.x * grid.y);
+        const int G = channel_groups(g, (int64_t)grid.x * grid.y, groups);
         const size_t lds = G > 1 ? (size_t)G * 2 * kKC * 64 * sizeof(float) : 0;
         if (lds > 0)
             (void)hipFuncSetAttribute((const void *)dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC>,
@@ -1568,12 +1574,12 @@ hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling
 
 template <Storage ST, bool CPLX>
 hipError_t launch_bwdp_fi_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int F,
-                            const void *x, const void *gout, int64_t gs_b, float *gofs,
-                            float *gwt, float *gscale) {
+                            int groups, const void *x, const void *gout, int64_t gs_b,
+                            float *gofs, float *gwt, float *gscale) {
     switch (F) {
-        case 16: return launch_bwdp_fi_f<ST, CPLX, 16>(st, g, tl, x, gout, gs_b, gofs, gwt, gscale);
-        case 8: return launch_bwdp_fi_f<ST, CPLX, 8>(st, g, tl, x, gout, gs_b, gofs, gwt, gscale);
-        case 4: return launch_bwdp_fi_f<ST, CPLX, 4>(st, g, tl, x, gout, gs_b, gofs, gwt, gscale);
+        case 16: return launch_bwdp_fi_f<ST, CPLX, 16>(st, g, tl, groups, x, gout, gs_b, gofs, gwt, gscale);
+        case 8: return launch_bwdp_fi_f<ST, CPLX, 8>(st, g, tl, groups, x, gout, gs_b, gofs, gwt, gscale);
+        case 4: return launch_bwdp_fi_f<ST, CPLX, 4>(st, g, tl, groups, x, gout, gs_b, gofs, gwt, gscale);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1671,7 +1677,7 @@ hipError_t launch_bwdx_sorted_t(hipStream_t st, const dev::Geo &g, const void *g
 ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
                              const XView &xv, void *out, double beta_re, double beta_im) {
     const dev::Geo g = make_geo(a, p);
-    const dev::Tiling tl = make_tiling(p.spatial);
+    const dev::Tiling tl = make_tiling(p.spatial, p.tune);
     hipError_t e = hipSuccess;
     hipStream_t st = h.stream;
     const int64_t os_b = p.os_b;
@@ -1679,16 +1685,16 @@ ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeo
         const int F = p.fi_frames;
         switch (p.k_storage) {
             case Storage::F16:
-                e = p.cplx ? launch_fwd_fi_t<Storage::F16, true>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im)
-                           : launch_fwd_fi_t<Storage::F16, false>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im);
+                e = p.cplx ? launch_fwd_fi_t<Storage::F16, true>(st, g, tl, F, p.tune.groups, xv.ptr, out, os_b, beta_re, beta_im)
+                           : launch_fwd_fi_t<Storage::F16, false>(st, g, tl, F, p.tune.groups, xv.ptr, out, os_b, beta_re, beta_im);
                 break;
             case Storage::BF16:
-                e = p.cplx ? launch_fwd_fi_t<Storage::BF16, true>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im)
-                           : launch_fwd_fi_t<Storage::BF16, false>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im);
+                e = p.cplx ? launch_fwd_fi_t<Storage::BF16, true>(st, g, tl, F, p.tune.groups, xv.ptr, out, os_b, beta_re, beta_im)
+                           : launch_fwd_fi_t<Storage::BF16, false>(st, g, tl, F, p.tune.groups, xv.ptr, out, os_b, beta_re, beta_im);
                 break;
             case Storage::F32:
-                e = p.cplx ? launch_fwd_fi_t<Storage::F32, true>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im)
-                           : launch_fwd_fi_t<Storage::F32, false>(st, g, tl, F, xv.ptr, out, os_b, beta_re, beta_im);
+                e = p.cplx ? launch_fwd_fi_t<Storage::F32, true>(st, g, tl, F, p.tune.groups, xv.ptr, out, os_b, beta_re, beta_im)
+                           : launch_fwd_fi_t<Storage::F32, false>(st, g, tl, F, p.tune.groups, xv.ptr, out, os_b, beta_re, beta_im);
                 break;
             default: return FFDAS_ERROR_INTERNAL;
         }
@@ -1739,6 +1745,10 @@ int bwdx_frames(const DasProblem &p) {
     }
     int cap = p.B >= 16 ? 16 : p.B >= 8 ? 8 : p.B >= 4 ? 4 : p.B >= 2 ? 2 : 1;
     if (p.cplx && cap > 8) cap = 8;   // 16 complex frames exceed the register budget
+    const int want = p.tune.bwdx_frames;
+    if ((want == 1 || want == 2 || want == 4 || want == 8 || want == 16) && want <= cap &&
+        lds(want) <= (size_t)kLdsLimit)
+        return want;
     for (int fs = cap; fs >= 1; fs /= 2)
         if (lds(fs) <= (size_t)kLdsLimit) return fs;
     return 0;
@@ -1774,7 +1784,7 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
                               const XView &xv, const void *gout, int64_t gs_b, void *gx,
                               float *gofs, float *gwt, float *gscale, void *gt) {
     const dev::Geo g = make_geo(a, p);
-    const dev::Tiling tl = make_tiling(p.spatial);
+    const dev::Tiling tl = make_tiling(p.spatial, p.tune);
     hipStream_t st = h.stream;
     hipError_t e = hipSuccess;
     const bool precise = p.k_storage == Storage::F32 || p.k_storage == Storage::F64;
@@ -1788,7 +1798,7 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
         }
         FFDAS_TRY(launch_fill_zero(h, gscale, 1));
 #define BWDP(ST, A64, CX) e = launch_bwdp_t<ST, A64, CX>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale)
-#define BWDPFI(ST, CX) e = launch_bwdp_fi_t<ST, CX>(st, g, tl, p.fi_frames, xv.ptr, gout, gs_b, gofs, gwt, gscale)
+#define BWDPFI(ST, CX) e = launch_bwdp_fi_t<ST, CX>(st, g, tl, p.fi_frames, p.tune.param_groups, xv.ptr, gout, gs_b, gofs, gwt, gscale)
         if (p.fi_frames > 0) {
             switch (p.k_storage) {
                 case Storage::F16: if (p.cplx) BWDPFI(Storage::F16, true); else BWDPFI(Storage::F16, false); break;
@@ -1844,14 +1854,14 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
         if (FS <= 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
         const dev::Axial ax = make_axial(p);
         if (p.acc64) {
-            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, ax, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<true, false, true>(st, g, ax, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<true, false, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
         } else if (precise) {
-            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, ax, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, true>(st, g, ax, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
         } else {
-            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, ax, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, false>(st, g, ax, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, false>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }
@@ -1907,6 +1917,10 @@ int fi_frames_for(const DasProblem &p, ffdas_alg_t alg) {
     if (const char *env = std::getenv("FFDAS_LAYOUT"))
         if (std::strcmp(env, "natural") == 0) return 0;
     const int esz = (int)storage_bytes(p.k_storage) * (p.cplx ? 2 : 1);
+    const int want = p.tune.fi_frames;
+    if ((want == 4 || want == 8 || want == 16) && (want * esz) % 16 == 0 &&
+        (double)((p.B + want - 1) / want) * want <= 2.0 * p.B)
+        return want;
     // cost ~ padded frames * (1 + G / F): per-voxel delay arithmetic is worth about G
     // interpolations and is shared by the F frames of a chunk
     constexpr double G = 10.0;

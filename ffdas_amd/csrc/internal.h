@@ -23,6 +23,20 @@ struct ffdas_tensor_desc {
 };
 
 // Library handle: device, stream, workspace.
+namespace ffdas {
+// Launch parameters a tuner may choose; 0 (-1 for bwdx_lg) selects the built-in heuristic.
+struct DasTune {
+    int groups = 0;         // waves per voxel tile splitting the channels (forward gather)
+    int fi_frames = 0;      // frames per frame-inner chunk
+    int tile_h = 0;         // 2-D wave tile, tile_h * tile_w == 64
+    int tile_w = 0;
+    int param_groups = 0;   // waves per voxel tile in the parameter adjoint
+    int bwdx_frames = 0;    // frames per grad_x workgroup
+    int bwdx_lg = -1;       // log2 of the lanes along depth in the grad_x scatter
+    int bwdx_threads = 0;   // grad_x workgroup size
+};
+}  // namespace ffdas
+
 struct ffdas_context {
     int device_id = 0;
     int cu_count = 0;
@@ -33,6 +47,9 @@ struct ffdas_context {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     hipStream_t ws_stream = nullptr;
+
+    // set by the autotuner while it times candidates: replaces the cached/heuristic choice
+    const ffdas::DasTune *force_tune = nullptr;
 
     ffdas_error_t check_device() const;
     ffdas_error_t workspace(size_t bytes, void **ptr);
@@ -61,6 +78,7 @@ struct DasProblem {
     bool x_needs_copy = false;            // cast / relayout into a contiguous workspace
     int fi_frames = 0;                    // > 0: frame-inner chunks of this many frames
     bool plain = false;                   // ALG1: plain kernels (natural layout, atomic adjoint)
+    DasTune tune;                         // launch parameters (tuned or heuristic)
     int64_t xs_b = 0, xs_c = 0, xs_t = 0; // strides of x as given (elements)
     int64_t os_b = 0;                     // batch stride of out (elements)
     // spatial shape of the output (without batch), used for voxel tiling
@@ -128,6 +146,11 @@ struct ProfScope {
     ~ProfScope() { profile_end(ev, st, name); }
 };
 int fi_frames_for(const DasProblem &p, ffdas_alg_t alg);
+
+// tuning cache (tune.cpp)
+std::string tune_key(const ffdas_context &h, const DasProblem &p, bool has_srcdir);
+void resolve_tune(const ffdas_context &h, DasProblem &p, bool has_srcdir);
+void tune_store(const std::string &key, const DasTune &t);
 size_t fi_bytes(const DasProblem &p);
 ffdas_error_t launch_relayout_fi(ffdas_context &h, const DasProblem &p, const void *x, void *dst);
 

@@ -20,7 +20,7 @@ import torch
 
 from . import _core
 from ._core import TensorView, as_tensor, check_device_tensor, element_code, handle_for
-from ._lib import Algorithm, ComputeType, DType, TensorDesc, check, lib
+from ._lib import Algorithm, ComputeType, DasTuning, DType, TensorDesc, check, lib
 
 
 class DowncastWarning(UserWarning):
@@ -288,3 +288,68 @@ def das_backward(x, srcpos, dstpos, offsets, weights, grad_out, *, srcdir=None, 
         c = 1 if len(dims) == 4 else 0
         gx = gx.transpose(c, c + 1)
     return gx, go, gw, gs
+
+
+def autotune(x, srcpos, dstpos, offsets, weights, *, srcdir=None, wavenum=0.0,
+             sparse_indices=None, scale=1.0, use_fp16=False, use_bf16=False,
+             channels_trailing=False, adjoint=True, reps=5,
+             algorithm: Algorithm = Algorithm.DEFAULT) -> dict:
+    """Time candidate launch parameters for this problem (ffdas_das_autotune) and keep the
+    fastest in the library's tuning cache: later :func:`das` / :func:`das_backward` calls with
+    the same signature (device, dtypes, channels, transmits, samples, frames, grid, sparse
+    count, directivity) use them.  With ``adjoint`` the grad_x and parameter kernels are tuned
+    too.  Synchronous.  Returns the chosen parameters and the measured times [ms]."""
+    x = as_tensor(x)
+    check_device_tensor(x, "x")
+    dev = x.device
+    srcpos = _astype(srcpos, torch.float32, "srcpos")
+    dstpos = _astype(dstpos, torch.float32, "dstpos")
+    offsets = _astype(offsets, torch.float32, "offsets")
+    weights = _astype(weights, torch.float32, "weights")
+    srcdir = _astype(srcdir, torch.float32, "srcdir")
+    if sparse_indices is not None:
+        sparse_indices = _astype(sparse_indices, torch.int32, "sparse_indices")
+    ct = _compute_type(x, use_fp16, use_bf16)
+    odt = _out_dtype(element_code(x))
+    shape, _ = _core.shape_strides(x)
+    batch = (shape[0],) if len(shape) == 4 else ()
+    out = torch.empty(batch + tuple(dstpos.shape[:-1]), dtype=odt, device=dev)
+    o_desc = TensorDesc(tuple(out.shape), tuple(out.stride()), element_code(out))
+    x_desc = _x_desc(x, channels_trailing)
+    dims = list(shape)
+    if channels_trailing:
+        c = 1 if len(dims) == 4 else 0
+        dims[c], dims[c + 1] = dims[c + 1], dims[c]
+    gx = torch.empty(dims, dtype=odt, device=dev) if adjoint else None
+    gx_desc = TensorDesc(dims, None, _out_code(odt)) if adjoint else None
+    res = DasTuning()
+    L = lib()
+    with torch.cuda.device(dev):
+        h = handle_for(x)
+        rc = L.ffdas_das_autotune(
+            h.h, _ptr(srcpos), _ptr(srcdir), float(wavenum), float(scale), x_desc.ptr, _ptr(x),
+            _ptr(dstpos), _ptr(offsets), _ptr(weights),
+            int(sparse_indices.shape[0]) if sparse_indices is not None else 0,
+            _ptr(sparse_indices), o_desc.ptr, _ptr(out),
+            gx_desc.ptr if gx_desc is not None else None, _ptr(gx), int(ct), int(algorithm),
+            int(reps), ctypes.byref(res))
+        check(rc, "ffdas_das_autotune")
+    return res.as_dict()
+
+
+def save_tuning(path) -> None:
+    """Write the tuning cache to a text file (one line per problem signature)."""
+    check(lib().ffdas_tune_save(str(path).encode()), "ffdas_tune_save")
+
+
+def load_tuning(path) -> None:
+    """Merge a tuning file written by :func:`save_tuning` into the cache."""
+    check(lib().ffdas_tune_load(str(path).encode()), "ffdas_tune_load")
+
+
+def clear_tuning() -> None:
+    check(lib().ffdas_tune_clear(), "ffdas_tune_clear")
+
+
+def tuning_entries() -> int:
+    return int(lib().ffdas_tune_count())

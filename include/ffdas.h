@@ -226,11 +226,51 @@ FFDAS_API ffdas_error_t ffdas_das_validate(ffdas_tensor_desc_t x_desc,
                                            const int *sparse_indices,
                                            ffdas_compute_type_t compute_type, ffdas_alg_t alg);
 
+/* Launch parameters chosen by ffdas_das_autotune, with the measured times [ms] of the
+ * chosen and of the built-in (heuristic) parameters.  0 means "heuristic" for a field. */
+typedef struct ffdas_das_tuning {
+    int groups;            /* waves per voxel tile splitting the channels (forward) */
+    int fi_frames;         /* frames per frame-inner chunk of x */
+    int tile_h, tile_w;    /* 2-D wave tile of voxels (tile_h * tile_w = 64) */
+    int param_groups;      /* waves per voxel tile in the offsets/weights/scale adjoint */
+    int bwdx_frames;       /* frames per grad_x workgroup */
+    int bwdx_depth_lanes;  /* lanes of a wave along the first grid dimension in grad_x */
+    int bwdx_threads;      /* grad_x workgroup size */
+    float forward_ms, backward_ms;
+    float default_forward_ms, default_backward_ms;
+    int candidates;        /* configurations timed */
+} ffdas_das_tuning_t;
+
+/* Autotuner (no reference counterpart; the reference picks its tile width from a fixed
+ * heuristic or an environment variable, src/das/das_alg1.cu:209-225).  Times candidate
+ * launch parameters of ffdas_das_ex (and, when grad_x != NULL, of ffdas_das_backward) on the
+ * handle's stream for exactly this problem signature -- device architecture, storage types,
+ * channels, transmits, samples, frames, voxel grid, sparse count, directivity -- and stores
+ * the fastest in the library's tuning cache, which every later call with the same signature
+ * uses.  `out` and `grad_x` are overwritten (scratch for the timed runs).  Synchronous: it
+ * waits for the timed runs.  reps: timed runs per candidate (1..100). */
+FFDAS_API ffdas_error_t ffdas_das_autotune(ffdas_handle_t handle, const float *srcpos,
+                                           const float *srcdir, float wavenum, float scale,
+                                           ffdas_tensor_desc_t x_desc, const void *x,
+                                           const float *dstpos, const float *offsets,
+                                           const float *weights, int sparse_count,
+                                           const int *sparse_indices,
+                                           ffdas_tensor_desc_t out_desc, void *out,
+                                           ffdas_tensor_desc_t grad_x_desc, void *grad_x,
+                                           ffdas_compute_type_t compute_type, ffdas_alg_t alg,
+                                           int reps, ffdas_das_tuning_t *result);
+
+/* Tuning cache persistence: one text line per problem signature. */
+FFDAS_API ffdas_error_t ffdas_tune_save(const char *path);
+FFDAS_API ffdas_error_t ffdas_tune_load(const char *path);
+FFDAS_API ffdas_error_t ffdas_tune_clear(void);
+FFDAS_API int ffdas_tune_count(void);
+
 /* Opt-in kernel timing: while enabled, every kernel the library launches is bracketed by
  * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
- * "das_bwd_x", "relayout_fi", "cast_rows", "transpose_gout", "absmax").  Counterpart of the
- * reference's NVTX ranges (no reference symbol). */
+ * "das_bwd_x", "das_bwd_x_plain", "relayout_fi", "cast_rows", "transpose_gout").
+ * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);
 FFDAS_API ffdas_error_t ffdas_profile_query(const char *name, double *total_ms, int64_t *count);

@@ -165,3 +165,30 @@ def test_validate_extensions():
     # a whole tensor beyond 2^31 elements is fine when one frame is not (64-bit frame offsets)
     assert validate((300, 1024, 64, 128), (300, 10), DType.R_16F) == 0
     assert L().ffdas_das_validate(None, None, NZ, NZ, NZ, NZ, 0, None, 0, 0) == E["NULL_POINTER"]
+
+
+def test_tuning_cache_roundtrip(tmp_path):
+    """ffdas_tune_load / _save / _clear / _count are host-only: a file written by hand is
+    loaded, saved back unchanged, and rejected when malformed."""
+    import ffdas_amd
+    ffdas_amd.clear_tuning()
+    assert ffdas_amd.tuning_entries() == 0
+    src = tmp_path / "tune.txt"
+    lines = ["gfx950|x=f32|k=f16|c=0|a64=0|sp=0|C=128|T=11|S=2048|B=16|N=256x256|dir=0 "
+             "8 16 8 8 4 16 4 1024",
+             "gfx950|x=f32|k=f32|c=1|a64=0|sp=8|C=64|T=64|S=2048|B=200|N=128x128|dir=1 "
+             "4 8 16 4 2 8 3 512"]
+    src.write_text("# comment\n" + "\n".join(lines) + "\n")
+    ffdas_amd.load_tuning(src)
+    assert ffdas_amd.tuning_entries() == 2
+    out = tmp_path / "out.txt"
+    ffdas_amd.save_tuning(out)
+    assert sorted(out.read_text().split("\n")[:-1]) == sorted(lines)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("key 1 2 3\n")
+    with pytest.raises(ffdas_amd.FfdasError):
+        ffdas_amd.load_tuning(bad)
+    with pytest.raises(ffdas_amd.FfdasError):
+        ffdas_amd.load_tuning(tmp_path / "missing.txt")
+    ffdas_amd.clear_tuning()
+    assert ffdas_amd.tuning_entries() == 0
