@@ -47,6 +47,8 @@ def parse():
                     help="voxel rows (of 256) in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the built-in launch heuristics instead of autotuned parameters")
     ap.add_argument("--profile-only", type=int, default=0, metavar="N",
                     help="run N steps and a 256 MiB calibration copy, print no JSON")
     return ap.parse_args()
@@ -79,6 +81,13 @@ def step(p, x, scale=1.0, algorithm=None):
     gx, go, _, gs = ffdas_amd.das_backward(x, p.srcpos, p.dstpos, p.offsets, p.weights, g,
                                            scale=scale, need_weights=False, algorithm=alg)
     return out, gx, go, gs
+
+
+def tune(p, x):
+    """Autotune the forward and adjoint launch parameters for this problem (kept in the
+    library's tuning cache for the timed steps)."""
+    import ffdas_amd
+    return ffdas_amd.autotune(x, p.srcpos, p.dstpos, p.offsets, p.weights, reps=3)
 
 
 def time_steps(fn, steps, warmup):
@@ -199,6 +208,13 @@ def main():
         f"{w.dstpos.shape[:-1]}, U/step {w.units:.4g}")
     p = Problem(w, dev, a.storage)
     U = w.units
+    tuned = None
+    if not a.no_autotune:
+        t0 = time.time()
+        tuned = tune(p, p.x)
+        if a.storage != "fp32":
+            tune(p, p.x32)
+        log(f"autotuned in {time.time() - t0:.1f}s: {tuned}")
     if a.profile_only:
         profile_only(p, p.x, a.profile_only)
         return
@@ -253,6 +269,8 @@ def main():
                    "ratio_vs_fp32_plain": round(value / v32, 3),
                    "ratio_vs_fp32_tuned": round(value / v32o, 3),
                    "rel_l2_vs_fp64": acc[0], "rel_l2_grad_x_vs_fp64": acc[1],
+                   "autotuned": None if tuned is None else
+                   {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()},
                    "wall_s": round(wall, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -7,8 +7,8 @@ from ._core import (TensorView, bfloat16, bfloat162, device_count, device_get, d
                     double2, float2, float32, float64, get_library_handle, half, half2, view)
 from ._lib import Algorithm, ComputeType, DType, FfdasError
 from .autograd import DasFunction, das_autograd
-from .das import (DowncastWarning, autotune, clear_tuning, das, das_backward, das_sparse,
-                  load_tuning, save_tuning, tuning_entries)
+from .das import (CapturedCall, DowncastWarning, autotune, clear_tuning, das, das_backward,
+                  das_batched, das_sparse, load_tuning, save_tuning, tuning_entries)
 
 __version__ = "0.1.0"
 
@@ -17,5 +17,6 @@ __all__ = [
     "ComputeType", "DType", "FfdasError", "DowncastWarning", "TensorView", "view", "half",
     "half2", "bfloat16", "bfloat162", "float32", "float64", "float2", "double2",
     "get_library_handle", "device_count", "device_get", "device_set", "autotune",
-    "save_tuning", "load_tuning", "clear_tuning", "tuning_entries",
+    "save_tuning", "load_tuning", "clear_tuning", "tuning_entries", "das_batched",
+    "CapturedCall",
 ]

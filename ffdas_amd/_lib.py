@@ -59,6 +59,17 @@ class Algorithm(enum.IntEnum):
     ALG4 = 4
 
 
+class DasProblem(ctypes.Structure):
+    """ffdas_das_problem_t (include/ffdas.h)."""
+    _fields_ = [("srcpos", ctypes.c_void_p), ("srcdir", ctypes.c_void_p),
+                ("wavenum", ctypes.c_float), ("scale", ctypes.c_float),
+                ("x_desc", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("dstpos", ctypes.c_void_p), ("offsets", ctypes.c_void_p),
+                ("weights", ctypes.c_void_p), ("sparse_count", ctypes.c_int),
+                ("sparse_indices", ctypes.c_void_p), ("beta", ctypes.c_void_p),
+                ("out_desc", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
 class DasTuning(ctypes.Structure):
     """ffdas_das_tuning_t (include/ffdas.h)."""
     _fields_ = [("groups", ctypes.c_int), ("fi_frames", ctypes.c_int),
@@ -124,6 +135,7 @@ def lib():
         "ffdas_event_elapsed_time": (i32, [u, u, P(f32)]),
         "ffdas_das_autotune": (i32, [vp, vp, vp, f32, f32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,
                                      vp, i32, i32, i32, P(DasTuning)]),
+        "ffdas_das_batched": (i32, [vp, i32, P(DasProblem), i32, i32]),
         "ffdas_tune_save": (i32, [ctypes.c_char_p]),
         "ffdas_tune_load": (i32, [ctypes.c_char_p]),
         "ffdas_tune_clear": (i32, []),

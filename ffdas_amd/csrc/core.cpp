@@ -34,6 +34,16 @@ ffdas_error_t ffdas_context::workspace(size_t bytes, void **ptr) {
         *ptr = nullptr;
         return FFDAS_SUCCESS;
     }
+    // Under stream capture nothing may allocate or synchronise: the workspace must already
+    // be large enough (one call of the same shape before capturing).
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) cs = hipStreamCaptureStatusNone;
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (capturing) {
+        if (bytes > ws_bytes) return FFDAS_ERROR_ALLOCATION_FAILED;
+        *ptr = ws;
+        return FFDAS_SUCCESS;
+    }
     if (ws && ws_stream != stream) {
         // the previous owner stream may still read the workspace
         if (hipStreamSynchronize(ws_stream) != hipSuccess) return FFDAS_ERROR_CUDA_SYNC_FAILED;
@@ -41,10 +51,9 @@ ffdas_error_t ffdas_context::workspace(size_t bytes, void **ptr) {
     }
     if (bytes > ws_bytes) {
         if (ws) {
-            hipError_t e = hipFreeAsync(ws, ws_stream);
+            retired.push_back(ws);
             ws = nullptr;
             ws_bytes = 0;
-            if (e != hipSuccess) return hip_status(e);
         }
         size_t want = bytes + bytes / 4;  // headroom against repeated growth
         want = (want + 4095) & ~size_t(4095);
@@ -181,9 +190,10 @@ ffdas_error_t ffdas_create(ffdas_handle_t *handle) {
 ffdas_error_t ffdas_destroy(ffdas_handle_t h) {
     if (!h) return FFDAS_ERROR_INVALID_HANDLE;
     ffdas_error_t err = FFDAS_SUCCESS;
-    if (h->ws) {
+    if (h->ws) h->retired.push_back(h->ws);
+    for (void *p : h->retired) {
         // release in stream order; if the runtime is already torn down, just leak
-        hipError_t e = hipFreeAsync(h->ws, h->ws_stream);
+        hipError_t e = hipFreeAsync(p, h->ws_stream);
         if (e != hipSuccess && e != hipErrorDeinitialized && e != hipErrorContextIsDestroyed)
             err = FFDAS_ERROR_CUDA_RUNTIME;
     }

@@ -175,16 +175,19 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
     const size_t xb = cast_bytes(p);
     const size_t gb = grad_x ? align_up(backward_workspace(p)) : 0;
     if (grad_x && gb == 0 && !p.plain) return FFDAS_ERROR_DIMS_TOO_LARGE;   // row exceeds LDS
+    const size_t pb = align_up(param_workspace(p, grad_offsets != nullptr,
+                                               grad_weights != nullptr, grad_scale != nullptr));
     void *ws = nullptr;
-    FFDAS_TRY(handle->workspace(xb + gb, &ws));
+    FFDAS_TRY(handle->workspace(xb + gb + pb, &ws));
     XView xv;
     FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
     void *gt = grad_x ? static_cast<char *>(ws) + xb : nullptr;
+    void *pws = static_cast<char *>(ws) + xb + gb;
     DasGeomArgs a;
     a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
     a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
     return launch_backward(*handle, p, a, xv, grad_out, p.os_b, grad_x, grad_offsets,
-                           grad_weights, grad_scale, gt);
+                           grad_weights, grad_scale, gt, pws);
 }
 
 ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
@@ -232,6 +235,71 @@ ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// batched forward
+// ---------------------------------------------------------------------------------------
+extern "C" ffdas_error_t ffdas_das_batched(ffdas_handle_t handle, int count,
+                                          const ffdas_das_problem_t *problems,
+                                          ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    FFDAS_TRY(handle->check_device());
+    if (count < 0) return FFDAS_ERROR_INVALID_ARGUMENT;
+    if (count == 0) return FFDAS_SUCCESS;
+    if (!problems) return FFDAS_ERROR_NULL_POINTER;
+    std::vector<BatchMember> ms(count);
+    std::vector<size_t> wsoff(count);
+    size_t total = 0;
+    for (int i = 0; i < count; i++) {
+        const ffdas_das_problem_t &q = problems[i];
+        if (!q.x_desc || !q.out_desc) return FFDAS_ERROR_NULL_POINTER;
+        if (!q.sparse_indices && q.sparse_count != 0) return FFDAS_ERROR_INVALID_ARGUMENT;
+        BatchMember &m = ms[i];
+        FFDAS_TRY(plan_das(*q.x_desc, *q.out_desc, q.srcpos, q.dstpos, q.offsets, q.weights,
+                           q.sparse_indices != nullptr, q.sparse_count, q.sparse_indices,
+                           compute_type, alg, m.p));
+        resolve_tune(*handle, m.p, q.srcdir != nullptr);
+        m.p.fi_frames = fi_frames_for(m.p, alg);
+        if (!q.x || !q.out) return FFDAS_ERROR_NULL_POINTER;
+        if (!(q.scale > 0.0f) || !std::isfinite(q.scale)) return FFDAS_ERROR_INVALID_ARGUMENT;
+        read_beta(q.beta, m.p, m.beta_re, m.beta_im);
+        m.a.srcpos = q.srcpos; m.a.srcdir = q.srcdir; m.a.dstpos = q.dstpos;
+        m.a.offsets = q.offsets; m.a.weights = q.weights; m.a.sparse_indices = q.sparse_indices;
+        m.a.wavenum = q.wavenum; m.a.scale = q.scale;
+        m.out = q.out;
+        wsoff[i] = total;
+        total += cast_bytes(m.p);
+    }
+    void *ws = nullptr;
+    FFDAS_TRY(handle->workspace(total, &ws));
+    // converted copies of x, then one gather launch per (storage, complex, chunk) group
+    std::vector<XView> xv(count);
+    for (int i = 0; i < count; i++) {
+        FFDAS_TRY(prepare_x(*handle, ms[i].p, problems[i].x,
+                            static_cast<char *>(ws) + wsoff[i], xv[i]));
+        ms[i].xfi = xv[i].ptr;
+    }
+    std::vector<bool> done(count, false);
+    for (int i = 0; i < count; i++) {
+        if (done[i]) continue;
+        if (ms[i].p.fi_frames <= 0) {
+            FFDAS_TRY(launch_forward(*handle, ms[i].p, ms[i].a, xv[i], ms[i].out, ms[i].beta_re,
+                                     ms[i].beta_im));
+            done[i] = true;
+            continue;
+        }
+        std::vector<BatchMember> group;
+        for (int j = i; j < count; j++) {
+            if (done[j] || ms[j].p.fi_frames != ms[i].p.fi_frames ||
+                ms[j].p.k_storage != ms[i].p.k_storage || ms[j].p.cplx != ms[i].p.cplx)
+                continue;
+            group.push_back(ms[j]);
+            done[j] = true;
+        }
+        FFDAS_TRY(launch_forward_batched(*handle, group));
+    }
+    return FFDAS_SUCCESS;
+}
 
 // ---------------------------------------------------------------------------------------
 // autotuner

@@ -233,24 +233,23 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
     store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, b0, g.B, beta_re, beta_im);
 }
 
-// frame-inner layout.  A workgroup of G waves shares one voxel tile (blockIdx.x); wave w
-// sums channels w, w+G, ... and the G partial sums are combined through LDS, so a launch
-// has G times more waves in flight than tiles (latency hiding for the gathers) without
-// repeating any delay arithmetic.  grid.y = frame chunk.
+// frame-inner layout.  A workgroup of G waves shares one voxel tile; wave w sums channels
+// w, w+G, ... and the G partial sums are combined through LDS, so a launch has G times more
+// waves in flight than tiles (latency hiding for the gathers) without repeating any delay
+// arithmetic.
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
-__global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
-                                                         void *__restrict__ outv, int64_t os_b,
-                                                         double beta_re, double beta_im) {
+__device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, const void *xv,
+                                            void *outv, int64_t os_b, double beta_re,
+                                            double beta_im, int tile, int chunk,
+                                            unsigned char *lds) {
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
-    extern __shared__ unsigned char fwd_lds[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int G = blockDim.x >> 6;
     bool valid;
-    int n = voxel_of(tl, blockIdx.x, lane, valid);
+    int n = voxel_of(tl, tile, lane, valid);
     if (!valid) n = 0;          // computed, never stored: keeps every lane at the barrier
-    const int chunk = blockIdx.y;
     const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
     A acc[FT][CW];
     das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, wave, G, acc);
@@ -259,7 +258,7 @@ __global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const
         return;
     }
     // red[w][i][lane], i = f * CW + c
-    A *red = reinterpret_cast<A *>(fwd_lds);
+    A *red = reinterpret_cast<A *>(lds);
 #pragma unroll
     for (int f = 0; f < FT; f++)
 #pragma unroll
@@ -292,6 +291,49 @@ __global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const
     }
 }
 
+// grid.x = voxel tile, grid.y = frame chunk
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+__global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
+                                                         void *__restrict__ outv, int64_t os_b,
+                                                         double beta_re, double beta_im) {
+    extern __shared__ unsigned char fwd_lds[];
+    fwd_fi_tile<ST, ACC64, CPLX, PRECISE, FT, KC>(g, tl, xv, outv, os_b, beta_re, beta_im,
+                                                   blockIdx.x, blockIdx.y, fwd_lds);
+}
+
+// Several problems (mixed probes, grids, frame counts, sparse tables) in one launch.  The
+// problem descriptors travel in the kernel arguments, so a captured graph holds them by
+// value.  Block b belongs to the problem whose [start, start + tiles * chunks) contains it.
+constexpr int kMaxBatch = 12;
+struct BatchItem {
+    Geo g;
+    Tiling tl;
+    const void *x;        // frame-inner copy of the problem's x
+    void *out;
+    int64_t os_b;
+    double beta_re, beta_im;
+    int start;            // first block of the problem
+    int tiles;            // voxel tiles per frame chunk
+};
+struct BatchArgs {
+    int count;
+    BatchItem item[kMaxBatch];
+};
+
+template <Storage ST, bool CPLX, bool PRECISE, int FT, int KC>
+__global__ __launch_bounds__(512) void das_fwd_fi_batched_kernel(BatchArgs args) {
+    extern __shared__ unsigned char fwd_lds[];
+    const int b = blockIdx.x;
+    int i = 0;
+    while (i + 1 < args.count && b >= args.item[i + 1].start) i++;
+    const BatchItem &it = args.item[i];
+    const int local = b - it.start;
+    const int chunk = local / it.tiles;
+    fwd_fi_tile<ST, false, CPLX, PRECISE, FT, KC>(it.g, it.tl, it.x, it.out, it.os_b, it.beta_re,
+                                                   it.beta_im, local - chunk * it.tiles, chunk,
+                                                   fwd_lds);
+}
+
 // ---------------------------------------------------------------------------------------
 // adjoint: offsets / weights / scale
 // ---------------------------------------------------------------------------------------
@@ -305,7 +347,7 @@ __device__ __forceinline__ AccT<ACC64> wave_sum(AccT<ACC64> v) {
 template <class Rows, Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
 __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Rows &xr, int n,
                                                           int b0, const void *goutv, int64_t gs_b,
-                                                          float *gofs, float *gwt, int accumulate,
+                                                          float *gofs, float *gwt,
                                                           int m0, int mstep, bool valid,
                                                           AccT<ACC64> *red) {
     using A = AccT<ACC64>;
@@ -415,9 +457,7 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
                 float *dst = (i & 1) ? gwt : gofs;
                 if (!valid || jj >= g.K || !dst) continue;
                 const int64_t o = (int64_t)jj * g.N + n;
-                const float v = (i & 1) ? (float)sum : (float)(sum * (A)g.scale);
-                if (accumulate) atomicAdd(dst + o, v);
-                else dst[o] = v;
+                dst[o] = (i & 1) ? (float)sum : (float)(sum * (A)g.scale);
             }
             __syncthreads();
             continue;
@@ -427,22 +467,17 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
             const int jj = kc + j;
             if (jj >= g.K || !valid) break;
             const int64_t o = (int64_t)jj * g.N + n;
-            const float vo = (float)(d_tau[j] * (A)g.scale);
-            const float vw = (float)d_w[j];
-            if (accumulate) {
-                if (gofs) atomicAdd(gofs + o, vo);
-                if (gwt) atomicAdd(gwt + o, vw);
-            } else {
-                if (gofs) gofs[o] = vo;
-                if (gwt) gwt[o] = vw;
-            }
+            if (gofs) gofs[o] = (float)(d_tau[j] * (A)g.scale);
+            if (gwt) gwt[o] = (float)d_w[j];
         }
     }
     return scale_part;
 }
 
+// The block's part of d/d scale, stored at its block index (summed later in a fixed order,
+// so the result does not depend on block scheduling).
 template <bool ACC64>
-__device__ __forceinline__ void block_scale_sum(AccT<ACC64> part, float *gscale) {
+__device__ __forceinline__ void block_scale_store(AccT<ACC64> part, double *partials) {
     using A = AccT<ACC64>;
     __shared__ A red[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -450,17 +485,43 @@ __device__ __forceinline__ void block_scale_sum(AccT<ACC64> part, float *gscale)
     if (lane == 0) red[wave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        A s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
-        atomicAdd(gscale, (float)s);
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += (double)red[w];
+        partials[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
+}
+
+// dst[i] = sum over c of part[c * n + i], c in order (frame-chunk partial gradients).
+__global__ void sum_chunks_kernel(const float *__restrict__ part, int chunks, int64_t n,
+                                  float *__restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int c = 0; c < chunks; c++) s += part[(int64_t)c * n + i];
+        dst[i] = s;
+    }
+}
+
+// *out = sum of n doubles in a fixed order (one block of 256 threads).
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restrict__ part, int n,
+                                                           float *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = (float)red[0];
 }
 
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
 __global__ __launch_bounds__(256) void das_bwd_param_kernel(
     Geo g, Tiling tl, const void *__restrict__ xv, int64_t xs_b, int64_t xs_c, int64_t xs_t,
     const void *__restrict__ goutv, int64_t gs_b, float *__restrict__ gofs,
-    float *__restrict__ gwt, float *__restrict__ gscale, int accumulate) {
+    float *__restrict__ gwt, int64_t pstride, double *__restrict__ scale_part) {
     constexpr int CW = CPLX ? 2 : 1;
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -470,17 +531,21 @@ __global__ __launch_bounds__(256) void das_bwd_param_kernel(
     if (valid) {
         const int b0 = blockIdx.y * FT;
         const NaturalRows<ST, FT, CW> xr(xv, xs_b, xs_c, xs_t, b0, g.B);
+        // one frame chunk per grid row; its partial gradients go to their own slice
+        float *go = gofs ? gofs + blockIdx.y * pstride : nullptr;
+        float *gw = gwt ? gwt + blockIdx.y * pstride : nullptr;
         part = das_bwd_param_body<NaturalRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
-            g, xr, n, b0, goutv, gs_b, gofs, gwt, accumulate, 0, 1, true, nullptr);
+            g, xr, n, b0, goutv, gs_b, go, gw, 0, 1, true, nullptr);
     }
-    if (gscale) block_scale_sum<ACC64>(part, gscale);
+    if (scale_part) block_scale_store<ACC64>(part, scale_part);
 }
 
 // Frame-inner layout, G waves per voxel tile splitting the channels (as the forward).
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
 __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     Geo g, Tiling tl, const void *__restrict__ xv, const void *__restrict__ goutv, int64_t gs_b,
-    float *__restrict__ gofs, float *__restrict__ gwt, float *__restrict__ gscale, int accumulate) {
+    float *__restrict__ gofs, float *__restrict__ gwt, int64_t pstride,
+    double *__restrict__ scale_part) {
     constexpr int CW = CPLX ? 2 : 1;
     extern __shared__ unsigned char param_lds[];
     const int lane = threadIdx.x & 63;
@@ -491,11 +556,13 @@ __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     if (!valid) n = 0;          // computed, never stored: keeps every lane at the barriers
     const int chunk = blockIdx.y;
     const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
+    float *go = gofs ? gofs + chunk * pstride : nullptr;
+    float *gw = gwt ? gwt + chunk * pstride : nullptr;
     AccT<ACC64> part = das_bwd_param_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
-        g, xr, n, chunk * FT, goutv, gs_b, gofs, gwt, accumulate, wave, G, valid,
+        g, xr, n, chunk * FT, goutv, gs_b, go, gw, wave, G, valid,
         reinterpret_cast<AccT<ACC64> *>(param_lds));
     if (!valid) part = 0;
-    if (gscale) block_scale_sum<ACC64>(part, gscale);
+    if (scale_part) block_scale_store<ACC64>(part, scale_part);
 }
 
 // Relayout + cast into the frame-inner layout: dst[chunk][c][t][k][f] = src[chunk*FT+f][c][t][k]
@@ -1401,29 +1468,53 @@ hipError_t launch_fwd_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl
     }
 }
 
+// Final and partial buffers of the parameter adjoint: with several frame chunks each chunk
+// writes its own slice of pofs / pwt, and every block its d/d scale part to spart; the
+// sums are then formed in a fixed order.
+struct ParamOut {
+    float *gofs = nullptr, *gwt = nullptr, *gscale = nullptr;
+    float *pofs = nullptr, *pwt = nullptr;
+    double *spart = nullptr;
+};
+
+hipError_t finish_params(hipStream_t st, const ParamOut &po, int chunks, int64_t kn, int blocks) {
+    if (chunks > 1) {
+        const int grid = (int)std::min<int64_t>((kn + 255) / 256, 4096);
+        if (po.gofs) dev::sum_chunks_kernel<<<grid, 256, 0, st>>>(po.pofs, chunks, kn, po.gofs);
+        if (po.gwt) dev::sum_chunks_kernel<<<grid, 256, 0, st>>>(po.pwt, chunks, kn, po.gwt);
+    }
+    if (po.gscale) dev::sum_partials_kernel<<<1, 256, 0, st>>>(po.spart, blocks, po.gscale);
+    return hipGetLastError();
+}
+
 template <Storage ST, bool ACC64, bool CPLX, int FT>
 hipError_t launch_bwdp_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
-                          const XView &xv, const void *gout, int64_t gs_b, float *gofs,
-                          float *gwt, float *gscale) {
+                          const XView &xv, const void *gout, int64_t gs_b, const ParamOut &po) {
     constexpr bool PRECISE = (ST == Storage::F32 || ST == Storage::F64);
     const int ntiles = tl.nta * tl.nth * tl.ntw;
     dim3 grid((ntiles + 3) / 4, (g.B + FT - 1) / FT);
+    const bool multi = grid.y > 1;
+    const int64_t kn = (int64_t)g.K * g.N;
+    {
     ProfScope ps(st, "das_bwd_param");
     dev::das_bwd_param_kernel<ST, ACC64, CPLX, PRECISE, FT, kKC><<<grid, 256, 0, st>>>(
-        g, tl, xv.ptr, xv.sb, xv.sc, xv.st, gout, gs_b, gofs, gwt, gscale, grid.y > 1 ? 1 : 0);
-    return hipGetLastError();
+        g, tl, xv.ptr, xv.sb, xv.sc, xv.st, gout, gs_b, multi ? po.pofs : po.gofs,
+        multi ? po.pwt : po.gwt, multi ? kn : 0, po.gscale ? po.spart : nullptr);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return finish_params(st, po, (int)grid.y, kn, (int)(grid.x * grid.y));
 }
 
 template <Storage ST, bool ACC64, bool CPLX>
 hipError_t launch_bwdp_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
-                         const XView &xv, const void *gout, int64_t gs_b, float *gofs,
-                         float *gwt, float *gscale) {
+                         const XView &xv, const void *gout, int64_t gs_b, const ParamOut &po) {
     switch (frame_tile_for(g.B)) {
-        case 16: return launch_bwdp_ft<ST, ACC64, CPLX, 16>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale);
-        case 8: return launch_bwdp_ft<ST, ACC64, CPLX, 8>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale);
-        case 4: return launch_bwdp_ft<ST, ACC64, CPLX, 4>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale);
-        case 2: return launch_bwdp_ft<ST, ACC64, CPLX, 2>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale);
-        default: return launch_bwdp_ft<ST, ACC64, CPLX, 1>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale);
+        case 16: return launch_bwdp_ft<ST, ACC64, CPLX, 16>(st, g, tl, xv, gout, gs_b, po);
+        case 8: return launch_bwdp_ft<ST, ACC64, CPLX, 8>(st, g, tl, xv, gout, gs_b, po);
+        case 4: return launch_bwdp_ft<ST, ACC64, CPLX, 4>(st, g, tl, xv, gout, gs_b, po);
+        case 2: return launch_bwdp_ft<ST, ACC64, CPLX, 2>(st, g, tl, xv, gout, gs_b, po);
+        default: return launch_bwdp_ft<ST, ACC64, CPLX, 1>(st, g, tl, xv, gout, gs_b, po);
     }
 }
 
@@ -1552,8 +1643,7 @@ hipError_t launch_fwd_fi_t(hipStream_t st, const dev::Geo &g, const dev::Tiling 
 
 template <Storage ST, bool CPLX, int F>
 hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int groups,
-                            const void *x, const void *gout, int64_t gs_b, float *gofs,
-                            float *gwt, float *gscale) {
+                            const void *x, const void *gout, int64_t gs_b, const ParamOut &po) {
     if constexpr (!fi_ok<ST, CPLX, F>()) {
         return hipErrorInvalidValue;
     } else {
@@ -1567,19 +1657,83 @@ hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         ProfScope ps(st, "das_bwd_param_fi");
         dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 64 * G, lds, st>>>(
-            g, tl, x, gout, gs_b, gofs, gwt, gscale, grid.y > 1 ? 1 : 0);
-        return hipGetLastError();
+            g, tl, x, gout, gs_b, grid.y > 1 ? po.pofs : po.gofs, grid.y > 1 ? po.pwt : po.gwt,
+            grid.y > 1 ? (int64_t)g.K * g.N : 0, po.gscale ? po.spart : nullptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return finish_params(st, po, (int)grid.y, (int64_t)g.K * g.N, (int)(grid.x * grid.y));
     }
 }
 
 template <Storage ST, bool CPLX>
 hipError_t launch_bwdp_fi_t(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, int F,
                             int groups, const void *x, const void *gout, int64_t gs_b,
-                            float *gofs, float *gwt, float *gscale) {
+                            const ParamOut &po) {
     switch (F) {
-        case 16: return launch_bwdp_fi_f<ST, CPLX, 16>(st, g, tl, groups, x, gout, gs_b, gofs, gwt, gscale);
-        case 8: return launch_bwdp_fi_f<ST, CPLX, 8>(st, g, tl, groups, x, gout, gs_b, gofs, gwt, gscale);
-        case 4: return launch_bwdp_fi_f<ST, CPLX, 4>(st, g, tl, groups, x, gout, gs_b, gofs, gwt, gscale);
+        case 16: return launch_bwdp_fi_f<ST, CPLX, 16>(st, g, tl, groups, x, gout, gs_b, po);
+        case 8: return launch_bwdp_fi_f<ST, CPLX, 8>(st, g, tl, groups, x, gout, gs_b, po);
+        case 4: return launch_bwdp_fi_f<ST, CPLX, 4>(st, g, tl, groups, x, gout, gs_b, po);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <Storage ST, bool CPLX, int F>
+hipError_t launch_fwd_batched_f(hipStream_t st, const std::vector<BatchMember> &ms) {
+    if constexpr (!fi_ok<ST, CPLX, F>()) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr bool PRECISE = (ST == Storage::F32);
+        for (size_t off = 0; off < ms.size(); off += dev::kMaxBatch) {
+            dev::BatchArgs args{};
+            const int n = (int)std::min(ms.size() - off, (size_t)dev::kMaxBatch);
+            int64_t start = 0;
+            int minC = 1 << 30;
+            int want = 0;
+            for (int i = 0; i < n; i++) {
+                const BatchMember &m = ms[off + i];
+                dev::BatchItem &it = args.item[i];
+                it.g = make_geo(m.a, m.p);
+                it.tl = make_tiling(m.p.spatial, m.p.tune);
+                const int tiles = it.tl.nta * it.tl.nth * it.tl.ntw;
+                const int chunks = (m.p.B + F - 1) / F;
+                it.x = m.xfi;
+                it.out = m.out;
+                it.os_b = m.p.os_b;
+                it.beta_re = m.beta_re;
+                it.beta_im = m.beta_im;
+                it.start = (int)start;
+                it.tiles = tiles;
+                start += (int64_t)tiles * chunks;
+                minC = std::min(minC, m.p.C);
+                if (i == 0) want = m.p.tune.groups;
+            }
+            args.count = n;
+            if (start == 0) continue;
+            if (start > 0x7fffffff) return hipErrorInvalidValue;
+            dev::Geo gc{};
+            gc.C = minC;
+            const int G = channel_groups(gc, start, want);
+            const size_t lds = G > 1 ? (size_t)G * F * (CPLX ? 2 : 1) * 64 * sizeof(float) : 0;
+            if (lds > 0)
+                (void)hipFuncSetAttribute(
+                    (const void *)dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC>,
+                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            ProfScope ps(st, "das_fwd_fi_batched");
+            dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC>
+                <<<(unsigned)start, 64 * G, lds, st>>>(args);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+}
+
+template <Storage ST, bool CPLX>
+hipError_t launch_fwd_batched_t(hipStream_t st, const std::vector<BatchMember> &ms, int F) {
+    switch (F) {
+        case 16: return launch_fwd_batched_f<ST, CPLX, 16>(st, ms);
+        case 8: return launch_fwd_batched_f<ST, CPLX, 8>(st, ms);
+        case 4: return launch_fwd_batched_f<ST, CPLX, 4>(st, ms);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1673,6 +1827,28 @@ hipError_t launch_bwdx_sorted_t(hipStream_t st, const dev::Geo &g, const void *g
 }
 
 }  // namespace
+
+ffdas_error_t launch_forward_batched(ffdas_context &h, const std::vector<BatchMember> &ms) {
+    if (ms.empty()) return FFDAS_SUCCESS;
+    const DasProblem &p = ms[0].p;
+    hipError_t e = hipErrorInvalidValue;
+    switch (p.k_storage) {
+        case Storage::F16:
+            e = p.cplx ? launch_fwd_batched_t<Storage::F16, true>(h.stream, ms, p.fi_frames)
+                       : launch_fwd_batched_t<Storage::F16, false>(h.stream, ms, p.fi_frames);
+            break;
+        case Storage::BF16:
+            e = p.cplx ? launch_fwd_batched_t<Storage::BF16, true>(h.stream, ms, p.fi_frames)
+                       : launch_fwd_batched_t<Storage::BF16, false>(h.stream, ms, p.fi_frames);
+            break;
+        case Storage::F32:
+            e = p.cplx ? launch_fwd_batched_t<Storage::F32, true>(h.stream, ms, p.fi_frames)
+                       : launch_fwd_batched_t<Storage::F32, false>(h.stream, ms, p.fi_frames);
+            break;
+        default: return FFDAS_ERROR_INTERNAL;
+    }
+    return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
 
 ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
                              const XView &xv, void *out, double beta_re, double beta_im) {
@@ -1780,9 +1956,55 @@ size_t backward_workspace(const DasProblem &p) {
     return 256 + nchunks * (size_t)fs * (size_t)p.N * esz + perm;
 }
 
+// Partial buffers of the parameter adjoint (see ParamOut), carved from `pws`; the layout
+// matches param_workspace().
+namespace {
+struct ParamLayout {
+    int chunks = 1;
+    int blocks = 0;
+    size_t ofs_bytes = 0, wt_bytes = 0, scale_bytes = 0;
+};
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+ParamLayout param_layout(const DasProblem &p, bool has_o, bool has_w, bool has_s) {
+    ParamLayout L;
+    const dev::Tiling tl = make_tiling(p.spatial, p.tune);
+    const int ntiles = tl.nta * tl.nth * tl.ntw;
+    const int FT = p.fi_frames > 0 ? p.fi_frames : frame_tile_for(p.B);
+    L.chunks = (p.B + FT - 1) / FT;
+    L.blocks = (p.fi_frames > 0 ? ntiles : (ntiles + 3) / 4) * L.chunks;
+    const size_t slice = (size_t)L.chunks * (size_t)p.K * (size_t)p.N * sizeof(float);
+    if (L.chunks > 1) {
+        L.ofs_bytes = has_o ? align256(slice) : 0;
+        L.wt_bytes = has_w ? align256(slice) : 0;
+    }
+    L.scale_bytes = has_s ? align256((size_t)L.blocks * sizeof(double)) : 0;
+    return L;
+}
+ParamOut param_out(const DasProblem &p, float *gofs, float *gwt, float *gscale, void *pws) {
+    const ParamLayout L = param_layout(p, gofs != nullptr, gwt != nullptr, gscale != nullptr);
+    ParamOut po;
+    po.gofs = gofs;
+    po.gwt = gwt;
+    po.gscale = gscale;
+    char *b = static_cast<char *>(pws);
+    if (L.ofs_bytes) po.pofs = reinterpret_cast<float *>(b);
+    b += L.ofs_bytes;
+    if (L.wt_bytes) po.pwt = reinterpret_cast<float *>(b);
+    b += L.wt_bytes;
+    if (L.scale_bytes) po.spart = reinterpret_cast<double *>(b);
+    return po;
+}
+}  // namespace
+
+size_t param_workspace(const DasProblem &p, bool has_o, bool has_w, bool has_s) {
+    if (!has_o && !has_w && !has_s) return 0;
+    const ParamLayout L = param_layout(p, has_o, has_w, has_s);
+    return L.ofs_bytes + L.wt_bytes + L.scale_bytes;
+}
+
 ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
                               const XView &xv, const void *gout, int64_t gs_b, void *gx,
-                              float *gofs, float *gwt, float *gscale, void *gt) {
+                              float *gofs, float *gwt, float *gscale, void *gt, void *pws) {
     const dev::Geo g = make_geo(a, p);
     const dev::Tiling tl = make_tiling(p.spatial, p.tune);
     hipStream_t st = h.stream;
@@ -1790,15 +2012,9 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     const bool precise = p.k_storage == Storage::F32 || p.k_storage == Storage::F64;
 
     if (gofs || gwt || gscale) {
-        const int FT = p.fi_frames > 0 ? p.fi_frames : frame_tile_for(p.B);
-        const bool multi = (p.B + FT - 1) / FT > 1;
-        if (multi) {
-            FFDAS_TRY(launch_fill_zero(h, gofs, (int64_t)p.K * p.N));
-            FFDAS_TRY(launch_fill_zero(h, gwt, (int64_t)p.K * p.N));
-        }
-        FFDAS_TRY(launch_fill_zero(h, gscale, 1));
-#define BWDP(ST, A64, CX) e = launch_bwdp_t<ST, A64, CX>(st, g, tl, xv, gout, gs_b, gofs, gwt, gscale)
-#define BWDPFI(ST, CX) e = launch_bwdp_fi_t<ST, CX>(st, g, tl, p.fi_frames, p.tune.param_groups, xv.ptr, gout, gs_b, gofs, gwt, gscale)
+        const ParamOut po = param_out(p, gofs, gwt, gscale, pws);
+#define BWDP(ST, A64, CX) e = launch_bwdp_t<ST, A64, CX>(st, g, tl, xv, gout, gs_b, po)
+#define BWDPFI(ST, CX) e = launch_bwdp_fi_t<ST, CX>(st, g, tl, p.fi_frames, p.tune.param_groups, xv.ptr, gout, gs_b, po)
         if (p.fi_frames > 0) {
             switch (p.k_storage) {
                 case Storage::F16: if (p.cplx) BWDPFI(Storage::F16, true); else BWDPFI(Storage::F16, false); break;

@@ -47,6 +47,9 @@ struct ffdas_context {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     hipStream_t ws_stream = nullptr;
+    // earlier, smaller workspace blocks: kept until the handle is destroyed, because a graph
+    // captured while one of them was current still reads it on every replay
+    std::vector<void *> retired;
 
     // set by the autotuner while it times candidates: replaces the cached/heuristic choice
     const ffdas::DasTune *force_tune = nullptr;
@@ -125,7 +128,9 @@ ffdas_error_t launch_forward(ffdas_context &h, const DasProblem &p, const DasGeo
                              const XView &xv, void *out, double beta_re, double beta_im);
 ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGeomArgs &a,
                               const XView &xv, const void *gout, int64_t gs_b, void *gx,
-                              float *gofs, float *gwt, float *gscale, void *gt);
+                              float *gofs, float *gwt, float *gscale, void *gt, void *pws);
+// workspace of the parameter adjoint's partial sums (frame chunks, d/d scale per block)
+size_t param_workspace(const DasProblem &p, bool has_offsets, bool has_weights, bool has_scale);
 // bytes of workspace the grad_x scatter needs for the transposed grad_out
 size_t backward_workspace(const DasProblem &p);
 int bwdx_frames(const DasProblem &p);
@@ -146,6 +151,18 @@ struct ProfScope {
     ~ProfScope() { profile_end(ev, st, name); }
 };
 int fi_frames_for(const DasProblem &p, ffdas_alg_t alg);
+
+// One problem of a batched forward launch: planned problem, geometry, its frame-inner copy
+// of x, output and beta.
+struct BatchMember {
+    DasProblem p;
+    DasGeomArgs a;
+    const void *xfi = nullptr;
+    void *out = nullptr;
+    double beta_re = 0.0, beta_im = 0.0;
+};
+// All members share storage type, complexness and frames per chunk (p.fi_frames > 0).
+ffdas_error_t launch_forward_batched(ffdas_context &h, const std::vector<BatchMember> &ms);
 
 // tuning cache (tune.cpp)
 std::string tune_key(const ffdas_context &h, const DasProblem &p, bool has_srcdir);

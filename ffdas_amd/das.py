@@ -20,7 +20,7 @@ import torch
 
 from . import _core
 from ._core import TensorView, as_tensor, check_device_tensor, element_code, handle_for
-from ._lib import Algorithm, ComputeType, DasTuning, DType, TensorDesc, check, lib
+from ._lib import Algorithm, ComputeType, DasProblem, DasTuning, DType, TensorDesc, check, lib
 
 
 class DowncastWarning(UserWarning):
@@ -353,3 +353,73 @@ def clear_tuning() -> None:
 
 def tuning_entries() -> int:
     return int(lib().ffdas_tune_count())
+
+
+def das_batched(problems, *, use_fp16=False, use_bf16=False,
+                algorithm: Algorithm = Algorithm.DEFAULT):
+    """Several forward problems in one call (ffdas_das_batched): mixed probes, voxel grids,
+    frame ensembles and sparse tables; problems that share storage type and frame chunking
+    run in one gather launch.  Each problem is a dict with the arguments of :func:`das` /
+    :func:`das_sparse` -- ``x, srcpos, dstpos, offsets, weights`` and optionally
+    ``srcdir, wavenum, sparse_indices, scale, channels_trailing, out`` -- and gets the same
+    result.  Returns the list of outputs."""
+    keep, structs, outs = [], [], []
+    dev = None
+    for q in problems:
+        x = as_tensor(q["x"])
+        check_device_tensor(x, "x", dev)
+        dev = x.device
+        srcpos = _astype(q["srcpos"], torch.float32, "srcpos")
+        dstpos = _astype(q["dstpos"], torch.float32, "dstpos")
+        offsets = _astype(q["offsets"], torch.float32, "offsets")
+        weights = _astype(q["weights"], torch.float32, "weights")
+        srcdir = _astype(q.get("srcdir"), torch.float32, "srcdir")
+        si = q.get("sparse_indices")
+        if si is not None:
+            si = _astype(si, torch.int32, "sparse_indices")
+        _validate_common(x, dstpos, offsets, weights)
+        shape, _ = _core.shape_strides(x)
+        batch = (shape[0],) if len(shape) == 4 else ()
+        odt = _out_dtype(element_code(x))
+        out = q.get("out")
+        if out is None:
+            out = torch.empty(batch + tuple(dstpos.shape[:-1]), dtype=odt, device=dev)
+        x_desc = _x_desc(x, bool(q.get("channels_trailing", False)))
+        o_desc = TensorDesc(tuple(out.shape), tuple(out.stride()), element_code(out))
+        beta = _beta_buffer(out.dtype)
+        keep += [x, srcpos, dstpos, offsets, weights, srcdir, si, x_desc, o_desc, beta]
+        structs.append(DasProblem(
+            _ptr(srcpos), _ptr(srcdir), float(q.get("wavenum", 0.0)), float(q.get("scale", 1.0)),
+            x_desc.ptr, _ptr(x), _ptr(dstpos), _ptr(offsets), _ptr(weights),
+            int(si.shape[0]) if si is not None else 0, _ptr(si),
+            ctypes.cast(beta, ctypes.c_void_p), o_desc.ptr, _ptr(out)))
+        outs.append(out)
+    if not structs:
+        return []
+    arr = (DasProblem * len(structs))(*structs)
+    ct = _compute_type(as_tensor(problems[0]["x"]), use_fp16, use_bf16)
+    with torch.cuda.device(dev):
+        h = handle_for(as_tensor(problems[0]["x"]))
+        check(lib().ffdas_das_batched(h.h, len(structs), arr, int(ct), int(algorithm)),
+              "ffdas_das_batched")
+    return outs
+
+
+class CapturedCall:
+    """A library call sequence captured once into a HIP graph (torch.cuda.CUDAGraph) and
+    replayed: ``fn`` is run once eagerly (it sizes the handle's workspace and resolves the
+    tuning), then captured; ``replay()`` re-runs every captured launch on the current stream
+    with the captured pointers, so inputs are updated in place and outputs read from the
+    tensors ``fn`` returned at capture time."""
+
+    def __init__(self, fn):
+        fn()
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.outputs = fn()
+        torch.cuda.synchronize()
+
+    def replay(self):
+        self.graph.replay()
+        return self.outputs

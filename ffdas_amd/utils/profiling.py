@@ -11,7 +11,8 @@ from contextlib import contextmanager
 from .._lib import check, lib
 
 KERNELS = ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi", "das_bwd_x",
-           "das_bwd_x_plain", "relayout_fi", "cast_rows", "transpose_gout")
+           "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
+           "transpose_gout")
 
 
 def enable(on: bool = True) -> None:

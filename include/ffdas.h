@@ -226,6 +226,35 @@ FFDAS_API ffdas_error_t ffdas_das_validate(ffdas_tensor_desc_t x_desc,
                                            const int *sparse_indices,
                                            ffdas_compute_type_t compute_type, ffdas_alg_t alg);
 
+/* One problem of ffdas_das_batched: the arguments of ffdas_das_ex. */
+typedef struct ffdas_das_problem {
+    const float *srcpos;
+    const float *srcdir;
+    float wavenum;
+    float scale;
+    ffdas_tensor_desc_t x_desc;
+    const void *x;
+    const float *dstpos;
+    const float *offsets;
+    const float *weights;
+    int sparse_count;
+    const int *sparse_indices;
+    const void *beta;            /* host scalar of the output type */
+    ffdas_tensor_desc_t out_desc;
+    void *out;
+} ffdas_das_problem_t;
+
+/* Several independent forward problems (mixed probes, grids, frame ensembles, sparse
+ * tables) in one call: each problem's result is that of ffdas_das_ex with the same
+ * arguments.  Problems that share the kernel storage type, complexness and frame chunking
+ * run in one gather launch (up to 12 problems per launch, descriptors passed by value, so a
+ * graph captured around this call holds them); the others run one launch each.  Form (2)
+ * of the batched/graph API; no reference counterpart (the reference launches one problem per
+ * call, src/das/das.cu:9-83). */
+FFDAS_API ffdas_error_t ffdas_das_batched(ffdas_handle_t handle, int count,
+                                          const ffdas_das_problem_t *problems,
+                                          ffdas_compute_type_t compute_type, ffdas_alg_t alg);
+
 /* Launch parameters chosen by ffdas_das_autotune, with the measured times [ms] of the
  * chosen and of the built-in (heuristic) parameters.  0 means "heuristic" for a field. */
 typedef struct ffdas_das_tuning {
@@ -269,7 +298,8 @@ FFDAS_API int ffdas_tune_count(void);
 /* Opt-in kernel timing: while enabled, every kernel the library launches is bracketed by
  * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
- * "das_bwd_x", "das_bwd_x_plain", "relayout_fi", "cast_rows", "transpose_gout").
+ * "das_bwd_x", "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
+ * "transpose_gout").
  * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);
