@@ -11,6 +11,7 @@ counted once per forward+adjoint step.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--storage fp16|bf16|fp32]
   python bench.py --profile-only N   # N steps + a 256 MiB copy, for rocprofv3 --pmc runs
+  python bench.py --config 5         # batched sparse compounding, three probes, one graph
 
 roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
 profiles/das_config4_<storage>_traffic.json, written by ffdas_amd.utils.pmc from two
@@ -47,6 +48,10 @@ def parse():
                     help="voxel rows (of 256) in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--config", type=int, default=4, choices=[4, 5],
+                    help="4: fwd+adjoint headline (default); 5: batched sparse compounding")
+    ap.add_argument("--c5-frames", type=int, default=200, help="config 5 ensemble size")
+    ap.add_argument("--c5-probes", default="linear64,linear256,matrix1024")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the built-in launch heuristics instead of autotuned parameters")
     ap.add_argument("--profile-only", type=int, default=0, metavar="N",
@@ -78,8 +83,11 @@ def step(p, x, scale=1.0, algorithm=None):
     alg = ffdas_amd.Algorithm.DEFAULT if algorithm is None else algorithm
     out = ffdas_amd.das(x, p.srcpos, p.dstpos, p.offsets, p.weights, scale=scale, algorithm=alg)
     g = 2.0 * (out - p.target)
+    # x is unchanged since the forward: the adjoint may gather from the forward's converted
+    # copy of x (the same contract autograd gives for its saved tensors)
     gx, go, _, gs = ffdas_amd.das_backward(x, p.srcpos, p.dstpos, p.offsets, p.weights, g,
-                                           scale=scale, need_weights=False, algorithm=alg)
+                                           scale=scale, need_weights=False, algorithm=alg,
+                                           reuse_x=True)
     return out, gx, go, gs
 
 
@@ -191,12 +199,90 @@ def cpu_baseline(w, rows):
                       f"{nthreads} threads"}
 
 
+def bench_config5(a):
+    """das_sparse over frame ensembles for mixed probes in one batched launch, captured in a
+    HIP graph and replayed (FP16 storage, FP32 accumulation).  Channel data are drawn on the
+    device (standard normal, FP16): the gather timing does not depend on sample values, and
+    the echo synthesis of the full ensembles (54 GB for the matrix probe) would dominate the
+    run; parity of this path is covered at reduced size with synthesised echoes."""
+    import torch
+    import ffdas_amd
+    from ffdas_amd import workloads as W
+    dev = torch.device("cuda:0")
+    names = [n for n in a.c5_probes.split(",") if n]
+    probs, U, geo_s = [], 0, time.time()
+    for name in names:
+        elem, grid, offs, wts, idx, _ = W.config5_geometry(name)
+        C = elem.shape[0]
+        B, T, S = a.c5_frames, 64, 2048
+        g = torch.Generator(device=dev).manual_seed(5)
+        x = torch.randn((B, C, T, S), dtype=torch.float16, device=dev, generator=g)
+        t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+        probs.append(dict(x=x, srcpos=t((elem * W.KS).astype(np.float32)),
+                          dstpos=t((grid * W.KS).astype(np.float32)), offsets=t(offs),
+                          weights=t(wts), sparse_indices=t(idx)))
+        U += int(np.prod(grid.shape[:-1])) * C * idx.shape[0] * B
+        log(f"{name}: x {tuple(x.shape)} fp16 {x.numel() * 2 / 1e9:.1f} GB, grid "
+            f"{grid.shape[:-1]}, k={idx.shape[0]}")
+    log(f"geometry and data in {time.time() - geo_s:.1f}s, U/ensemble {U:.4g}")
+    run = lambda: ffdas_amd.das_batched(probs)
+    ms_eager, _ = time_steps(run, a.steps, a.warmup)
+    cap = ffdas_amd.CapturedCall(run)
+    ms_graph, wall = time_steps(cap.replay, a.steps, a.warmup)
+    from ffdas_amd.utils import profiling
+    with profiling.profiled():
+        for _ in range(max(1, a.steps)):
+            run()
+        torch.cuda.synchronize()
+    kt = {k: v["avg_ms"] for k, v in profiling.kernel_stats().items()}
+    value = U / (ms_graph * 1e-3) / 1e9
+    log(f"eager {ms_eager:.2f} ms, graph {ms_graph:.2f} ms per ensemble: {value:.1f} G U/s; "
+        f"kernels {kt}")
+    achieved = U * 4 / (kt["das_fwd_fi_batched"] * 1e-3) / 1e9
+    cpu = None
+    if not a.no_cpu:
+        import oracle
+        w = W.config5(probe="linear64", n_frames=16, scale=1.0)
+        n = w.units
+        oracle.das(w.x, w.srcpos, w.dstpos, w.offsets, w.weights,
+                   sparse_indices=w.sparse_indices, bits=32)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.das(w.x, w.srcpos, w.dstpos, w.offsets, w.weights,
+                       sparse_indices=w.sparse_indices, bits=32)
+            ts.append(time.perf_counter() - t0)
+        cpu = {"value": n / float(np.median(ts)) / 1e9, "unit": "Gvoxel·ch·tx/s",
+               "cores": oracle.hardware_threads(), "kind": "port",
+               "sample": f"FP32 oracle das_sparse, 64-element probe, 128x128 voxels, 64 "
+                         f"transmits k=8, 16 frames ({n:.3g} U), median of 3"}
+    line = {"metric": "Gvoxel·ch·tx/s batched das_sparse (mixed probes, one graph)",
+            "value": round(value, 2), "unit": "Gvoxel·ch·tx/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_graph, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (device-drawn FP16 noise)",
+            "config": {"workload": "batched sparse compounding: " + ",".join(names) +
+                                   f", {a.c5_frames} frames, 64 focused transmits, k=8, "
+                                   "2048 samples", "storage": "fp16", "units_per_step": U,
+                       "eager_ms": round(ms_eager, 3), "wall_s": round(wall, 3)},
+            "roofline": {"bound": "hbm", "kernel": "das_fwd_fi_batched",
+                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_unit": 4, "units_per_launch": U,
+                         "kernel_ms": {k: round(v, 3) for k, v in kt.items()}},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "described in DESIGN.md but not built", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.config == 5:
+        bench_config5(a)
+        return
     import torch
     import ffdas_amd
     from ffdas_amd import workloads as W

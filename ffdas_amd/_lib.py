@@ -136,6 +136,7 @@ def lib():
         "ffdas_das_autotune": (i32, [vp, vp, vp, f32, f32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,
                                      vp, i32, i32, i32, P(DasTuning)]),
         "ffdas_das_batched": (i32, [vp, i32, P(DasProblem), i32, i32]),
+        "ffdas_set_reuse_x": (i32, [vp, i32]),
         "ffdas_tune_save": (i32, [ctypes.c_char_p]),
         "ffdas_tune_load": (i32, [ctypes.c_char_p]),
         "ffdas_tune_clear": (i32, []),

@@ -42,7 +42,8 @@ class DasFunction(torch.autograd.Function):
             sparse_indices=sparse_indices, scale=ctx.s, use_fp16=(ct == _das.ComputeType.FP16),
             use_bf16=(ct == _das.ComputeType.BF16), channels_trailing=ctx.channels_trailing,
             need_x=need[0], need_offsets=need[1], need_weights=need[2],
-            need_scale=need[3] and ctx.scale_is_tensor, algorithm=ctx.algorithm)
+            need_scale=need[3] and ctx.scale_is_tensor, algorithm=ctx.algorithm,
+            reuse_x=True)   # x is a saved tensor: autograd rejects in-place changes to it
         if gx is not None and gx.dtype != x.dtype:
             gx = gx.to(x.dtype)
         if go is not None:

@@ -47,6 +47,41 @@ size_t cast_bytes(const DasProblem &p) {
 // Produce the view of x the kernels gather from: the caller's tensor when it already has
 // the kernel storage type, otherwise a contiguous converted copy in workspace `ws`
 // (cast_bytes(p) bytes).
+// Whether the copy recorded in h.xcopy was made from this x for this problem.
+bool xcopy_matches(const ffdas_context &h, const ffdas_tensor_desc &xd, const DasProblem &p,
+                   const void *x, void *ws) {
+    const auto &c = h.xcopy;
+    return c.valid && c.x == x && c.ws == ws && c.dims == xd.dims && c.strides == xd.strides &&
+           c.dtype == (int)xd.dtype && c.storage == (int)p.k_storage &&
+           c.fi_frames == p.fi_frames && c.cast == (p.x_needs_copy ? 1 : 0);
+}
+
+void record_xcopy(ffdas_context &h, const ffdas_tensor_desc &xd, const DasProblem &p,
+                  const void *x, void *ws) {
+    auto &c = h.xcopy;
+    c.valid = p.fi_frames > 0 || p.x_needs_copy;
+    c.x = x;
+    c.ws = ws;
+    c.dims = xd.dims;
+    c.strides = xd.strides;
+    c.dtype = (int)xd.dtype;
+    c.storage = (int)p.k_storage;
+    c.fi_frames = p.fi_frames;
+    c.cast = p.x_needs_copy ? 1 : 0;
+}
+
+// The view of an already converted copy at `ws` (no launch).
+XView copy_view(const DasProblem &p, const void *ws) {
+    XView xv;
+    xv.ptr = ws;
+    if (p.fi_frames <= 0) {
+        xv.st = p.S;
+        xv.sc = (int64_t)p.T * p.S;
+        xv.sb = (int64_t)p.C * p.T * p.S;
+    }
+    return xv;
+}
+
 ffdas_error_t prepare_x(ffdas_context &h, const DasProblem &p, const void *x, void *ws,
                         XView &xv) {
     if (p.fi_frames > 0) {
@@ -93,6 +128,7 @@ ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, con
     FFDAS_TRY(handle->workspace(cast_bytes(p), &ws));
     XView xv;
     FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
+    record_xcopy(*handle, xd, p, x, ws);
     DasGeomArgs a;
     a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
     a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
@@ -172,7 +208,9 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
             s *= xg.dims[i];
         }
     }
-    const size_t xb = cast_bytes(p);
+    // grad_x alone does not read x: no converted copy is needed then
+    const bool need_xv = grad_offsets || grad_weights || grad_scale;
+    const size_t xb = need_xv ? cast_bytes(p) : 0;
     const size_t gb = grad_x ? align_up(backward_workspace(p)) : 0;
     if (grad_x && gb == 0 && !p.plain) return FFDAS_ERROR_DIMS_TOO_LARGE;   // row exceeds LDS
     const size_t pb = align_up(param_workspace(p, grad_offsets != nullptr,
@@ -180,7 +218,15 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
     void *ws = nullptr;
     FFDAS_TRY(handle->workspace(xb + gb + pb, &ws));
     XView xv;
-    FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
+    if (xb == 0) handle->xcopy.valid = false;   // grad_x's buffers start at the workspace base
+    if (need_xv) {
+        if (handle->reuse_x && xb > 0 && xcopy_matches(*handle, xd, p, x, ws)) {
+            xv = copy_view(p, ws);
+        } else {
+            FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
+            record_xcopy(*handle, xd, p, x, ws);
+        }
+    }
     void *gt = grad_x ? static_cast<char *>(ws) + xb : nullptr;
     void *pws = static_cast<char *>(ws) + xb + gb;
     DasGeomArgs a;
@@ -272,6 +318,7 @@ extern "C" ffdas_error_t ffdas_das_batched(ffdas_handle_t handle, int count,
     }
     void *ws = nullptr;
     FFDAS_TRY(handle->workspace(total, &ws));
+    handle->xcopy.valid = false;
     // converted copies of x, then one gather launch per (storage, complex, chunk) group
     std::vector<XView> xv(count);
     for (int i = 0; i < count; i++) {
@@ -335,7 +382,10 @@ ffdas_error_t time_runs(ffdas_context &h, int reps, Fn fn, float &ms) {
 struct ForceGuard {
     ffdas_context *h;
     explicit ForceGuard(ffdas_context *hh) : h(hh) {}
-    ~ForceGuard() { h->force_tune = nullptr; }
+    ~ForceGuard() {
+        h->force_tune = nullptr;
+        h->xcopy.valid = false;
+    }
 };
 
 }  // namespace
@@ -359,6 +409,7 @@ extern "C" ffdas_error_t ffdas_das_autotune(
                        sparse_count, sparse_indices, compute_type, alg, p));
     const std::string key = tune_key(*handle, p, srcdir != nullptr);
     ForceGuard guard(handle);
+    handle->xcopy.valid = false;
     const double zero[2] = {0.0, 0.0};   // host beta of any output type
     auto forward = [&]() {
         return das_forward_common(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
@@ -372,12 +423,12 @@ extern "C" ffdas_error_t ffdas_das_autotune(
         return time_runs(*handle, reps, fn, ms);
     };
 
-    // forward: frames per frame-inner chunk x channel groups x 2-D wave tile
+    // The frame chunking and the 2-D tile shape are shared by the forward and the parameter
+    // adjoint (one layout of x), so with the adjoint they are chosen on the sum of both
+    // times; the channel groups of each kernel are chosen separately for every such pair.
     const DasTune heuristic;
     float fwd_default = 0.0f;
     FFDAS_TRY(measure(heuristic, forward, fwd_default));
-    DasTune best = heuristic;
-    float fwd_best = fwd_default;
     std::vector<int> frames;
     for (int F : {16, 8, 4}) {
         DasProblem q = p;
@@ -396,93 +447,120 @@ extern "C" ffdas_error_t ffdas_das_autotune(
     } else {
         frames.push_back(0);
     }
-    for (int F : frames)
-        for (int G : groups)
-            for (auto tt : tiles) {
-                DasTune t = heuristic;
-                t.fi_frames = F;
-                t.groups = G;
-                t.tile_h = tt.first;
-                t.tile_w = tt.second;
-                float ms = 0.0f;
-                FFDAS_TRY(measure(t, forward, ms));
-                if (ms < fwd_best) {
-                    fwd_best = ms;
-                    best = t;
-                }
-            }
 
-    // adjoint: parameter kernel channel groups, grad_x frames / depth lanes / workgroup size
-    float bwd_default = 0.0f, bwd_best = 0.0f;
+    float *gofs = nullptr;
     if (grad_x) {
-        handle->force_tune = &best;
-        FFDAS_TRY(forward());   // out now holds a forward result: used as grad_out below
-        float *gofs = nullptr;
         const size_t gbytes = (size_t)p.K * (size_t)p.N * sizeof(float);
         if (hipMalloc(reinterpret_cast<void **>(&gofs), gbytes) != hipSuccess)
             return FFDAS_ERROR_ALLOCATION_FAILED;
-        auto param = [&]() {
-            return ffdas_das_backward(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
-                                      offsets, weights, sparse_count, sparse_indices, out_desc,
-                                      out, nullptr, nullptr, gofs, nullptr, nullptr,
-                                      compute_type, alg);
-        };
-        auto bwdx = [&]() {
-            return ffdas_das_backward(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
-                                      offsets, weights, sparse_count, sparse_indices, out_desc,
-                                      out, grad_x_desc, grad_x, nullptr, nullptr, nullptr,
-                                      compute_type, alg);
-        };
-        ffdas_error_t err = FFDAS_SUCCESS;
-        float pd = 0.0f, xd = 0.0f;
+    }
+    struct FreeOnExit {
+        float *p;
+        ~FreeOnExit() { if (p) (void)hipFree(p); }
+    } free_gofs{gofs};
+    auto param = [&]() {
+        return ffdas_das_backward(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                  offsets, weights, sparse_count, sparse_indices, out_desc, out,
+                                  nullptr, nullptr, gofs, nullptr, nullptr, compute_type, alg);
+    };
+    auto bwdx = [&]() {
+        return ffdas_das_backward(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                  offsets, weights, sparse_count, sparse_indices, out_desc, out,
+                                  grad_x_desc, grad_x, nullptr, nullptr, nullptr, compute_type,
+                                  alg);
+    };
+
+    DasTune best = heuristic;
+    float fwd_best = fwd_default, param_best = 0.0f, param_default = 0.0f;
+    float total_best = 0.0f;
+    bool have = false;
+    if (grad_x) {
+        FFDAS_TRY(forward());   // out holds a forward result: the grad_out of the timed adjoints
+        FFDAS_TRY(measure(heuristic, param, param_default));
+    }
+    for (int F : frames)
+        for (auto tt : tiles) {
+            DasTune layout = heuristic;
+            layout.fi_frames = F;
+            layout.tile_h = tt.first;
+            layout.tile_w = tt.second;
+            DasTune bf = layout;
+            float fb = 0.0f;
+            for (int G : groups) {
+                DasTune t = layout;
+                t.groups = G;
+                float ms = 0.0f;
+                FFDAS_TRY(measure(t, forward, ms));
+                if (G == groups.front() || ms < fb) {
+                    fb = ms;
+                    bf = t;
+                }
+            }
+            float pb = 0.0f;
+            if (grad_x) {
+                for (int G : groups) {
+                    DasTune t = bf;
+                    t.param_groups = G;
+                    float ms = 0.0f;
+                    FFDAS_TRY(measure(t, param, ms));
+                    if (G == groups.front() || ms < pb) {
+                        pb = ms;
+                        bf.param_groups = G;
+                    }
+                }
+            }
+            if (!have || fb + pb < total_best) {
+                have = true;
+                total_best = fb + pb;
+                best = bf;
+                fwd_best = fb;
+                param_best = pb;
+            }
+        }
+    if (fwd_default + param_default <= total_best) {   // the heuristic is at least as fast
+        best = heuristic;
+        fwd_best = fwd_default;
+        param_best = param_default;
+    }
+
+    // grad_x: frames per workgroup, lanes along depth, workgroup size
+    float bwd_default = 0.0f, bwd_best = 0.0f;
+    if (grad_x && !p.plain) {
         DasTune t0 = best;
-        t0.param_groups = 0;
         t0.bwdx_frames = 0;
         t0.bwdx_lg = -1;
         t0.bwdx_threads = 0;
-        err = measure(t0, param, pd);
-        if (err == FFDAS_SUCCESS) err = measure(t0, bwdx, xd);
-        bwd_default = pd + xd;
-        float pb = pd, xb = xd;
-        DasTune bp = t0, bx = t0;
-        if (err == FFDAS_SUCCESS && best.fi_frames >= 0)
-            for (int G : {1, 2, 4, 8}) {
-                if (G > p.C || err != FFDAS_SUCCESS) break;
-                DasTune t = t0;
-                t.param_groups = G;
-                float ms = 0.0f;
-                err = measure(t, param, ms);
-                if (err == FFDAS_SUCCESS && ms < pb) {
-                    pb = ms;
-                    bp = t;
-                }
-            }
-        if (!p.plain)
-            for (int F : {16, 8, 4, 2})
-                for (int lg : {2, 3, 4, 5, 6})
-                    for (int th : {512, 1024}) {
-                        if (err != FFDAS_SUCCESS) break;
-                        DasTune t = t0;
-                        t.bwdx_frames = F;
-                        t.bwdx_lg = lg;
-                        t.bwdx_threads = th;
-                        DasProblem q = p;
-                        q.tune = t;
-                        if (bwdx_frames(q) != F) continue;
-                        float ms = 0.0f;
-                        err = measure(t, bwdx, ms);
-                        if (err == FFDAS_SUCCESS && ms < xb) {
-                            xb = ms;
-                            bx = t;
-                        }
+        float xd = 0.0f;
+        FFDAS_TRY(measure(t0, bwdx, xd));
+        float xb = xd;
+        DasTune bx = t0;
+        for (int F : {16, 8, 4, 2})
+            for (int lg : {2, 3, 4, 5, 6})
+                for (int th : {512, 1024}) {
+                    DasTune t = t0;
+                    t.bwdx_frames = F;
+                    t.bwdx_lg = lg;
+                    t.bwdx_threads = th;
+                    DasProblem q = p;
+                    q.tune = t;
+                    if (bwdx_frames(q) != F) continue;
+                    float ms = 0.0f;
+                    FFDAS_TRY(measure(t, bwdx, ms));
+                    if (ms < xb) {
+                        xb = ms;
+                        bx = t;
                     }
-        (void)hipFree(gofs);
-        if (err != FFDAS_SUCCESS) return err;
-        best.param_groups = bp.param_groups;
+                }
         best.bwdx_frames = bx.bwdx_frames;
         best.bwdx_lg = bx.bwdx_lg;
         best.bwdx_threads = bx.bwdx_threads;
-        bwd_best = pb + xb;
+        bwd_default = param_default + xd;
+        bwd_best = param_best + xb;
+    } else if (grad_x) {
+        float xd = 0.0f;
+        FFDAS_TRY(measure(best, bwdx, xd));
+        bwd_default = param_default + xd;
+        bwd_best = param_best + xd;
     }
     tune_store(key, best);
     if (result) {
@@ -500,5 +578,11 @@ extern "C" ffdas_error_t ffdas_das_autotune(
         result->default_backward_ms = bwd_default;
         result->candidates = candidates;
     }
+    return FFDAS_SUCCESS;
+}
+
+extern "C" ffdas_error_t ffdas_set_reuse_x(ffdas_handle_t handle, int enable) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    handle->reuse_x = enable != 0;
     return FFDAS_SUCCESS;
 }

@@ -403,31 +403,36 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
                     A v[2][FT][CW];
                     xr.load(xr.offset(m, sq[j], tp.k), v);
                     const A t = (A)tp.t;
+                    // By linearity in the samples: with A0 = sum_f g x[k], A1 = sum_f g x[k+1],
+                    // sum_f g (x[k+1] - x[k]) = A1 - A0 and sum_f g lerp = A0 + t (A1 - A0):
+                    // two FMAs per frame instead of four.
                     if constexpr (!CPLX) {
-                        A sd_ = A(0), sl_ = A(0);
+                        A a0 = A(0), a1 = A(0);
 #pragma unroll
                         for (int f = 0; f < FT; f++) {
-                            const A d = v[1][f][0] - v[0][f][0];
-                            sd_ += gv[f][0] * d;
-                            sl_ += gv[f][0] * (v[0][f][0] + t * d);
+                            a0 = fmaA(gv[f][0], v[0][f][0], a0);
+                            a1 = fmaA(gv[f][0], v[1][f][0], a1);
                         }
+                        const A sd_ = a1 - a0;
+                        const A sl_ = fmaA(t, sd_, a0);
                         const A dt = (A)wt[j] * sd_;
                         d_tau[j] += dt;
                         d_w[j] += sl_;
                         scale_part += dt * ((A)ofs[j] + (A)R.rh + (A)R.rl);
                     } else {
-                        // Q = sum conj(g) (x1 - x0), L = sum conj(g) lerp
-                        A qr = 0, qi = 0, lr = 0, li = 0;
+                        // B0 = sum conj(g) x[k], B1 = sum conj(g) x[k+1];
+                        // Q = B1 - B0 = sum conj(g) (x1 - x0), L = B0 + t Q = sum conj(g) lerp
+                        A b0r = 0, b0i = 0, b1r = 0, b1i = 0;
 #pragma unroll
                         for (int f = 0; f < FT; f++) {
-                            const A dr = v[1][f][0] - v[0][f][0], di = v[1][f][1] - v[0][f][1];
-                            const A pr = v[0][f][0] + t * dr, pi = v[0][f][1] + t * di;
                             const A gr = gv[f][0], gi = gv[f][1];
-                            qr += gr * dr + gi * di;
-                            qi += gr * di - gi * dr;
-                            lr += gr * pr + gi * pi;
-                            li += gr * pi - gi * pr;
+                            b0r = fmaA(gr, v[0][f][0], fmaA(gi, v[0][f][1], b0r));
+                            b0i = fmaA(gr, v[0][f][1], fmaA(-gi, v[0][f][0], b0i));
+                            b1r = fmaA(gr, v[1][f][0], fmaA(gi, v[1][f][1], b1r));
+                            b1i = fmaA(gr, v[1][f][1], fmaA(-gi, v[1][f][0], b1i));
                         }
+                        const A qr = b1r - b0r, qi = b1i - b0i;
+                        const A lr = fmaA(t, qr, b0r), li = fmaA(t, qi, b0i);
                         float sn, cs;
                         phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
                         const A er = (A)cs, ei = (A)sn, kap = (A)g.wavenum;
@@ -573,9 +578,10 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
                                                           int64_t sc, int64_t st, int B, int C,
                                                           int T, int S, DST *__restrict__ dst) {
     __shared__ float tile[FT][64 * CW + 1];
-    const int seg = blockIdx.x;                 // 64-sample segment
-    const int row = blockIdx.y;                 // c * T + t
-    const int chunk = blockIdx.z;
+    const int nseg = (S + 63) >> 6;
+    const int row = blockIdx.x / nseg;          // c * T + t (grid.x: rows x segments, so
+    const int seg = blockIdx.x - row * nseg;    // C*T is not bound by the 65535 of grid.y)
+    const int chunk = blockIdx.y;
     const int c = row / T, t = row - c * T;
     const int k0 = seg * 64;
     for (int i = threadIdx.x; i < FT * 64 * CW; i += blockDim.x) {
@@ -1741,7 +1747,9 @@ hipError_t launch_fwd_batched_t(hipStream_t st, const std::vector<BatchMember> &
 template <Storage SS, class DST, bool CPLX, int F>
 hipError_t launch_relayout_f(hipStream_t st, const DasProblem &p, const void *x, void *dst) {
     constexpr int CW = CPLX ? 2 : 1;
-    dim3 grid((p.S + 63) / 64, p.C * p.T, (p.B + F - 1) / F);
+    const int64_t blocks = (int64_t)((p.S + 63) / 64) * p.C * p.T;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    dim3 grid((unsigned)blocks, (p.B + F - 1) / F);
     ProfScope ps(st, "relayout_fi");
     dev::relayout_fi_kernel<SS, DST, F, CW><<<grid, 256, 0, st>>>(
         static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B, p.C,

@@ -53,6 +53,16 @@ struct ffdas_context {
 
     // set by the autotuner while it times candidates: replaces the cached/heuristic choice
     const ffdas::DasTune *force_tune = nullptr;
+    // The converted copy of x at the start of the workspace: which x it was made from.
+    // ffdas_das_backward reuses it when reuse_x is set and everything matches.
+    struct {
+        bool valid = false;
+        const void *x = nullptr;
+        void *ws = nullptr;
+        std::vector<int64_t> dims, strides;
+        int dtype = 0, storage = 0, fi_frames = 0, cast = 0;
+    } xcopy;
+    bool reuse_x = false;
 
     ffdas_error_t check_device() const;
     ffdas_error_t workspace(size_t bytes, void **ptr);

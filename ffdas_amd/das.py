@@ -241,11 +241,13 @@ def das_sparse(x, srcpos, dstpos, offsets, weights, sparse_indices, *, srcdir=No
 def das_backward(x, srcpos, dstpos, offsets, weights, grad_out, *, srcdir=None, wavenum=0.0,
                  sparse_indices=None, scale=1.0, use_fp16=False, use_bf16=False,
                  channels_trailing=False, need_x=True, need_offsets=True, need_weights=True,
-                 need_scale=True, algorithm: Algorithm = Algorithm.DEFAULT):
+                 need_scale=True, algorithm: Algorithm = Algorithm.DEFAULT, reuse_x=False):
     """Adjoint of :func:`das` / :func:`das_sparse` via ffdas_das_backward.
 
     Returns ``(grad_x, grad_offsets, grad_weights, grad_scale)``; unrequested entries are None.
-    grad_x has x's shape (float32/complex64, float64/complex128 for double data)."""
+    grad_x has x's shape (float32/complex64, float64/complex128 for double data).
+    ``reuse_x``: x is unchanged since the last forward on this device, so the converted copy
+    that forward made may be gathered from again (ffdas_set_reuse_x)."""
     x = as_tensor(x)
     check_device_tensor(x, "x")
     dev = x.device
@@ -276,6 +278,7 @@ def das_backward(x, srcpos, dstpos, offsets, weights, grad_out, *, srcdir=None, 
     L = lib()
     with torch.cuda.device(dev):
         h = handle_for(x)
+        check(L.ffdas_set_reuse_x(h.h, int(bool(reuse_x))), "ffdas_set_reuse_x")
         rc = L.ffdas_das_backward(
             h.h, _ptr(srcpos), _ptr(srcdir), float(wavenum), float(scale), x_desc.ptr, _ptr(x),
             _ptr(dstpos), _ptr(offsets), _ptr(weights),
@@ -283,6 +286,7 @@ def das_backward(x, srcpos, dstpos, offsets, weights, grad_out, *, srcdir=None, 
             _ptr(sparse_indices), g_desc.ptr, _ptr(grad_out),
             gx_desc.ptr if gx_desc is not None else None, _ptr(gx), _ptr(go), _ptr(gw), _ptr(gs),
             int(ct), int(algorithm))
+        L.ffdas_set_reuse_x(h.h, 0)
         check(rc, "ffdas_das_backward")
     if gx is not None and channels_trailing:
         c = 1 if len(dims) == 4 else 0

@@ -211,6 +211,110 @@ def config2(seed: int = 2, n: int = 128, n_samples: int = 4096, n_frames: int = 
                               srcdir="(0,0,1,cos30)"))
 
 
+# ---------------------------------------------------------------------------------------
+# config 5: batched sparse compounding over frame ensembles, mixed probes
+# ---------------------------------------------------------------------------------------
+def focused_delay(points_m: np.ndarray, foci_m: np.ndarray) -> np.ndarray:
+    """One-way focused-transmit delay in samples, (T, *points): a virtual source at the focus
+    f fed from the element below it, o = (f_x, f_y, 0): |f - o| + sign(z - f_z) |y - f|."""
+    sh = (-1,) + (1,) * (points_m.ndim - 1) + (3,)
+    f = foci_m.reshape(sh)
+    d = np.linalg.norm(points_m[None] - f, axis=-1)
+    sgn = np.where(points_m[None, ..., 2] >= f[..., 2], 1.0, -1.0)
+    return (f[..., 2] + sgn * d) * KS
+
+
+def nearest_foci(points_m: np.ndarray, foci_m: np.ndarray, k: int) -> np.ndarray:
+    """int32 (k, *points): the k foci with the smallest lateral distance to each voxel."""
+    lat = points_m[..., None, :2] - foci_m[:, :2]
+    d = np.linalg.norm(lat, axis=-1)                                  # (*points, T)
+    idx = np.argsort(d, axis=-1, kind="stable")[..., :k]
+    return np.moveaxis(idx, -1, 0).astype(np.int32)
+
+
+CONFIG5_PROBES = {
+    # name: (elements, grid shape, lateral half width [m], depth range [m], foci layout)
+    "linear64": dict(kind="linear", n=64, pitch=0.3e-3, grid=(128, 128), half=9.6e-3,
+                     depth=(5e-3, 45e-3), focus_z=25e-3),
+    "linear256": dict(kind="linear", n=256, pitch=0.2e-3, grid=(192, 192), half=25.6e-3,
+                      depth=(5e-3, 45e-3), focus_z=25e-3),
+    "matrix1024": dict(kind="matrix", n=32, pitch=0.3e-3, grid=(96, 96, 96), half=9.6e-3,
+                       depth=(5e-3, 30e-3), focus_z=17.5e-3),
+}
+
+
+def config5_geometry(probe: str, n_tx: int = 64, k: int = 8, scale: float = 1.0):
+    """Geometry of one config-5 probe: element positions, voxel grid, focused-transmit delays,
+    the k-nearest-foci index table and weights 1/k.  `scale` shrinks the grid (tests).
+    Returns (elem [m], grid [m], offsets [samples], weights, sparse_indices, foci [m])."""
+    c = CONFIG5_PROBES[probe]
+    if c["kind"] == "linear":
+        elem = linear_array(c["n"], c["pitch"])
+        nz, nx = (max(2, int(round(v * scale))) for v in c["grid"])
+        grid = _grid2d(nz, nx, c["depth"], c["half"])
+        fx = np.linspace(-c["half"], c["half"], n_tx)
+        foci = np.stack([fx, np.zeros(n_tx), np.full(n_tx, c["focus_z"])], -1)
+    else:
+        elem = matrix_array(c["n"], c["pitch"])
+        nd, ny, nx = (max(2, int(round(v * scale))) for v in c["grid"])
+        lat_y = np.linspace(-c["half"], c["half"], ny)
+        lat_x = np.linspace(-c["half"], c["half"], nx)
+        dep = np.linspace(c["depth"][0], c["depth"][1], nd)
+        zz, yy, xx = np.meshgrid(dep, lat_y, lat_x, indexing="ij")
+        grid = np.stack([xx, yy, zz], axis=-1)
+        m = int(round(math.sqrt(n_tx)))
+        g = np.linspace(-c["half"], c["half"], m)
+        fxx, fyy = np.meshgrid(g, g, indexing="ij")
+        foci = np.stack([fxx.ravel(), fyy.ravel(), np.full(m * m, c["focus_z"])], -1)
+    offsets = focused_delay(grid, foci)                               # (T, *grid)
+    idx = nearest_foci(grid, foci, k)                                 # (k, *grid)
+    offs_k = np.take_along_axis(offsets, idx.astype(np.int64), axis=0).astype(np.float32)
+    weights = np.full(offs_k.shape, 1.0 / k, dtype=np.float32)
+    return elem, grid, offs_k, weights, idx, foci
+
+
+def config5(seed: int = 5, probe: str = "linear64", n_frames: int = 200, n_tx: int = 64,
+            k: int = 8, n_samples: int = 2048, scale: float = 1.0, n_tissue: int = 40,
+            n_moving: int = 20) -> Workload:
+    """Batched sparse compounding for one probe: das_sparse over an n_frames ensemble, n_tx
+    focused transmits, each voxel compounding its k nearest foci (int32 table (k, *grid),
+    weights 1/k).  Scatterers: n_tissue static ones (amplitude x100) and n_moving in a 5 mm
+    band at the focal depth moving 0.05 mm/frame laterally.  Echoes are synthesised on the
+    host, so use this at reduced size (scale, frames, channels); the full-size benchmark
+    draws channel data on the device (the timing does not depend on the sample values)."""
+    rng = np.random.default_rng(seed)
+    elem, grid, offs, weights, idx, foci = config5_geometry(probe, n_tx, k, scale)
+    c = CONFIG5_PROBES[probe]
+    half = c["half"] * 0.9
+    d0, d1 = c["depth"]
+    lat2 = c["kind"] == "matrix"
+    tis = np.stack([rng.uniform(-half, half, n_tissue),
+                    rng.uniform(-half, half, n_tissue) if lat2 else np.zeros(n_tissue),
+                    rng.uniform(d0, d1, n_tissue)], -1)
+    tis_amp = 100.0 * rng.uniform(0.5, 1.0, n_tissue)
+    mv0 = np.stack([rng.uniform(-half, half, n_moving),
+                    rng.uniform(-half, half, n_moving) if lat2 else np.zeros(n_moving),
+                    rng.uniform(c["focus_z"] - 2.5e-3, c["focus_z"] + 2.5e-3, n_moving)], -1)
+    mv_amp = rng.uniform(0.5, 1.0, n_moving)
+
+    def scat(b):
+        mv = mv0.copy()
+        mv[:, 0] = (mv[:, 0] + b * 0.05e-3 + half) % (2 * half) - half
+        return np.concatenate([tis, mv]), np.concatenate([tis_amp, mv_amp])
+
+    x = synth_echoes(rng, n_frames, elem, lambda p: focused_delay(p, foci), n_tx, n_samples,
+                     scat)
+    return Workload(f"config5_{probe}", x, (elem * KS).astype(np.float32),
+                    (grid * KS).astype(np.float32), offs, weights, sparse_indices=idx,
+                    meta=dict(seed=seed, probe=probe, frames=n_frames, transmits=n_tx, k=k,
+                              samples=n_samples, scale=scale,
+                              transmit="focused, virtual source at the focus fed from the "
+                                       "element below it",
+                              selection="k nearest foci by lateral distance",
+                              motion="0.05 mm/frame lateral, 5 mm band at the focal depth",
+                              tissue="static, amplitude x100"))
+
+
 def small_random(seed=0, B=2, C=6, T=3, S=96, N=(5, 7), K=None, complex_iq=False,
                  srcdir=False, sparse=False):
     """Small random problem with delays that land inside and outside [0, S-1)."""

@@ -289,6 +289,14 @@ FFDAS_API ffdas_error_t ffdas_das_autotune(ffdas_handle_t handle, const float *s
                                            ffdas_compute_type_t compute_type, ffdas_alg_t alg,
                                            int reps, ffdas_das_tuning_t *result);
 
+/* Reuse of the converted copy of x in the adjoint (no reference counterpart).  Every
+ * forward keeps the copy of x it gathers from (FP16/BF16 cast, frame-inner relayout) at the
+ * start of the handle's workspace.  While enabled, ffdas_das_backward gathers from that copy
+ * instead of making a new one when it was made from the same x pointer, descriptor, storage
+ * type and chunking and nothing has used the workspace since; the caller guarantees that x
+ * was not modified in between (autograd guarantees it for saved tensors). */
+FFDAS_API ffdas_error_t ffdas_set_reuse_x(ffdas_handle_t handle, int enable);
+
 /* Tuning cache persistence: one text line per problem signature. */
 FFDAS_API ffdas_error_t ffdas_tune_save(const char *path);
 FFDAS_API ffdas_error_t ffdas_tune_load(const char *path);

@@ -201,3 +201,24 @@ def test_plain_adjoint_alg1(dev, case):
     assert rel_l2(go.cpu().numpy(), ref["grad_offsets"]) < 1e-4
     assert rel_l2(gw.cpu().numpy(), ref["grad_weights"]) < 1e-5
     assert abs(float(gs) - ref["grad_scale"]) <= 1e-4 * max(1.0, abs(ref["grad_scale"]))
+
+
+def test_reuse_x_matches_a_fresh_copy(dev):
+    """reuse_x gathers from the forward's converted copy: identical gradients when x is
+    unchanged; a forward of another x in between is detected and a fresh copy is made."""
+    import torch
+    import ffdas_amd
+    w = W.small_random(seed=61, B=16, C=6, T=3, S=120, N=(12, 10))
+    w2 = W.small_random(seed=62, B=16, C=6, T=3, S=120, N=(12, 10))
+    d, d2 = to_dev(w, dev, torch.float16), to_dev(w2, dev, torch.float16)
+    args = lambda q: (q["x"], q["srcpos"], q["dstpos"], q["offsets"], q["weights"])
+    g = torch.randn((16, 12, 10), device=dev)
+    fresh = ffdas_amd.das_backward(*args(d), g)
+    ffdas_amd.das(*args(d))
+    reused = ffdas_amd.das_backward(*args(d), g, reuse_x=True)
+    ffdas_amd.das(*args(d))
+    ffdas_amd.das(*args(d2))                     # another x overwrites the copy
+    other = ffdas_amd.das_backward(*args(d), g, reuse_x=True)
+    for a, b, c in zip(fresh, reused, other):
+        assert torch.equal(a, b)
+        assert torch.equal(a, c)
