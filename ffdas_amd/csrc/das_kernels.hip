@@ -606,6 +606,64 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
     }
 }
 
+// The same relayout with 16-byte vector loads and stores, for rows whose start and length are
+// 16-byte multiples: a block moves a segment of kRelL samples of one (chunk, c, t) row for
+// all FT frames (each frame's segment read as 16-byte vectors, the frame-inner segment
+// written as 16-byte vectors), transposing through LDS.
+constexpr int kRelL = 128;
+
+template <Storage SS, class DST, int FT, int CW>
+__global__ __launch_bounds__(256) void relayout_fi_vec_kernel(
+    const typename Elem<SS>::raw *__restrict__ src, int64_t sb, int64_t sc, int64_t st, int B,
+    int C, int T, int S, DST *__restrict__ dst) {
+    using SR = typename Elem<SS>::raw;
+    constexpr int SV = 16 / (int)sizeof(SR);        // source elements per vector
+    constexpr int DV = 16 / (int)sizeof(DST);       // destination elements per vector
+    constexpr int RL = kRelL * CW;                  // elements of one frame's segment
+    __shared__ float tile[FT][RL + 1];
+    const int nseg = (S + kRelL - 1) / kRelL;
+    const int row = blockIdx.x / nseg;
+    const int seg = blockIdx.x - row * nseg;
+    const int chunk = blockIdx.y;
+    const int c = row / T, t = row - c * T;
+    const int e0 = seg * RL;                        // first element of the segment
+    const int SE = S * CW;                          // elements per row
+    for (int i = threadIdx.x; i < FT * (RL / SV); i += blockDim.x) {
+        const int f = i / (RL / SV), e = (i - f * (RL / SV)) * SV;
+        const int b = chunk * FT + f;
+        float v[SV];
+        if (b < B && e0 + e < SE) {
+            const SR *p = src + ((int64_t)b * sb + (int64_t)c * sc + (int64_t)t * st) * CW + e0 + e;
+            SR r[SV];
+            const uint4 q = *reinterpret_cast<const uint4 *>(p);
+            __builtin_memcpy(r, &q, 16);
+#pragma unroll
+            for (int j = 0; j < SV; j++) v[j] = (float)Elem<SS>::real(r[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < SV; j++) v[j] = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < SV; j++) tile[f][e + j] = v[j];
+    }
+    __syncthreads();
+    DST *d = dst + (((((int64_t)chunk * C + c) * T + t) * S) + (int64_t)seg * kRelL) * FT * CW;
+    const int kmax = min(kRelL, S - seg * kRelL);   // samples of this segment inside the row
+    for (int i = threadIdx.x; i < kmax * FT * CW / DV; i += blockDim.x) {
+        DST o[DV];
+#pragma unroll
+        for (int j = 0; j < DV; j++) {
+            const int idx = i * DV + j;
+            const int kk = idx / (FT * CW), r = idx - kk * FT * CW;
+            const int f = r / CW, cc = r - f * CW;
+            o[j] = convert_to<DST>(tile[f][kk * CW + cc]);
+        }
+        uint4 q;
+        __builtin_memcpy(&q, o, 16);
+        reinterpret_cast<uint4 *>(d)[i] = q;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // adjoint: x (channel-row scatter through LDS)
 // ---------------------------------------------------------------------------------------
@@ -704,21 +762,25 @@ __device__ __forceinline__ void fx_add2(unsigned long long *p, double w1, double
 __device__ __forceinline__ int lds_slot(int k) { return k ^ ((k >> 5) & 31); }
 
 // Packed fixed point: two frames share one 64-bit LDS word, frame 2p in the low and frame
-// 2p+1 in the high 32 bits, each an integer sum in units of 2^-E.  Adding
-// c0 + c1 * 2^32 (two's complement) for every contribution leaves
-// sum0 + sum1 * 2^32 mod 2^64 in the word; as long as |sum0|, |sum1| < 2^31 the low half
-// sign-extended is sum0 and the rest is sum1.  c0 and c1 come from the FP64
-// round-to-integer FMA (the integer sits in the low mantissa bits of y - 1.5 * 2^52), so the
-// packed word is lo = lo32(y0), hi = hi32(y0) - 0x43380000 + lo32(y1).
-__device__ __forceinline__ void fx_add_pair(unsigned long long *p, double y0, double y1) {
-    const unsigned long long b0 = (unsigned long long)__double_as_longlong(y0);
-    const unsigned long long b1 = (unsigned long long)__double_as_longlong(y1);
-    const unsigned lo = (unsigned)b0;
-    const unsigned hi = (unsigned)(b0 >> 32) + (unsigned)b1 + 0xBCC80000u;
-    atomicAdd(p, ((unsigned long long)hi << 32) | lo);
+// 2p+1 in the high 32 bits, each an integer sum in units of 2^-E.  Every contribution c is
+// below 2^22 in magnitude (the exponent guarantees it), so the FP32 FMA y = w g + 1.5*2^23
+// forms w g exactly and rounds it to the integer c, which then sits in the mantissa:
+// bits(y) = 0x4B400000 + c.  The addend is the pair of raw bits (bits(y0), bits(y1)) as one
+// 64-bit integer; after n contributions the word holds
+//   n * 0x4B400000 * (1 + 2^32) + sum0 + sum1 * 2^32   (mod 2^64),
+// and with n from the per-sample count the bias is removed; as long as |sum0|, |sum1| < 2^31
+// the low half sign-extended is sum0 and the rest is sum1.
+constexpr float kFxMagicF = 12582912.0f;                        // 1.5 * 2^23
+constexpr unsigned long long kFxMagicBitsF = 0x4B400000ull;
+
+__device__ __forceinline__ void fx_add_pair(unsigned long long *p, float y0, float y1) {
+    atomicAdd(p, ((unsigned long long)__float_as_uint(y1) << 32) | __float_as_uint(y0));
 }
 
-__device__ __forceinline__ void fx_unpack_pair(unsigned long long w, long long &s0, long long &s1) {
+__device__ __forceinline__ void fx_unpack_pair(unsigned long long w, unsigned n, long long &s0,
+                                               long long &s1) {
+    const unsigned long long bias = (unsigned long long)n * kFxMagicBitsF;
+    w -= bias + (bias << 32);
     s0 = (long long)(int)(unsigned)w;
     s1 = ((long long)w - s0) >> 32;
 }
@@ -784,11 +846,19 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
         const double bound = (double)__uint_as_float(maxes[0]) * (double)__uint_as_float(maxes[1]);
         if (bound > 0.0) frexp(bound, &eM);
     }
+    // PACK scales the weights in FP32: 2^E must keep max|w| * 2^E finite (a grad_out below
+    // about 2^(-100) max|w|... is then resolved more coarsely than 2^-24 of its largest term)
+    int eW = 0;
+    if (PACK) {
+        const float wmax = __uint_as_float(maxes[0]);
+        if (wmax > 0.0f) frexpf(wmax, &eW);
+    }
     auto exponent = [&](int logc) {
-        // PACK: sums < 2^30; !PACK: contributions < 2^min(40, 61 - logc) (FP64 trick range,
-        // 63-bit sums)
+        // PACK: sums < 2^30 and contributions < 2^22 (FP32 trick range, logc >= 8);
+        // !PACK: contributions < 2^min(40, 61 - logc) (FP64 trick range, 63-bit sums)
         const int top = PACK ? 30 - logc - (CPLX ? 1 : 0) : min(40, 61 - logc);
-        const int E = top - eM;
+        int E = top - eM;
+        if (PACK) E = max(-126, min(E, 126 - eW));
         return E < -900 ? -900 : (E > 900 ? 900 : E);
     };
     int E = exponent(PACK ? kLogCnt0 : log2cnt);
@@ -834,8 +904,9 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
     };
 
     for (int pass = 0; pass < 2; pass++) {
-        const bool counting = PACK && pass == 0;
+        const bool counting = PACK && pass == 0;   // pass 1 keeps pass 0's counts
         const double sc = ldexp(1.0, E);
+        const float scf = PACK ? ldexpf(1.0f, E) : 0.0f;
         for (int i = threadIdx.x; i < S32 * WS; i += blockDim.x) lds[i] = 0ull;
         if (counting)
             for (int i = threadIdx.x; i < S32; i += blockDim.x) cnt[i] = 0u;
@@ -851,52 +922,65 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
                 atomicAdd(cnt + s0, 1u);
                 atomicAdd(cnt + s1, 1u);
             }
-            const double t = (double)tp.t;
-            const double wd = (double)w * sc;
-            if constexpr (!CPLX) {
-                const double w1 = wd * t, w0 = wd - w1;
-                if constexpr (PACK) {
+            if constexpr (PACK) {
+                // FP32: weights scaled by 2^E, the FMA rounds each product to its integer
+                const float tf = tp.t;
+                const float wf = w * scf;
+                if constexpr (!CPLX) {
+                    const float w1 = wf * tf, w0 = wf - w1;
 #pragma unroll
                     for (int p = 0; p < NW; p++) {
-                        const double ga = (double)v.gv[2 * p], gb = (double)v.gv[2 * p + 1];
-                        fx_add_pair(l0 + p * S32, __builtin_fma(w0, ga, kFxMagic), __builtin_fma(w0, gb, kFxMagic));
-                        fx_add_pair(l1 + p * S32, __builtin_fma(w1, ga, kFxMagic), __builtin_fma(w1, gb, kFxMagic));
+                        const float ga = (float)v.gv[2 * p], gb = (float)v.gv[2 * p + 1];
+                        fx_add_pair(l0 + p * S32, __builtin_fmaf(w0, ga, kFxMagicF),
+                                    __builtin_fmaf(w0, gb, kFxMagicF));
+                        fx_add_pair(l1 + p * S32, __builtin_fmaf(w1, ga, kFxMagicF),
+                                    __builtin_fmaf(w1, gb, kFxMagicF));
                     }
                 } else {
+                    float sn, cs;
+                    phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
+                    const float cr = wf * cs, ci = wf * sn;
+                    const float omt = 1.0f - tf;
+                    // conj(coef) * g = (cr gr + ci gi) + i (cr gi - ci gr), split by (1-t), t
+                    const float a_r = cr * omt, a_i = ci * omt, b_r = cr * tf, b_i = ci * tf;
+                    // words [p][re, im], each packing frames 2p, 2p+1
+#pragma unroll
+                    for (int p = 0; p < NW; p++) {
+                        const float gr0 = (float)v.gv[4 * p], gi0 = (float)v.gv[4 * p + 1];
+                        const float gr1 = (float)v.gv[4 * p + 2], gi1 = (float)v.gv[4 * p + 3];
+                        fx_add_pair(l0 + (2 * p) * S32,
+                                    __builtin_fmaf(a_r, gr0, __builtin_fmaf(a_i, gi0, kFxMagicF)),
+                                    __builtin_fmaf(a_r, gr1, __builtin_fmaf(a_i, gi1, kFxMagicF)));
+                        fx_add_pair(l0 + (2 * p + 1) * S32,
+                                    __builtin_fmaf(a_r, gi0, __builtin_fmaf(-a_i, gr0, kFxMagicF)),
+                                    __builtin_fmaf(a_r, gi1, __builtin_fmaf(-a_i, gr1, kFxMagicF)));
+                        fx_add_pair(l1 + (2 * p) * S32,
+                                    __builtin_fmaf(b_r, gr0, __builtin_fmaf(b_i, gi0, kFxMagicF)),
+                                    __builtin_fmaf(b_r, gr1, __builtin_fmaf(b_i, gi1, kFxMagicF)));
+                        fx_add_pair(l1 + (2 * p + 1) * S32,
+                                    __builtin_fmaf(b_r, gi0, __builtin_fmaf(-b_i, gr0, kFxMagicF)),
+                                    __builtin_fmaf(b_r, gi1, __builtin_fmaf(-b_i, gr1, kFxMagicF)));
+                    }
+                }
+                return;
+            } else {
+                // one 64-bit integer per frame: the FP64 FMA rounds each product
+                const double t = (double)tp.t;
+                const double wd = (double)w * sc;
+                if constexpr (!CPLX) {
+                    const double w1 = wd * t, w0 = wd - w1;
 #pragma unroll
                     for (int f = 0; f < FS; f++) {
                         const double gd = (double)v.gv[f];
                         fx_add(l0 + f * S32, w0, gd);
                         fx_add(l1 + f * S32, w1, gd);
                     }
-                }
-            } else {
-                float sn, cs;
-                phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
-                const double cr = wd * (double)cs, ci = wd * (double)sn;
-                const double omt = 1.0 - t;
-                // conj(coef) * g = (cr gr + ci gi) + i (cr gi - ci gr), split by (1-t) and t
-                const double a_r = cr * omt, a_i = ci * omt, b_r = cr * t, b_i = ci * t;
-                if constexpr (PACK) {
-                    // words [p][re, im], each packing frames 2p, 2p+1
-#pragma unroll
-                    for (int p = 0; p < NW; p++) {
-                        const double gr0 = (double)v.gv[4 * p], gi0 = (double)v.gv[4 * p + 1];
-                        const double gr1 = (double)v.gv[4 * p + 2], gi1 = (double)v.gv[4 * p + 3];
-                        fx_add_pair(l0 + (2 * p) * S32,
-                                    __builtin_fma(a_r, gr0, __builtin_fma(a_i, gi0, kFxMagic)),
-                                    __builtin_fma(a_r, gr1, __builtin_fma(a_i, gi1, kFxMagic)));
-                        fx_add_pair(l0 + (2 * p + 1) * S32,
-                                    __builtin_fma(a_r, gi0, __builtin_fma(-a_i, gr0, kFxMagic)),
-                                    __builtin_fma(a_r, gi1, __builtin_fma(-a_i, gr1, kFxMagic)));
-                        fx_add_pair(l1 + (2 * p) * S32,
-                                    __builtin_fma(b_r, gr0, __builtin_fma(b_i, gi0, kFxMagic)),
-                                    __builtin_fma(b_r, gr1, __builtin_fma(b_i, gi1, kFxMagic)));
-                        fx_add_pair(l1 + (2 * p + 1) * S32,
-                                    __builtin_fma(b_r, gi0, __builtin_fma(-b_i, gr0, kFxMagic)),
-                                    __builtin_fma(b_r, gi1, __builtin_fma(-b_i, gr1, kFxMagic)));
-                    }
                 } else {
+                    float sn, cs;
+                    phase<PRECISE>(g.rev_h, g.rev_l, tp, sn, cs);
+                    const double cr = wd * (double)cs, ci = wd * (double)sn;
+                    const double omt = 1.0 - t;
+                    const double a_r = cr * omt, a_i = ci * omt, b_r = cr * t, b_i = ci * t;
 #pragma unroll
                     for (int f = 0; f < FS; f++) {
                         const double gr = (double)v.gv[2 * f], gi = (double)v.gv[2 * f + 1];
@@ -964,7 +1048,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
 #pragma unroll
                 for (int c = 0; c < CW; c++) {
                     long long v0, v1;
-                    fx_unpack_pair(wp[(size_t)(p * CW + c) * S32], v0, v1);
+                    fx_unpack_pair(wp[(size_t)(p * CW + c) * S32], cnt[lds_slot(k)], v0, v1);
                     if (2 * p < nf) dst[(int64_t)(2 * p) * gx_b * CW + c] = (A)((double)v0 * inv);
                     if (2 * p + 1 < nf) dst[(int64_t)(2 * p + 1) * gx_b * CW + c] = (A)((double)v1 * inv);
                 }
@@ -1747,6 +1831,20 @@ hipError_t launch_fwd_batched_t(hipStream_t st, const std::vector<BatchMember> &
 template <Storage SS, class DST, bool CPLX, int F>
 hipError_t launch_relayout_f(hipStream_t st, const DasProblem &p, const void *x, void *dst) {
     constexpr int CW = CPLX ? 2 : 1;
+    constexpr size_t ES = sizeof(typename dev::Elem<SS>::raw) * CW;   // bytes per sample
+    const bool vec = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (p.xs_b * ES) % 16 == 0 &&
+                     (p.xs_c * ES) % 16 == 0 && (p.xs_t * ES) % 16 == 0 &&
+                     ((size_t)p.S * ES) % 16 == 0 && (F * CW * sizeof(DST)) % 16 == 0;
+    if (vec) {
+        const int64_t blocks = (int64_t)((p.S + dev::kRelL - 1) / dev::kRelL) * p.C * p.T;
+        if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+        dim3 grid((unsigned)blocks, (p.B + F - 1) / F);
+        ProfScope ps(st, "relayout_fi");
+        dev::relayout_fi_vec_kernel<SS, DST, F, CW><<<grid, 256, 0, st>>>(
+            static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B,
+            p.C, p.T, p.S, static_cast<DST *>(dst));
+        return hipGetLastError();
+    }
     const int64_t blocks = (int64_t)((p.S + 63) / 64) * p.C * p.T;
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
     dim3 grid((unsigned)blocks, (p.B + F - 1) / F);
