@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--c5-probes", default="linear64,linear256,matrix1024")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the built-in launch heuristics instead of autotuned parameters")
+    ap.add_argument("--tuning-cache", default=None, metavar="FILE",
+                    help="load tuned launch parameters from FILE if it exists, else autotune "
+                         "and save them there (a profiled run then times only the tuned "
+                         "configuration)")
     ap.add_argument("--profile-only", type=int, default=0, metavar="N",
                     help="run N steps and a 256 MiB calibration copy, print no JSON")
     return ap.parse_args()
@@ -295,12 +299,18 @@ def main():
     p = Problem(w, dev, a.storage)
     U = w.units
     tuned = None
-    if not a.no_autotune:
+    if a.tuning_cache and os.path.exists(a.tuning_cache):
+        ffdas_amd.load_tuning(a.tuning_cache)
+        tuned = {"loaded": a.tuning_cache}
+        log(f"tuning loaded from {a.tuning_cache}")
+    elif not a.no_autotune:
         t0 = time.time()
         tuned = tune(p, p.x)
         if a.storage != "fp32":
             tune(p, p.x32)
         log(f"autotuned in {time.time() - t0:.1f}s: {tuned}")
+        if a.tuning_cache:
+            ffdas_amd.save_tuning(a.tuning_cache)
     if a.profile_only:
         profile_only(p, p.x, a.profile_only)
         return
