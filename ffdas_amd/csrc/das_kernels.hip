@@ -756,6 +756,78 @@ __device__ __forceinline__ void fx_add2(unsigned long long *p, double w1, double
     atomicAdd(p, (unsigned long long)__double_as_longlong(y) - kFxMagicBits);
 }
 
+// Inverted index of a sparse table (K, N) of transmit indices: for every transmit s the
+// entries q * K + j (voxel q in axial order, slot j) with sparse[j][n(q)] == s, so the grad_x
+// row of transmit s visits only them.  Entries with an index outside [0, T) select nothing.
+// Counts and placements are aggregated per block in LDS (one global atomic per block and
+// transmit); the order inside a transmit's range is not fixed, the fixed-point sums it feeds
+// do not depend on it.
+constexpr int kMaxSparseT = 4096;
+
+__global__ __launch_bounds__(256) void sparse_count_kernel(const int *__restrict__ sparse,
+                                                           int64_t entries, int T,
+                                                           unsigned *__restrict__ count) {
+    __shared__ unsigned h[kMaxSparseT];
+    const bool local = T <= kMaxSparseT;
+    if (local)
+        for (int i = threadIdx.x; i < T; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < entries;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int t = sparse[e];
+        if ((unsigned)t >= (unsigned)T) continue;
+        if (local) atomicAdd(h + t, 1u);
+        else atomicAdd(count + t, 1u);
+    }
+    __syncthreads();
+    if (local)
+        for (int i = threadIdx.x; i < T; i += blockDim.x)
+            if (h[i]) atomicAdd(count + i, h[i]);
+}
+
+// start[t] = sum of count[0..t), start[T] = total (one thread; T is a transmit count).
+__global__ void sparse_scan_kernel(const unsigned *__restrict__ count, int T,
+                                   unsigned *__restrict__ start) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned acc = 0;
+    for (int t = 0; t < T; t++) {
+        start[t] = acc;
+        acc += count[t];
+    }
+    start[T] = acc;
+}
+
+__global__ __launch_bounds__(256) void sparse_fill_kernel(const int *__restrict__ sparse, int K,
+                                                          int N, int T, Axial ax,
+                                                          const unsigned *__restrict__ start,
+                                                          unsigned *__restrict__ cursor,
+                                                          int *__restrict__ list) {
+    __shared__ unsigned h[kMaxSparseT], base[kMaxSparseT];
+    const bool local = T <= kMaxSparseT;
+    const int64_t entries = (int64_t)K * N;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local)
+        for (int i = threadIdx.x; i < T; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    int t = -1;
+    unsigned rank = 0;
+    if (e < entries) {
+        t = sparse[e];
+        if ((unsigned)t >= (unsigned)T) t = -1;
+    }
+    if (t >= 0 && local) rank = atomicAdd(h + t, 1u);
+    __syncthreads();
+    if (local)
+        for (int i = threadIdx.x; i < T; i += blockDim.x)
+            base[i] = h[i] ? atomicAdd(cursor + i, h[i]) : 0u;
+    __syncthreads();
+    if (t < 0) return;
+    const unsigned pos = start[t] + (local ? base[t] + rank : atomicAdd(cursor + t, 1u));
+    const int j = (int)(e / N), n = (int)(e - (int64_t)j * N);
+    const int d0 = n / ax.R, r = n - d0 * ax.R;      // n = d0 * R + r  ->  q = r * D0 + d0
+    list[pos] = (r * ax.D0 + d0) * K + j;
+}
+
 // LDS slot of sample k: within each 32-sample block the low five bits are XORed with the
 // block number, so samples a multiple of 4 or 8 apart (neighbouring voxels along depth) land
 // in different LDS banks.  A bijection on [0, 32 * ceil(S / 32)).
@@ -818,7 +890,9 @@ template <bool ACC64, bool CPLX, bool PRECISE, int FS>
 __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const void *__restrict__ gtv,
                                                          const unsigned *__restrict__ maxes,
                                                          int log2cnt, void *__restrict__ gxv,
-                                                         int64_t gx_b, int64_t gx_c, int64_t gx_t) {
+                                                         int64_t gx_b, int64_t gx_c, int64_t gx_t,
+                                                         const unsigned *__restrict__ sp_start,
+                                                         const int *__restrict__ sp_list) {
     using A = AccT<ACC64>;
     using U64 = unsigned long long;
     constexpr bool PACK = !ACC64 && FS >= 2;
@@ -876,7 +950,11 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
     const int D0T = 1 << ax.lg, RT = 64 >> ax.lg;
     const int dl = lane & (D0T - 1), rl = lane >> ax.lg;
     const int per_r = (ax.D0 + D0T - 1) >> ax.lg;
-    const int nitems = per_r * ((ax.R + RT - 1) / RT);
+    // dense: every voxel of the grid; sparse: the (voxel, slot) entries of the inverted
+    // index that select this row's transmit, 64 per wave step
+    const unsigned e_lo = dense ? 0u : sp_start[s];
+    const unsigned e_hi = dense ? 0u : sp_start[s + 1];
+    const int nitems = dense ? per_r * ((ax.R + RT - 1) / RT) : (int)((e_hi - e_lo + 63u) >> 6);
     const A *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
 
     struct Vox {
@@ -886,21 +964,29 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
         bool valid;
     };
     auto fetch = [&](int item, Vox &v) {
-        const int rg = item / per_r;
-        const int d0 = (item - rg * per_r) * D0T + dl;
-        const int r = rg * RT + rl;
-        v.valid = d0 < ax.D0 && r < ax.R;
-        v.q = v.valid ? r * ax.D0 + d0 : 0;     // invalid lanes read voxel 0, never scatter
+        int64_t o = 0;    // offsets / weights entry of the voxel's slot
+        if (dense) {
+            const int rg = item / per_r;
+            const int d0 = (item - rg * per_r) * D0T + dl;
+            const int r = rg * RT + rl;
+            v.valid = d0 < ax.D0 && r < ax.R;
+            v.q = v.valid ? r * ax.D0 + d0 : 0;     // invalid lanes read voxel 0, never scatter
+            o = (int64_t)s * g.N + v.q;
+        } else {
+            const unsigned e = e_lo + (unsigned)item * 64u + (unsigned)lane;
+            v.valid = e < e_hi;
+            const int code = v.valid ? sp_list[e] : 0;      // q * K + j
+            v.q = code / g.K;
+            o = (int64_t)(code - v.q * g.K) * g.N + v.q;
+        }
         const A *gp = gchunk + (int64_t)v.q * FS * CW;
 #pragma unroll
         for (int i = 0; i < FS * CW; i++) v.gv[i] = gp[i];
         v.x = g.dstpos[3 * (int64_t)v.q];
         v.y = g.dstpos[3 * (int64_t)v.q + 1];
         v.z = g.dstpos[3 * (int64_t)v.q + 2];
-        if (dense) {
-            v.ofs = g.offsets[(int64_t)s * g.N + v.q];
-            v.w = g.weights[(int64_t)s * g.N + v.q];
-        }
+        v.ofs = g.offsets[o];
+        v.w = g.weights[o];
     };
 
     for (int pass = 0; pass < 2; pass++) {
@@ -997,15 +1083,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
             const SplitPt yp = split_point(v.x, v.y, v.z);
             const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
             if (g.srcdir && !((R.dx * sdx + R.dy * sdy + R.dz * sdz) * R.rinv > sdw)) return;
-            if (dense) {
-                scatter(v, R, v.ofs, v.w);
-            } else {
-                for (int j = 0; j < g.K; j++) {
-                    const int64_t o = (int64_t)j * g.N + v.q;
-                    if (g.sparse[o] != s) continue;
-                    scatter(v, R, g.offsets[o], g.weights[o]);
-                }
-            }
+            scatter(v, R, v.ofs, v.w);
         };
 
         // two voxel buffers in turn: the next voxel's loads overlap this one's scatter
@@ -1674,10 +1752,28 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
     ga.offsets = perm + 3 * (int64_t)g.N;
     ga.weights = ga.offsets + (int64_t)g.K * g.N;
     ga.sparse = g.sparse ? reinterpret_cast<const int *>(ga.weights + (int64_t)g.K * g.N) : nullptr;
+    // sparse: inverted index [count T][start T+1][cursor T][list K*N] after the geometry
+    unsigned *sp_start = nullptr;
+    int *sp_list = nullptr;
+    if (g.sparse) {
+        unsigned *count = reinterpret_cast<unsigned *>(perm + (3 + 3 * (int64_t)g.K) * g.N);
+        sp_start = count + g.T;
+        unsigned *cursor = sp_start + g.T + 1;
+        sp_list = reinterpret_cast<int *>(cursor + g.T);
+        const int64_t entries = (int64_t)g.K * g.N;
+        dev::fill_zero_kernel<<<(g.T + 255) / 256, 256, 0, st>>>(reinterpret_cast<float *>(count), g.T);
+        dev::fill_zero_kernel<<<(g.T + 255) / 256, 256, 0, st>>>(reinterpret_cast<float *>(cursor), g.T);
+        const int cgrid = (int)std::min<int64_t>((entries + 255) / 256, 2048);
+        dev::sparse_count_kernel<<<cgrid, 256, 0, st>>>(g.sparse, entries, g.T, count);
+        dev::sparse_scan_kernel<<<1, 64, 0, st>>>(count, g.T, sp_start);
+        if ((entries + 255) / 256 > 0x7fffffff) return hipErrorInvalidValue;
+        dev::sparse_fill_kernel<<<(unsigned)((entries + 255) / 256), 256, 0, st>>>(
+            g.sparse, g.K, g.N, g.T, ax, sp_start, cursor, sp_list);
+    }
     if (const char *env = std::getenv("FFDAS_BWDX_THREADS")) threads = std::atoi(env);
     if (threads != 256 && threads != 512 && threads != 1024) threads = 1024;
     dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, threads, lds, st>>>(
-        ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t);
+        ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t, sp_start, sp_list);
     return hipGetLastError();
 }
 
@@ -2059,7 +2155,8 @@ size_t backward_workspace(const DasProblem &p) {
     const size_t esz = 8 * (p.cplx ? 2 : 1);   // pre-scaled double grad_out
     const size_t nchunks = (size_t)(p.B + fs - 1) / fs;
     const size_t perm = (size_t)p.N * (3 + 3 * (size_t)p.K) * sizeof(float);   // axial geometry
-    return 256 + nchunks * (size_t)fs * (size_t)p.N * esz + perm;
+    const size_t inv = p.sparse ? ((3 * (size_t)p.T + 1) + (size_t)p.K * p.N) * 4 : 0;
+    return 256 + nchunks * (size_t)fs * (size_t)p.N * esz + perm + inv;
 }
 
 // Partial buffers of the parameter adjoint (see ParamOut), carved from `pws`; the layout

@@ -222,3 +222,21 @@ def test_reuse_x_matches_a_fresh_copy(dev):
     for a, b, c in zip(fresh, reused, other):
         assert torch.equal(a, b)
         assert torch.equal(a, c)
+
+
+@pytest.mark.parametrize("K,T", [(3, 5), (1, 4), (6, 2)])
+def test_sparse_adjoint_inverted_index(dev, K, T):
+    """The sparse grad_x visits each transmit's entries through an inverted index; indices
+    outside [0, T) select nothing, in the forward and in every gradient."""
+    w = W.small_random(seed=70 + K, B=5, C=4, T=T, K=K, S=110, N=(9, 8), sparse=True)
+    bad = np.random.default_rng(K).random(w.sparse_indices.shape) < 0.15
+    w.sparse_indices[bad] = np.where(np.random.default_rng(1).random(bad.sum()) < 0.5, -1, T)
+    rng = np.random.default_rng(11)
+    gout = rng.standard_normal((5,) + w.dstpos.shape[:-1]).astype(np.float32)
+    ref = oracle.das_adjoint(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             sparse_indices=w.sparse_indices)
+    gx, go, gw, gs = _grads(w, dev, gout)
+    assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
+    assert rel_l2(go.cpu().numpy(), ref["grad_offsets"]) < 1e-4
+    assert rel_l2(gw.cpu().numpy(), ref["grad_weights"]) < 1e-5
+    assert abs(float(gs) - ref["grad_scale"]) <= 1e-4 * max(1.0, abs(ref["grad_scale"]))
