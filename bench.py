@@ -102,6 +102,21 @@ def tune(p, x):
     return ffdas_amd.autotune(x, p.srcpos, p.dstpos, p.offsets, p.weights, reps=3)
 
 
+def bwdx_frames_used(a, tuned):
+    """Frames per grad_x chunk of the timed storage type: from the autotuner's result or the
+    loaded tuning file; 16 (the library's default for real data here) when untuned."""
+    fs = 0
+    if tuned and "bwdx_frames" in tuned:
+        fs = tuned["bwdx_frames"]
+    elif tuned and "loaded" in tuned:
+        tag = "|x=" + {"fp16": "f16", "bf16": "bf16", "fp32": "f32"}[a.storage] + "|"
+        with open(tuned["loaded"]) as f:
+            for line in f:
+                if tag in line:
+                    fs = int(line.split()[6])
+    return fs if fs > 0 else 16
+
+
 def time_steps(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -337,8 +352,25 @@ def main():
                    "das_bwd_param": 2 * esz, "das_bwd_x": 2 * 4, "das_bwd_x_plain": 2 * 4}
     dom = max((k for k in kt if k in bytes_per_u), key=kt.get)
     achieved = U * bytes_per_u[dom] / (kt[dom] * 1e-3) / 1e9
+    # on-chip figure of the dominant kernel: grad_x is bound by 64-bit LDS atomics (per tap
+    # and frame chunk of FS frames: FS/2 frame pairs at each of the two samples, plus two
+    # 32-bit per-sample counts, i.e. 1 + 2/FS per unit),
+    # measured on gfx950 at 8.9 lane-ops/clk/CU without bank conflicts; the gathers by the
+    # vector L1 (64 B/clk/CU), 2 x sizeof(storage) bytes per unit
+    clk, cus = 2.4e9, 256
+    if dom == "das_bwd_x":
+        fs = bwdx_frames_used(a, tuned)
+        ops = U * (1.0 + 2.0 / fs)
+        onchip = {"resource": "LDS 64-bit atomics", "unit": "lane-ops/clk/CU",
+                  "achieved": round(ops / (kt[dom] * 1e-3) / (clk * cus), 2), "peak": 8.9}
+    else:
+        onchip = {"resource": "vector L1", "unit": "B/clk/CU",
+                  "achieved": round(U * bytes_per_u[dom] / (kt[dom] * 1e-3) / (clk * cus), 1),
+                  "peak": 64.0}
+    onchip["frac"] = round(onchip["achieved"] / onchip["peak"], 3)
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "onchip": onchip,
             "traffic": traffic_for(a.storage, dom), "bytes_per_unit": bytes_per_u[dom],
             "units_per_launch": U,
             "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
