@@ -141,6 +141,7 @@ def lib():
         "ffdas_tune_load": (i32, [ctypes.c_char_p]),
         "ffdas_tune_clear": (i32, []),
         "ffdas_tune_count": (i32, []),
+        "ffdas_truncate_rank": (i32, [vp, vp, vp, i64, i64, vp, vp]),
         "ffdas_profile_enable": (i32, [i32]),
         "ffdas_profile_reset": (i32, []),
         "ffdas_profile_query": (i32, [ctypes.c_char_p, P(ctypes.c_double), P(i64)]),

@@ -190,6 +190,7 @@ ffdas_error_t ffdas_create(ffdas_handle_t *handle) {
 ffdas_error_t ffdas_destroy(ffdas_handle_t h) {
     if (!h) return FFDAS_ERROR_INVALID_HANDLE;
     ffdas_error_t err = FFDAS_SUCCESS;
+    ffdas::release_rank_solver(*h);
     if (h->ws) h->retired.push_back(h->ws);
     for (void *p : h->retired) {
         // release in stream order; if the runtime is already torn down, just leak

@@ -64,6 +64,9 @@ struct ffdas_context {
     } xcopy;
     bool reuse_x = false;
 
+    // rocBLAS handle used by the rank-truncation eigensolver (created on first use)
+    void *blas = nullptr;
+
     ffdas_error_t check_device() const;
     ffdas_error_t workspace(size_t bytes, void **ptr);
 };
@@ -180,5 +183,40 @@ void resolve_tune(const ffdas_context &h, DasProblem &p, bool has_srcdir);
 void tune_store(const std::string &key, const DasTune &t);
 size_t fi_bytes(const DasProblem &p);
 ffdas_error_t launch_relayout_fi(ffdas_context &h, const DasProblem &p, const void *x, void *dst);
+
+}  // namespace ffdas
+
+namespace ffdas {
+
+// One ffdas_truncate_rank call, as the kernels see it (rank_kernels.hip).  Element strides
+// are in elements (complex = one element).
+struct RankPlan {
+    bool dbl = false, cplx = false;
+    int64_t m = 0, n = 0;          // x is (m, n)
+    int p = 0;                     // min(m, n): Gram size
+    int64_t L = 0;                 // max(m, n): Gram vector length = projection vector count
+    int64_t gsa = 0, gsk = 0;      // Gram vector a, element k at a*gsa + k*gsk
+    bool gvec = false;             // 16-byte loads along k
+    int64_t xsj = 0, xsf = 0;      // projection vector j, element f of x at j*xsj + f*xsf
+    int64_t osj = 0, osf = 0;      // same for out
+    bool xvec = false, ovec = false;
+    bool keep = true;              // project onto the kept set (else subtract the dropped set)
+    int r = 0;                     // eigenvector columns used
+    int q_split = 0, c_first = 0, c_second = 0;
+    // launch shape (rank_plan_launch_shape)
+    int bs = 0, nb = 0, splits = 0, pp = 0, rp = 0, rt = 0, passes = 0;
+    int64_t kchunk = 0;
+    size_t part_bytes = 0;
+};
+
+void rank_plan_launch_shape(RankPlan &pl, int cu_count);
+ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x, void *part,
+                               void *H);
+ffdas_error_t launch_rank_project(hipStream_t st, const RankPlan &pl, const void *V,
+                                  const void *x, void *out, void *Qh, void *Qn);
+ffdas_error_t launch_rank_copy(hipStream_t st, bool dbl, bool cplx, const void *x, int64_t xs0,
+                               int64_t xs1, void *out, int64_t os0, int64_t os1, int64_t m,
+                               int64_t n);
+void release_rank_solver(ffdas_context &h);
 
 }  // namespace ffdas

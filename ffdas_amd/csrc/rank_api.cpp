@@ -1,0 +1,222 @@
+// ffdas_truncate_rank: rank truncation filter (reference: src/truncate_rank/truncate_rank.cu:6-42,
+// src/truncate_rank/truncate_rank_impl.cuh:382-531; Python bindings/python/ffdas/
+// truncate_rank.py:28-82).  Gram matrix and projection run on the kernels of
+// rank_kernels.hip; the p x p Hermitian eigendecomposition runs on rocSOLVER (syevd/heevd),
+// loaded on first use so that the library does not pull rocSOLVER in at load time.
+#include <dlfcn.h>
+
+#include <climits>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "internal.h"
+
+namespace ffdas {
+namespace {
+
+struct Solver {
+    bool ok = false;
+    decltype(&rocblas_create_handle) create = nullptr;
+    decltype(&rocblas_destroy_handle) destroy = nullptr;
+    decltype(&rocblas_set_stream) set_stream = nullptr;
+    decltype(&rocsolver_ssyevd) ssyevd = nullptr;
+    decltype(&rocsolver_dsyevd) dsyevd = nullptr;
+    decltype(&rocsolver_cheevd) cheevd = nullptr;
+    decltype(&rocsolver_zheevd) zheevd = nullptr;
+};
+
+void *open_lib(const char *name) {
+    const char *root = std::getenv("ROCM_PATH");
+    const std::string dir = std::string(root && *root ? root : "/opt/rocm") + "/lib/";
+    void *h = dlopen((dir + name).c_str(), RTLD_NOW | RTLD_LOCAL);
+    return h ? h : dlopen(name, RTLD_NOW | RTLD_LOCAL);
+}
+
+const Solver &solver() {
+    static Solver s;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        void *blas = open_lib("librocblas.so.5");
+        void *sol = open_lib("librocsolver.so.0");
+        if (!blas || !sol) return;
+        s.create = (decltype(s.create))dlsym(blas, "rocblas_create_handle");
+        s.destroy = (decltype(s.destroy))dlsym(blas, "rocblas_destroy_handle");
+        s.set_stream = (decltype(s.set_stream))dlsym(blas, "rocblas_set_stream");
+        s.ssyevd = (decltype(s.ssyevd))dlsym(sol, "rocsolver_ssyevd");
+        s.dsyevd = (decltype(s.dsyevd))dlsym(sol, "rocsolver_dsyevd");
+        s.cheevd = (decltype(s.cheevd))dlsym(sol, "rocsolver_cheevd");
+        s.zheevd = (decltype(s.zheevd))dlsym(sol, "rocsolver_zheevd");
+        s.ok = s.create && s.destroy && s.set_stream && s.ssyevd && s.dsyevd && s.cheevd &&
+               s.zheevd;
+    });
+    return s;
+}
+
+size_t align_up(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Eigendecomposition of the column-major p x p Hermitian H in place (eigenvectors in the
+// columns, eigenvalues ascending in D), then a check of rocSOLVER's info on the host -- the
+// one synchronisation of the call, as the reference's (truncate_rank_impl.cuh:82-87).
+ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E, int *info) {
+    const Solver &s = solver();
+    if (!s.ok) return FFDAS_ERROR_CUSOLVER;
+    if (!h.blas) {
+        rocblas_handle bh = nullptr;
+        if (s.create(&bh) != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
+        h.blas = bh;
+    }
+    rocblas_handle bh = (rocblas_handle)h.blas;
+    if (s.set_stream(bh, h.stream) != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
+    const rocblas_evect ev = rocblas_evect_original;
+    const rocblas_fill up = rocblas_fill_upper;
+    rocblas_status st;
+    if (pl.dbl && pl.cplx)
+        st = s.zheevd(bh, ev, up, pl.p, (rocblas_double_complex *)H, pl.p, (double *)D,
+                      (double *)E, info);
+    else if (pl.dbl)
+        st = s.dsyevd(bh, ev, up, pl.p, (double *)H, pl.p, (double *)D, (double *)E, info);
+    else if (pl.cplx)
+        st = s.cheevd(bh, ev, up, pl.p, (rocblas_float_complex *)H, pl.p, (float *)D,
+                      (float *)E, info);
+    else
+        st = s.ssyevd(bh, ev, up, pl.p, (float *)H, pl.p, (float *)D, (float *)E, info);
+    if (st != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
+    int hinfo = 0;
+    if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, h.stream) != hipSuccess)
+        return FFDAS_ERROR_CUDA_MEMCPY_FAILED;
+    if (hipStreamSynchronize(h.stream) != hipSuccess) return FFDAS_ERROR_CUDA_SYNC_FAILED;
+    return hinfo == 0 ? FFDAS_SUCCESS : FFDAS_ERROR_CUSOLVER;
+}
+
+bool overlaps(const void *a, size_t na, const void *b, size_t nb) {
+    const char *x = (const char *)a, *y = (const char *)b;
+    return x < y + nb && y < x + na;
+}
+
+}  // namespace
+
+void release_rank_solver(ffdas_context &h) {
+    if (h.blas) {
+        const Solver &s = solver();
+        if (s.ok) s.destroy((rocblas_handle)h.blas);
+        h.blas = nullptr;
+    }
+}
+
+}  // namespace ffdas
+
+using namespace ffdas;
+
+extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
+                                             const void *x, int64_t start, int64_t stop,
+                                             ffdas_tensor_desc_t out_desc, void *out) {
+    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
+    if (!x_desc || !out_desc) return FFDAS_ERROR_NULL_POINTER;
+    FFDAS_TRY(handle->check_device());
+    const ffdas_tensor_desc xd = *x_desc, od = *out_desc;
+    if (xd.dtype != od.dtype) return FFDAS_ERROR_UNSUPPORTED_TYPE;
+    RankPlan pl;
+    switch (xd.dtype) {
+        case FFDAS_R_32F: break;
+        case FFDAS_C_32F: pl.cplx = true; break;
+        case FFDAS_R_64F: pl.dbl = true; break;
+        case FFDAS_C_64F: pl.dbl = pl.cplx = true; break;
+        default: return FFDAS_ERROR_UNSUPPORTED_TYPE;
+    }
+    if (xd.dims.size() != 2 || od.dims != xd.dims) return FFDAS_ERROR_INVALID_DIMS;
+    if (xd.dims[0] > INT_MAX || xd.dims[1] > INT_MAX) return FFDAS_ERROR_INVALID_DIMS;
+    const int64_t m = xd.dims[0], n = xd.dims[1];
+    const int64_t p = m < n ? m : n;
+    // the reference validates the range in its bindings (truncate_rank.py:58-66); the C entry
+    // point checks it too, since the kernels index eigenvector columns with it
+    if (p < 1 || start < 0 || start >= p || stop <= start || stop > p)
+        return FFDAS_ERROR_INVALID_ARGUMENT;
+    if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
+    const int64_t xs0 = xd.strides[0], xs1 = xd.strides[1];
+    const int64_t os0 = od.strides[0], os1 = od.strides[1];
+    if (xs0 < 0 || xs1 < 0) return FFDAS_ERROR_NON_CONTIGUOUS;
+    // out must not overlap itself: row-major or column-major dense layouts, or any strides
+    // that keep distinct elements apart
+    const bool out_ok = (os1 >= 1 && os0 >= os1 * n) || (os0 >= 1 && os1 >= os0 * m) || m * n <= 1;
+    if (!out_ok) return FFDAS_ERROR_NON_CONTIGUOUS;
+
+    const size_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
+    const size_t rsz = pl.dbl ? 8 : 4;
+    hipStream_t st = handle->stream;
+    const int64_t kept = stop - start, dropped = p - kept;
+    if (dropped == 0) {
+        // every component kept: the reconstruction is x itself
+        if (x == out && xs0 == os0 && xs1 == os1) return FFDAS_SUCCESS;
+        return launch_rank_copy(st, pl.dbl, pl.cplx, x, xs0, xs1, out, os0, os1, m, n);
+    }
+
+    pl.m = m;
+    pl.n = n;
+    pl.p = (int)p;
+    const bool rows = m <= n;   // Gram of the rows (m x m), projection of the columns
+    pl.L = rows ? n : m;
+    pl.gsa = rows ? xs0 : xs1;
+    pl.gsk = rows ? xs1 : xs0;
+    pl.xsj = rows ? xs1 : xs0;
+    pl.xsf = rows ? xs0 : xs1;
+    pl.osj = rows ? os1 : os0;
+    pl.osf = rows ? os0 : os1;
+    const auto aligned = [&](const void *ptr, int64_t stride_between) {
+        return ((uintptr_t)ptr % 16 == 0) && ((stride_between * (int64_t)esz) % 16 == 0);
+    };
+    pl.gvec = pl.gsk == 1 && aligned(x, pl.gsa);
+    pl.keep = kept <= dropped;
+    pl.r = (int)(pl.keep ? kept : dropped);
+    // eigenvalues ascend: component i (descending) is column p-1-i
+    if (pl.keep) {
+        pl.q_split = pl.r;
+        pl.c_first = (int)(p - stop);
+        pl.c_second = 0;
+    } else {
+        pl.q_split = (int)(p - stop);   // columns [0, p-stop) and [p-start, p)
+        pl.c_first = 0;
+        pl.c_second = (int)(p - start);
+    }
+    rank_plan_launch_shape(pl, handle->cu_count);
+
+    // x is read again by later passes: an output that overlaps it needs a private copy
+    const size_t xspan = (size_t)((m - 1) * xs0 + (n - 1) * xs1 + 1) * esz;
+    const size_t ospan = (size_t)((m - 1) * os0 + (n - 1) * os1 + 1) * esz;
+    const bool need_copy = pl.passes > 1 && overlaps(x, xspan, out, ospan);
+
+    const size_t b_part = align_up(pl.part_bytes);
+    const size_t b_H = align_up((size_t)p * p * esz);
+    const size_t b_D = align_up((size_t)p * rsz);
+    const size_t b_info = align_up(sizeof(int));
+    const size_t b_Q = align_up((size_t)pl.rp * pl.pp * esz);
+    const size_t b_x = need_copy ? align_up((size_t)m * n * esz) : 0;
+    void *ws = nullptr;
+    FFDAS_TRY(handle->workspace(b_part + b_H + 2 * b_D + b_info + 2 * b_Q + b_x, &ws));
+    handle->xcopy.valid = false;   // the converted-x record of the delay-and-sum calls
+    char *w = (char *)ws;
+    void *part = w;
+    void *H = w + b_part;
+    void *D = w + b_part + b_H;
+    void *E = w + b_part + b_H + b_D;
+    int *info = (int *)(w + b_part + b_H + 2 * b_D);
+    void *Qh = w + b_part + b_H + 2 * b_D + b_info;
+    void *Qn = (char *)Qh + b_Q;
+    const void *xs = x;
+    if (need_copy) {
+        void *xc = (char *)Qn + b_Q;
+        FFDAS_TRY(launch_rank_copy(st, pl.dbl, pl.cplx, x, xs0, xs1, xc, n, 1, m, n));
+        xs = xc;
+        pl.xsj = rows ? 1 : n;
+        pl.xsf = rows ? n : 1;
+    }
+    pl.xvec = pl.xsf == 1 && aligned(xs, pl.xsj);
+    pl.ovec = pl.osf == 1 && aligned(out, pl.osj);
+
+    FFDAS_TRY(launch_rank_gram(st, pl, x, part, H));
+    FFDAS_TRY(eigh(*handle, pl, H, D, E, info));
+    return launch_rank_project(st, pl, H, xs, out, Qh, Qn);
+}

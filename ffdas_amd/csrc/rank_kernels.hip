@@ -1,0 +1,624 @@
+// Rank truncation (truncated-SVD reconstruction) kernels for gfx950.
+//
+// Definition (reference: src/truncate_rank/truncate_rank_impl.cuh:382-451, docs/api/
+// truncate_rank.md): x is an (m, n) matrix; with p = min(m, n) and the p x p Gram matrix
+// of the shorter side, the output keeps the singular components [start, stop) (descending
+// order) and removes the rest.  Here the matrix is seen as two families of vectors:
+//   * Gram vectors:       p vectors of length L = max(m, n)  (rows when m <= n, else columns);
+//   * projection vectors: L vectors of length p              (columns when m <= n, else rows).
+// H[a][b] = sum_k v_a[k] conj(v_b[k]) / n is formed over the Gram vectors; with V the
+// eigenvectors of H (ascending eigenvalues, so component i is column p-1-i), every projection
+// vector w is replaced by V_k V_k^H w (kept set), or by w - V_d V_d^H w (dropped set) when the
+// dropped set is the smaller one -- the same projector, with min(kept, dropped) columns.
+// For m > n, H is conj(X^H X), whose eigenvectors are the conjugates of the right singular
+// vectors; V_k V_k^H applied to the transposed rows is then exactly x W_k W_k^H.
+//
+// Kernels (all matrix products on the 16x16x4 MFMA, FP32 or FP64, complex as four real
+// products):
+//   rank_gram_kernel      split-K Gram: one wave per (upper-triangle block, k-range),
+//                         block partials written in full, no atomics;
+//   rank_gram_reduce      sums the partials in a fixed order, scales by 1/n, mirrors the
+//                         lower triangle as the conjugate of the upper, column-major out;
+//   rank_select_kernel    gathers the chosen eigenvector columns into Qh (r x p, conjugated)
+//                         and Qn (p x r), zero padded to 16-multiples;
+//   rank_project_kernel   per wave 16 projection vectors: C = Qh w (MFMA, k over p), then
+//                         out = base +/- Qn C (MFMA, k over r), C kept in registers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+
+#include "internal.h"
+
+namespace ffdas {
+namespace {
+
+template <typename T>
+struct Mfma;
+
+// 16x16x4 MFMA: lane (jl, g) = (lane % 16, lane / 16) supplies A[jl][g] and B[g][jl] and
+// holds output register v of row row(g, v), column jl: 4g + v for FP32, 4v + g for FP64
+// (measured on gfx950).
+template <>
+struct Mfma<float> {
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    static constexpr bool kRowsInRegs = true;   // a lane's four outputs are consecutive rows
+    static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
+    static __device__ __forceinline__ V4 mma(float a, float b, V4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<double> {
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    static constexpr bool kRowsInRegs = false;  // rows g, g + 4, g + 8, g + 12
+    static __device__ __forceinline__ int row(int g, int v) { return 4 * v + g; }
+    static __device__ __forceinline__ V4 mma(double a, double b, V4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+// Four elements e, e+se, e+2se, e+3se of one vector starting at element offset `off`
+// (complex: interleaved pairs).  `n` of them are valid (the rest read as 0); `vec` selects
+// 16-byte loads when se == 1 and the run is 16-byte aligned (checked by the host).
+template <typename T, bool CPLX>
+__device__ __forceinline__ void load4(const T *__restrict__ x, int64_t off, int64_t se, int n,
+                                      bool vec, T (&re)[4], T (&im)[4]) {
+    // values go through scalars and are assigned once at the end: stores into the arrays from
+    // both branches would be merged into one store with a selected address, which forces the
+    // arrays into scratch memory
+    T r0 = 0, r1 = 0, r2 = 0, r3 = 0, i0 = 0, i1 = 0, i2 = 0, i3 = 0;
+    if (vec && n == 4) {
+        if constexpr (sizeof(T) == 4 && !CPLX) {
+            const float4 v = *reinterpret_cast<const float4 *>(x + off);
+            r0 = v.x; r1 = v.y; r2 = v.z; r3 = v.w;
+        } else if constexpr (sizeof(T) == 4 && CPLX) {
+            const float4 *q = reinterpret_cast<const float4 *>(x + 2 * off);
+            const float4 a = q[0], b = q[1];
+            r0 = a.x; i0 = a.y; r1 = a.z; i1 = a.w;
+            r2 = b.x; i2 = b.y; r3 = b.z; i3 = b.w;
+        } else if constexpr (!CPLX) {
+            const double2 *q = reinterpret_cast<const double2 *>(x + off);
+            const double2 a = q[0], b = q[1];
+            r0 = a.x; r1 = a.y; r2 = b.x; r3 = b.y;
+        } else {
+            const double2 *q = reinterpret_cast<const double2 *>(x + 2 * off);
+            const double2 a = q[0], b = q[1], c = q[2], d = q[3];
+            r0 = a.x; i0 = a.y; r1 = b.x; i1 = b.y;
+            r2 = c.x; i2 = c.y; r3 = d.x; i3 = d.y;
+        }
+    } else {
+        const int64_t c = CPLX ? 2 : 1;
+        if (n > 0) {
+            r0 = x[c * off];
+            if constexpr (CPLX) i0 = x[c * off + 1];
+        }
+        if (n > 1) {
+            r1 = x[c * (off + se)];
+            if constexpr (CPLX) i1 = x[c * (off + se) + 1];
+        }
+        if (n > 2) {
+            r2 = x[c * (off + 2 * se)];
+            if constexpr (CPLX) i2 = x[c * (off + 2 * se) + 1];
+        }
+        if (n > 3) {
+            r3 = x[c * (off + 3 * se)];
+            if constexpr (CPLX) i3 = x[c * (off + 3 * se) + 1];
+        }
+    }
+    re[0] = r0; re[1] = r1; re[2] = r2; re[3] = r3;
+    im[0] = i0; im[1] = i1; im[2] = i2; im[3] = i3;
+}
+
+// Four consecutive entries of a zero-padded row-major coefficient matrix (always in bounds).
+template <typename T, bool CPLX>
+__device__ __forceinline__ void coef4(const T *__restrict__ q, int64_t off, T (&re)[4],
+                                      T (&im)[4]) {
+    load4<T, CPLX>(q, off, 1, 4, true, re, im);
+}
+
+// ---------------------------------------------------------------------------------------
+// Gram matrix
+// ---------------------------------------------------------------------------------------
+// One wave: block (bi, bj), bi <= bj, of size BS = 16*WT, over k in [k_lo, k_hi).  Lane
+// (jl, g) = (lane % 16, lane / 16) feeds, at MFMA step s, row jl of a 16-row tile and k-slot
+// g, i.e. element k = kb + 4g + s; both operands use the same k mapping, so the sum over k is
+// exact whatever the order.  Output D[row(g, v)][jl] of tile (ta, tb) is H-partial
+// [16ta + row(g, v)][16tb + jl] of the block.
+template <typename T, bool CPLX, int WT>
+__global__ __launch_bounds__(64) void rank_gram_kernel(const T *__restrict__ x, int64_t sa,
+                                                       int64_t sk, int p, int64_t L,
+                                                       int64_t kchunk, int nb, bool vec,
+                                                       T *__restrict__ part) {
+    using M = Mfma<T>;
+    using V4 = typename M::V4;
+    constexpr int BS = 16 * WT;
+    // Workgroups are dealt to the eight XCDs in turn (id % 8); all blocks of one k-range are
+    // given ids on the same XCD, so the rows they share are fetched into one L2 once.
+    // splits is a multiple of 8.
+    const int nblk = nb * (nb + 1) / 2;
+    const int wid = blockIdx.x, xcd = wid & 7, wi = wid >> 3;
+    const int split = xcd + 8 * (wi / nblk);
+    int idx = wi % nblk, bi = 0;
+    while (idx >= nb - bi) {
+        idx -= nb - bi;
+        bi++;
+    }
+    const int bj = bi + idx;
+    const bool diag = bi == bj;
+    const int lane = threadIdx.x, jl = lane & 15, g = lane >> 4;
+    const int64_t k_lo = (int64_t)split * kchunk;
+    const int64_t k_hi = k_lo + kchunk < L ? k_lo + kchunk : L;   // empty when k_lo >= L
+
+    V4 accr[WT][WT], acci[WT][WT];
+#pragma unroll
+    for (int a = 0; a < WT; a++)
+#pragma unroll
+        for (int b = 0; b < WT; b++) {
+            accr[a][b] = V4{0, 0, 0, 0};
+            acci[a][b] = V4{0, 0, 0, 0};
+        }
+    int64_t offa[WT], offb[WT];
+    bool oka[WT], okb[WT];
+#pragma unroll
+    for (int t = 0; t < WT; t++) {
+        const int ra = bi * BS + 16 * t + jl, rb = bj * BS + 16 * t + jl;
+        oka[t] = ra < p;
+        okb[t] = rb < p;
+        offa[t] = (int64_t)ra * sa;
+        offb[t] = (int64_t)rb * sa;
+    }
+
+    for (int64_t kb = k_lo; kb < k_hi; kb += 16) {
+        const int64_t k0 = kb + 4 * g;
+        const int64_t left = k_hi - k0;
+        const int nk = left >= 4 ? 4 : (left > 0 ? (int)left : 0);
+        T ar[WT][4], ai[WT][4], br[WT][4], bim[WT][4];
+#pragma unroll
+        for (int t = 0; t < WT; t++)
+            load4<T, CPLX>(x, offa[t] + k0 * sk, sk, oka[t] ? nk : 0, vec, ar[t], ai[t]);
+        if (diag) {
+#pragma unroll
+            for (int t = 0; t < WT; t++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    br[t][s] = ar[t][s];
+                    bim[t][s] = ai[t][s];
+                }
+        } else {
+#pragma unroll
+            for (int t = 0; t < WT; t++)
+                load4<T, CPLX>(x, offb[t] + k0 * sk, sk, okb[t] ? nk : 0, vec, br[t], bim[t]);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int a = 0; a < WT; a++)
+#pragma unroll
+                for (int b = 0; b < WT; b++) {
+                    // v_a conj(v_b) = (ar br + ai bi) + i (ai br - ar bi)
+                    accr[a][b] = M::mma(ar[a][s], br[b][s], accr[a][b]);
+                    if constexpr (CPLX) {
+                        accr[a][b] = M::mma(ai[a][s], bim[b][s], accr[a][b]);
+                        acci[a][b] = M::mma(ai[a][s], br[b][s], acci[a][b]);
+                        acci[a][b] = M::mma(-ar[a][s], bim[b][s], acci[a][b]);
+                    }
+                }
+    }
+
+    constexpr int C = CPLX ? 2 : 1;
+    T *dst = part + ((int64_t)split * nblk + (wi % nblk)) * (int64_t)(BS * BS * C);
+#pragma unroll
+    for (int a = 0; a < WT; a++)
+#pragma unroll
+        for (int b = 0; b < WT; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int e = (16 * a + M::row(g, v)) * BS + 16 * b + jl;
+                if constexpr (CPLX) {
+                    dst[2 * e] = accr[a][b][v];
+                    dst[2 * e + 1] = acci[a][b][v];
+                } else {
+                    dst[e] = accr[a][b][v];
+                }
+            }
+}
+
+// H (column-major p x p) from the block partials: 1024 threads = 16 waves per 64 entries,
+// wave w sums the splits [w*S/16, (w+1)*S/16) in order, then the 16 sums are added in order.
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(1024) void rank_gram_reduce_kernel(const T *__restrict__ part,
+                                                                int p, int BS, int nb, int nblk,
+                                                                int splits, T scale,
+                                                                T *__restrict__ H) {
+    __shared__ T red[16][64][2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool ok = i < (int64_t)p * p;
+    T sr = 0, si = 0;
+    int r = 0, c = 0;
+    bool lower = false;
+    if (ok) {
+        r = (int)(i % p);
+        c = (int)(i / p);
+        lower = r > c;
+        const int a = lower ? c : r, b = lower ? r : c;
+        const int bi = a / BS, bj = b / BS;
+        const int blk = bi * nb - bi * (bi - 1) / 2 + (bj - bi);
+        const int64_t blkel = (int64_t)BS * BS;
+        const int64_t off = blk * blkel + (a % BS) * BS + (b % BS);
+        const int s0 = (int)((int64_t)splits * w / 16), s1 = (int)((int64_t)splits * (w + 1) / 16);
+        for (int s = s0; s < s1; s++) {
+            const int64_t e = (int64_t)s * nblk * blkel + off;
+            if constexpr (CPLX) {
+                sr += part[2 * e];
+                si += part[2 * e + 1];
+            } else {
+                sr += part[e];
+            }
+        }
+    }
+    red[w][lane][0] = sr;
+    red[w][lane][1] = si;
+    __syncthreads();
+    if (w != 0 || !ok) return;
+    T tr = 0, ti = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        tr += red[k][lane][0];
+        ti += red[k][lane][1];
+    }
+    tr *= scale;
+    ti *= scale;
+    if (lower) ti = -ti;  // H[r][c] = conj(H[c][r])
+    if (r == c) ti = 0;
+    if constexpr (CPLX) {
+        H[2 * i] = tr;
+        H[2 * i + 1] = ti;
+    } else {
+        H[i] = tr;
+    }
+}
+
+// Qh[q][f] = conj(V[f][col(q)]) (rp x pp), Qn[f][q] = V[f][col(q)] (pp x rp); V column-major
+// p x p.  col(q) = c_first + q for q < q_split, c_second + (q - q_split) otherwise.
+template <typename T, bool CPLX>
+__global__ void rank_select_kernel(const T *__restrict__ V, int p, int pp, int r, int rp,
+                                   int q_split, int c_first, int c_second, T *__restrict__ Qh,
+                                   T *__restrict__ Qn) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rp * pp) return;
+    const int q = (int)(i / pp), f = (int)(i % pp);
+    T vr = 0, vi = 0;
+    if (q < r && f < p) {
+        const int col = q < q_split ? c_first + q : c_second + (q - q_split);
+        const int64_t e = (int64_t)col * p + f;
+        vr = CPLX ? V[2 * e] : V[e];
+        vi = CPLX ? V[2 * e + 1] : T(0);
+    }
+    const int64_t h = (int64_t)q * pp + f, n = (int64_t)f * rp + q;
+    if constexpr (CPLX) {
+        Qh[2 * h] = vr;
+        Qh[2 * h + 1] = -vi;
+        Qn[2 * n] = vr;
+        Qn[2 * n + 1] = vi;
+    } else {
+        Qh[h] = vr;
+        Qn[n] = vr;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Projection
+// ---------------------------------------------------------------------------------------
+// 256 threads = 4 waves, each wave 16 projection vectors j = j0 + jl.  Coefficient tiles
+// q0 + [0, 16*RT).  base: 0 -> out = sign*P w, 1 -> out = x + sign*P w, 2 -> out += sign*P w.
+template <typename T, bool CPLX, int RT>
+__global__ __launch_bounds__(256) void rank_project_kernel(
+    const T *x, int64_t xsj, int64_t xsf, bool xvec, T *out, int64_t osj,
+    int64_t osf, bool ovec, int64_t L, int p, int pp, int rp, int q0,
+    const T *__restrict__ Qh, const T *__restrict__ Qn, int base, T sign) {
+    using M = Mfma<T>;
+    using V4 = typename M::V4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jl = lane & 15, g = lane >> 4;
+    const int64_t j = ((int64_t)blockIdx.x * 4 + wave) * 16 + jl;
+    const bool jok = j < L;
+    const int64_t xo = jok ? j * xsj : 0, oo = jok ? j * osj : 0;
+
+    V4 cr[RT], ci[RT];
+#pragma unroll
+    for (int t = 0; t < RT; t++) {
+        cr[t] = V4{0, 0, 0, 0};
+        ci[t] = V4{0, 0, 0, 0};
+    }
+    // C[q][j] = sum_f Qh[q][f] w_j[f]
+    for (int kb = 0; kb < pp; kb += 16) {
+        const int f0 = kb + 4 * g;
+        const int nf = !jok ? 0 : (p - f0 >= 4 ? 4 : (p - f0 > 0 ? p - f0 : 0));
+        T wr[4], wi[4];
+        load4<T, CPLX>(x, xo + (int64_t)f0 * xsf, xsf, nf, xvec, wr, wi);
+#pragma unroll
+        for (int t = 0; t < RT; t++) {
+            T hr[4], hi[4];
+            coef4<T, CPLX>(Qh, (int64_t)(q0 + 16 * t + jl) * pp + f0, hr, hi);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                cr[t] = M::mma(hr[s], wr[s], cr[t]);
+                if constexpr (CPLX) {
+                    cr[t] = M::mma(-hi[s], wi[s], cr[t]);
+                    ci[t] = M::mma(hr[s], wi[s], ci[t]);
+                    ci[t] = M::mma(hi[s], wr[s], ci[t]);
+                }
+            }
+        }
+    }
+    // D[f][j] = sum_q Qn[f][q] C[q][j]; lane holds C[16t + row(g, s)][jl] = c[t][s], the B
+    // operand of k-slot g at step s when the A operand uses the same q = 16t + row(g, s)
+    for (int ft = 0; ft < pp; ft += 16) {
+        V4 dr = V4{0, 0, 0, 0}, di = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < RT; t++) {
+            T nr[4], ni[4];
+            if constexpr (M::kRowsInRegs)
+                coef4<T, CPLX>(Qn, (int64_t)(ft + jl) * rp + q0 + 16 * t + 4 * g, nr, ni);
+            else
+                load4<T, CPLX>(Qn, (int64_t)(ft + jl) * rp + q0 + 16 * t + g, 4, 4, false, nr,
+                               ni);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                dr = M::mma(nr[s], cr[t][s], dr);
+                if constexpr (CPLX) {
+                    dr = M::mma(-ni[s], ci[t][s], dr);
+                    di = M::mma(nr[s], ci[t][s], di);
+                    di = M::mma(ni[s], cr[t][s], di);
+                }
+            }
+        }
+        if constexpr (!M::kRowsInRegs) {
+            // lane holds D[ft + g + 4v][j]
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int f = ft + g + 4 * v;
+                if (!jok || f >= p) continue;
+                const int64_t xe = xo + (int64_t)f * xsf, oe = oo + (int64_t)f * osf;
+                T br = 0, bi = 0;
+                if (base == 1) {
+                    br = CPLX ? x[2 * xe] : x[xe];
+                    bi = CPLX ? x[2 * xe + 1] : T(0);
+                } else if (base == 2) {
+                    br = CPLX ? out[2 * oe] : out[oe];
+                    bi = CPLX ? out[2 * oe + 1] : T(0);
+                }
+                if constexpr (CPLX) {
+                    out[2 * oe] = br + sign * dr[v];
+                    out[2 * oe + 1] = bi + sign * di[v];
+                } else {
+                    out[oe] = br + sign * dr[v];
+                }
+            }
+            continue;
+        }
+        // lane holds D[ft + 4g + v][j]
+        const int f0 = ft + 4 * g;
+        const int nf = !jok ? 0 : (p - f0 >= 4 ? 4 : (p - f0 > 0 ? p - f0 : 0));
+        if (nf == 0) continue;
+        T br[4], bi[4];
+        if (base == 1) {
+            load4<T, CPLX>(x, xo + (int64_t)f0 * xsf, xsf, nf, xvec, br, bi);
+        } else if (base == 2) {
+            load4<T, CPLX>(out, oo + (int64_t)f0 * osf, osf, nf, ovec, br, bi);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++) br[v] = bi[v] = T(0);
+        }
+        T yr[4], yi[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            yr[v] = br[v] + sign * dr[v];
+            yi[v] = bi[v] + sign * di[v];
+        }
+        const int64_t e0 = oo + (int64_t)f0 * osf;
+        if (ovec && nf == 4) {
+            if constexpr (sizeof(T) == 4 && !CPLX) {
+                *reinterpret_cast<float4 *>(out + e0) = float4{yr[0], yr[1], yr[2], yr[3]};
+            } else if constexpr (sizeof(T) == 4 && CPLX) {
+                float4 *q = reinterpret_cast<float4 *>(out + 2 * e0);
+                q[0] = float4{yr[0], yi[0], yr[1], yi[1]};
+                q[1] = float4{yr[2], yi[2], yr[3], yi[3]};
+            } else if constexpr (!CPLX) {
+                double2 *q = reinterpret_cast<double2 *>(out + e0);
+                q[0] = double2{yr[0], yr[1]};
+                q[1] = double2{yr[2], yr[3]};
+            } else {
+                double2 *q = reinterpret_cast<double2 *>(out + 2 * e0);
+#pragma unroll
+                for (int v = 0; v < 4; v++) q[v] = double2{yr[v], yi[v]};
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (v < nf) {
+                    const int64_t e = e0 + v * osf;
+                    if constexpr (CPLX) {
+                        out[2 * e] = yr[v];
+                        out[2 * e + 1] = yi[v];
+                    } else {
+                        out[e] = yr[v];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Strided 2-D copy (keep-all case, or the private copy of x for in-place multi-pass calls).
+template <typename T, bool CPLX>
+__global__ void rank_copy_kernel(const T *__restrict__ x, int64_t xs0, int64_t xs1, T *out,
+                                 int64_t os0, int64_t os1, int64_t m, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m * n) return;
+    const int64_t a = i / n, b = i % n;
+    const int64_t s = a * xs0 + b * xs1, d = a * os0 + b * os1;
+    if constexpr (CPLX) {
+        out[2 * d] = x[2 * s];
+        out[2 * d + 1] = x[2 * s + 1];
+    } else {
+        out[d] = x[s];
+    }
+}
+
+template <typename T, bool CPLX, int WT>
+void gram_launch(hipStream_t st, const RankPlan &pl, const void *x, void *part, void *H) {
+    const int nblk = pl.nb * (pl.nb + 1) / 2;
+    ProfScope ps(st, "rank_gram");
+    hipLaunchKernelGGL((rank_gram_kernel<T, CPLX, WT>), dim3(nblk * pl.splits), dim3(64), 0, st,
+                       (const T *)x, pl.gsa, pl.gsk, pl.p, pl.L, pl.kchunk, pl.nb, pl.gvec,
+                       (T *)part);
+    const int64_t pe = (int64_t)pl.p * pl.p;
+    hipLaunchKernelGGL((rank_gram_reduce_kernel<T, CPLX>), dim3((unsigned)((pe + 63) / 64)),
+                       dim3(1024), 0, st, (const T *)part, pl.p, pl.bs, pl.nb, nblk, pl.splits,
+                       (T)(1.0 / (double)pl.n), (T *)H);
+}
+
+template <typename T, bool CPLX>
+void gram_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *part, void *H) {
+    switch (pl.bs) {
+        case 16: gram_launch<T, CPLX, 1>(st, pl, x, part, H); break;
+        case 32: gram_launch<T, CPLX, 2>(st, pl, x, part, H); break;
+        default: gram_launch<T, CPLX, 4>(st, pl, x, part, H); break;
+    }
+}
+
+template <typename T, bool CPLX, int RT>
+void project_launch(hipStream_t st, const RankPlan &pl, const void *x, void *out, int q0,
+                    const void *Qh, const void *Qn, int base, double sign) {
+    const unsigned blocks = (unsigned)((pl.L + 63) / 64);
+    ProfScope ps(st, "rank_project");
+    hipLaunchKernelGGL((rank_project_kernel<T, CPLX, RT>), dim3(blocks), dim3(256), 0, st,
+                       (const T *)x, pl.xsj, pl.xsf, pl.xvec, (T *)out, pl.osj, pl.osf, pl.ovec,
+                       pl.L, pl.p, pl.pp, pl.rp, q0, (const T *)Qh, (const T *)Qn, base, (T)sign);
+}
+
+template <typename T, bool CPLX>
+void project_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *out,
+                      const void *Qh, const void *Qn) {
+    // passes of at most 8 coefficient tiles; the first writes, the others accumulate
+    const double sign = pl.keep ? 1.0 : -1.0;
+    for (int pass = 0; pass < pl.passes; pass++) {
+        const int q0 = pass * 16 * pl.rt;
+        const int base = pass > 0 ? 2 : (pl.keep ? 0 : 1);
+        switch (pl.rt) {
+            case 1: project_launch<T, CPLX, 1>(st, pl, x, out, q0, Qh, Qn, base, sign); break;
+            case 2: project_launch<T, CPLX, 2>(st, pl, x, out, q0, Qh, Qn, base, sign); break;
+            case 4: project_launch<T, CPLX, 4>(st, pl, x, out, q0, Qh, Qn, base, sign); break;
+            default: project_launch<T, CPLX, 8>(st, pl, x, out, q0, Qh, Qn, base, sign); break;
+        }
+    }
+}
+
+}  // namespace
+
+void rank_plan_launch_shape(RankPlan &pl, int cu_count) {
+    // measured on gfx950 (64 and 200 frames, real and complex FP32): single-tile blocks with
+    // many waves for p <= 64, 32 x 32 blocks above; 64 x 64 blocks were slower everywhere
+    pl.bs = pl.p <= 64 ? 16 : 32;
+    if (const char *e = std::getenv("FFDAS_RANK_BS")) {
+        const int v = std::atoi(e);
+        if (v == 16 || v == 32 || v == 64) pl.bs = v;
+    }
+    int waves_per_cu = 16;
+    if (const char *e = std::getenv("FFDAS_RANK_WAVES")) waves_per_cu = std::max(1, std::atoi(e));
+    pl.nb = (pl.p + pl.bs - 1) / pl.bs;
+    const int64_t nblk = (int64_t)pl.nb * (pl.nb + 1) / 2;
+    const int64_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
+    // about eight waves per CU, at least 256 k per wave, partials within 256 MiB
+    int64_t splits = std::max<int64_t>(
+        1, ((int64_t)waves_per_cu * std::max(cu_count, 1) + nblk - 1) / nblk);
+    splits = std::min<int64_t>(splits, std::max<int64_t>(1, pl.L / 256));
+    const int64_t cap = (256LL << 20) / (nblk * pl.bs * pl.bs * esz);
+    splits = std::max<int64_t>(1, std::min<int64_t>(splits, cap));
+    int64_t kchunk = (pl.L + splits - 1) / splits;
+    kchunk = (kchunk + 15) / 16 * 16;
+    pl.kchunk = kchunk;
+    // a multiple of 8 (one k-range per XCD in turn); trailing k-ranges may be empty
+    pl.splits = (int)(((pl.L + kchunk - 1) / kchunk + 7) / 8 * 8);
+    pl.part_bytes = (size_t)(pl.splits * nblk * pl.bs * pl.bs * esz);
+    pl.pp = (pl.p + 15) / 16 * 16;
+    const int tiles = (pl.r + 15) / 16;
+    if (tiles <= 8) {
+        pl.rt = tiles <= 1 ? 1 : (tiles <= 2 ? 2 : (tiles <= 4 ? 4 : 8));
+        pl.passes = 1;
+    } else {
+        pl.rt = 8;
+        pl.passes = (tiles + 7) / 8;
+    }
+    pl.rp = 16 * pl.rt * pl.passes;
+}
+
+ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x, void *part,
+                               void *H) {
+    if (pl.dbl && pl.cplx)
+        gram_dispatch<double, true>(st, pl, x, part, H);
+    else if (pl.dbl)
+        gram_dispatch<double, false>(st, pl, x, part, H);
+    else if (pl.cplx)
+        gram_dispatch<float, true>(st, pl, x, part, H);
+    else
+        gram_dispatch<float, false>(st, pl, x, part, H);
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+ffdas_error_t launch_rank_project(hipStream_t st, const RankPlan &pl, const void *V,
+                                  const void *x, void *out, void *Qh, void *Qn) {
+    const int64_t n = (int64_t)pl.rp * pl.pp;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+#define FFDAS_RANK_SELECT(T, C)                                                              \
+    hipLaunchKernelGGL((rank_select_kernel<T, C>), dim3(blocks), dim3(256), 0, st,          \
+                       (const T *)V, pl.p, pl.pp, pl.r, pl.rp, pl.q_split, pl.c_first,       \
+                       pl.c_second, (T *)Qh, (T *)Qn)
+    if (pl.dbl) {
+        if (pl.cplx) {
+            FFDAS_RANK_SELECT(double, true);
+            project_dispatch<double, true>(st, pl, x, out, Qh, Qn);
+        } else {
+            FFDAS_RANK_SELECT(double, false);
+            project_dispatch<double, false>(st, pl, x, out, Qh, Qn);
+        }
+    } else {
+        if (pl.cplx) {
+            FFDAS_RANK_SELECT(float, true);
+            project_dispatch<float, true>(st, pl, x, out, Qh, Qn);
+        } else {
+            FFDAS_RANK_SELECT(float, false);
+            project_dispatch<float, false>(st, pl, x, out, Qh, Qn);
+        }
+    }
+#undef FFDAS_RANK_SELECT
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+ffdas_error_t launch_rank_copy(hipStream_t st, bool dbl, bool cplx, const void *x, int64_t xs0,
+                               int64_t xs1, void *out, int64_t os0, int64_t os1, int64_t m,
+                               int64_t n) {
+    const int64_t tot = m * n;
+    if (tot == 0) return FFDAS_SUCCESS;
+    const unsigned blocks = (unsigned)((tot + 255) / 256);
+#define FFDAS_RANK_COPY(T, C)                                                                 \
+    hipLaunchKernelGGL((rank_copy_kernel<T, C>), dim3(blocks), dim3(256), 0, st, (const T *)x, \
+                       xs0, xs1, (T *)out, os0, os1, m, n)
+    if (dbl && cplx)
+        FFDAS_RANK_COPY(double, true);
+    else if (dbl)
+        FFDAS_RANK_COPY(double, false);
+    else if (cplx)
+        FFDAS_RANK_COPY(float, true);
+    else
+        FFDAS_RANK_COPY(float, false);
+#undef FFDAS_RANK_COPY
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+}  // namespace ffdas

@@ -153,3 +153,33 @@ def mex_das_sparse(x, srcpos, dstpos, offsets, weights, sparse_indices_1based, s
     _check_tables(name, count, dstnd, dstpos, t, count)
     return _call(name, x, srcpos, dstpos, offsets, weights, srcdir, wavenum, algorithm, use_fp16,
                  sparse=sparse, count=count)
+
+
+def mex_truncate_rank(x: torch.Tensor, start_1based: int, stop: int | None = None):
+    """truncate_rank.m + ffdas_truncate_rank.cpp:13-70: MATLAB's last dimension (the leading
+    dimension of the row-major view) is kept, the others are flattened; start is 1-based and
+    stop inclusive (so the same number as the 0-based exclusive stop); default stop
+    min(m, n)."""
+    name = "ffdas_truncate_rank"
+    if x.dim() < 2:
+        raise MexError(f"{name}:error: input must have at least two dimensions (got {x.dim()})")
+    n = int(x.shape[0])
+    m = 1
+    for d in x.shape[1:]:
+        m *= int(d)
+    if stop is None:                       # truncate_rank.m: stop = min(m, n)
+        stop = min(m, n)
+    start = int(start_1based) - 1
+    if start < 0 or start >= min(m, n):
+        raise MexError(f"{name}:error: start must be > 0 and < min(m, n) (got {start})")
+    if stop <= start or stop > min(m, n):
+        raise MexError(f"{name}:error: stop must be > start and <= min(m, n) (got {stop})")
+    xv = x.reshape(n, m)
+    out = torch.empty_like(xv)
+    x_desc = TensorDesc((n, m), tuple(xv.stride()), element_code(xv))
+    o_desc = TensorDesc((n, m), tuple(out.stride()), element_code(out))
+    with torch.cuda.device(x.device):
+        h = handle_for(x)
+        check(lib().ffdas_truncate_rank(h.h, x_desc.ptr, xv.data_ptr(), start, int(stop),
+                                        o_desc.ptr, out.data_ptr()), name)
+    return out.reshape(x.shape)

@@ -1,0 +1,183 @@
+"""truncate_rank on the GPU against the FP64 oracle (oracle/rank.py): the reference's own
+cases (bindings/python/tests/test_truncate_rank.py:37-94), every dtype, both orientations
+(m <= n: Gram of the rows; m > n: Gram of the columns), strided and in-place calls, more
+than 128 eigenvector columns (several projection passes), and a 64 x 64^3 complex volume
+ensemble built from known singular factors, so its exact truncation is known without an SVD.
+
+Tolerances: rel-L2 against the FP64 result, 1e-4 for FP32/complex64 (the Gram matrix
+squares the condition number; eigenvector errors scale with eps * s_max^2 / gap, ~1e-6 for
+these Gaussian matrices) and 1e-9 for FP64/complex128."""
+import numpy as np
+import pytest
+
+import oracle.rank as R
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"float32": 1e-4, "complex64": 1e-4, "float64": 1e-9, "complex128": 1e-9}
+
+
+def _rand(shape, dtype, seed):
+    import torch
+    g = torch.Generator().manual_seed(seed)
+    if dtype.is_complex:
+        real = torch.float32 if dtype == torch.complex64 else torch.float64
+        return torch.complex(torch.randn(shape, generator=g, dtype=real),
+                             torch.randn(shape, generator=g, dtype=real)).to(dtype)
+    return torch.randn(shape, generator=g, dtype=dtype)
+
+
+def _check(x, start, stop, tol, **kw):
+    import ffdas_amd
+    got = ffdas_amd.truncate_rank(x, start, stop, **kw)
+    want = R.svd_filter(x.cpu().numpy(), start, stop)
+    err = rel_l2(got.cpu().numpy(), want)
+    assert err < tol, err
+    return got
+
+
+def test_reference_keep_all(dev):
+    import torch
+    x = _rand((8, 32), torch.float32, 1).to(dev)
+    import ffdas_amd
+    got = ffdas_amd.truncate_rank(x, 0, 8)
+    assert torch.equal(got, x)
+
+
+@pytest.mark.parametrize("start, stop", [(0, 4), (1, 8), (2, 6)])
+def test_reference_truncated(dev, start, stop):
+    import torch
+    _check(_rand((16, 64), torch.float32, 2).to(dev), start, stop, 1e-4)
+
+
+def test_reference_output_shape_3d(dev):
+    import torch
+    import ffdas_amd
+    x = _rand((8, 4, 16), torch.float32, 3).to(dev)
+    assert ffdas_amd.truncate_rank(x, 0, 4).shape == x.shape
+    _check(x, 0, 4, 1e-4)
+
+
+@pytest.mark.parametrize("shape", [(16, 64), (64, 16), (32, 32)])
+def test_reference_shape_variants(dev, shape):
+    import torch
+    k = min(shape)
+    _check(_rand(shape, torch.float32, 4).to(dev), 1, k - 1, 1e-4)
+
+
+def test_reference_more_rows_than_cols(dev):
+    import torch
+    _check(_rand((64, 8), torch.float32, 5).to(dev), 0, 8, 1e-6)
+
+
+def test_reference_preallocated(dev):
+    import torch
+    import ffdas_amd
+    x = _rand((4, 16), torch.float32, 6).to(dev)
+    out = torch.zeros_like(x)
+    assert ffdas_amd.truncate_rank(x, 0, out=out) is out
+
+
+@pytest.mark.parametrize("dtype", ["float32", "complex64", "float64", "complex128"])
+@pytest.mark.parametrize("shape, start, stop", [
+    ((24, 300), 1, None),       # drop the largest component (subtract mode)
+    ((24, 300), 3, 7),          # a band (keep mode)
+    ((300, 20), 2, 17),         # m > n: Gram of the columns
+    ((5, 7, 9), 0, 2),          # 3-D, p = 5
+    ((70, 1000), 10, 60),       # p = 70: several Gram blocks
+])
+def test_dtypes_and_orientations(dev, dtype, shape, start, stop):
+    import torch
+    x = _rand(shape, getattr(torch, dtype), 7).to(dev)
+    _check(x, start, stop, TOL[dtype])
+
+
+def test_more_than_128_columns_and_in_place(dev):
+    """r = min(kept, dropped) = 150 > 128: two projection passes; in place, the second pass
+    reads a private copy of x."""
+    import torch
+    import ffdas_amd
+    x = _rand((300, 2000), torch.float32, 8).to(dev)
+    want = R.svd_filter(x.cpu().numpy(), 100, 250)
+    got = ffdas_amd.truncate_rank(x, 100, 250)
+    assert rel_l2(got.cpu().numpy(), want) < 1e-4
+    y = x.clone()
+    ffdas_amd.truncate_rank(y, 100, 250, out=y)
+    assert torch.equal(y, got)
+
+
+def test_in_place_single_pass(dev):
+    import torch
+    import ffdas_amd
+    x = _rand((32, 500), torch.complex64, 9).to(dev)
+    got = ffdas_amd.truncate_rank(x, 2)
+    ffdas_amd.truncate_rank(x, 2, out=x)
+    assert torch.equal(x, got)
+
+
+def test_strided_input(dev):
+    """A transposed view: element strides (1, m) -- no 16-byte loads on either side."""
+    import torch
+    base = _rand((400, 24), torch.complex64, 10).to(dev)
+    x = base.t()                                   # (24, 400), strides (1, 24)
+    _check(x, 1, 20, 1e-4)
+    base = _rand((30, 410), torch.float32, 11).to(dev)
+    _check(base.t(), 3, None, 1e-4)                # (410, 30): m > n, strided
+
+
+def test_idempotent_and_deterministic(dev):
+    import torch
+    import ffdas_amd
+    x = _rand((40, 3000), torch.float32, 12).to(dev)
+    a = ffdas_amd.truncate_rank(x, 4, 30)
+    b = ffdas_amd.truncate_rank(x, 4, 30)
+    assert torch.equal(a, b)
+    c = ffdas_amd.truncate_rank(a, 0, 26)          # a has rank 26: keeping it all is a
+    assert float((c - a).norm() / a.norm()) < 1e-4
+
+
+def test_complement_adds_up(dev):
+    import torch
+    import ffdas_amd
+    x = _rand((50, 800), torch.complex128, 13).to(dev)
+    a = ffdas_amd.truncate_rank(x, 0, 7)
+    b = ffdas_amd.truncate_rank(x, 7)
+    assert float((a + b - x).norm() / x.norm()) < 1e-10
+
+
+def test_clutter_volume_known_factors(dev):
+    """64 frames of a 64^3 complex volume (the clutter-filter example's size), x = U S V^H
+    with known factors and a tissue-like spectrum (five strong components); start = 5 removes
+    them.  The expected result is U[:, 5:] S[5:] V[:, 5:]^H, no SVD needed."""
+    import torch
+    import ffdas_amd
+    m, n = 64, 64 ** 3
+    g = torch.Generator().manual_seed(14)
+    u, _ = torch.linalg.qr(torch.randn(m, m, dtype=torch.complex128, generator=g))
+    v, _ = torch.linalg.qr(torch.randn(n, m, dtype=torch.complex128, generator=g))
+    s = torch.cat([torch.tensor([4000.0, 2500.0, 1600.0, 900.0, 500.0]),
+                   torch.linspace(40.0, 1.0, m - 5)]).to(torch.complex128)
+    x = ((u * s) @ v.conj().T).to(dev)
+    want = ((u[:, 5:] * s[5:]) @ v[:, 5:].conj().T).to(dev)
+    got = ffdas_amd.truncate_rank(x.to(torch.complex64).reshape(m, 64, 64, 64), 5)
+    err = float((got.reshape(m, n).to(torch.complex128) - want).norm() / want.norm())
+    assert err < 1e-3, err           # FP32 Gram: 4000^2 / 40^2 condition of the kept part
+    got64 = ffdas_amd.truncate_rank(x, 5)
+    assert float((got64 - want).norm() / want.norm()) < 1e-9
+
+
+def test_matlab_gateway(dev):
+    """truncate_rank.m / ffdas_truncate_rank.cpp: 1-based start, inclusive stop, last MATLAB
+    dimension kept; the same numbers as the Python call on the row-major view."""
+    import torch
+    import ffdas_amd
+    from tests.mex_gateway import MexError, mex_truncate_rank
+    x = _rand((20, 6, 7), torch.complex64, 15).to(dev)     # MATLAB (7, 6, 20)
+    got = mex_truncate_rank(x, 2, 9)
+    assert torch.equal(got, ffdas_amd.truncate_rank(x, 1, 9))
+    assert torch.equal(mex_truncate_rank(x, 3), ffdas_amd.truncate_rank(x, 2))
+    with pytest.raises(MexError):
+        mex_truncate_rank(x, 0)
+    with pytest.raises(MexError):
+        mex_truncate_rank(x, 5, 4)
