@@ -212,6 +212,9 @@ struct RankPlan {
 void rank_plan_launch_shape(RankPlan &pl, int cu_count);
 ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x, void *part,
                                void *H);
+bool rank_jacobi_fits(const RankPlan &pl, int lds_bytes);
+ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, int *info,
+                                 int max_sweeps);
 ffdas_error_t launch_rank_project(hipStream_t st, const RankPlan &pl, const void *V,
                                   const void *x, void *out, void *Qh, void *Qn);
 ffdas_error_t launch_rank_copy(hipStream_t st, bool dbl, bool cplx, const void *x, int64_t xs0,

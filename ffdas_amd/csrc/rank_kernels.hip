@@ -311,6 +311,234 @@ __global__ void rank_select_kernel(const T *__restrict__ V, int p, int pp, int r
 }
 
 // ---------------------------------------------------------------------------------------
+// Eigendecomposition of a small Hermitian matrix: parallel cyclic Jacobi in one workgroup
+// ---------------------------------------------------------------------------------------
+// A and V (pe x pe, pe = p rounded up to even, row-major, complex as pairs) live in LDS.
+// Each round pairs the indices by the round-robin tournament (pe - 1 rounds visit every pair
+// once); the pe/2 rotations of a round are disjoint, so every 2x2 block (P, Q) of A is
+// updated independently as G_P^H B G_Q, and V's rows as V G.  G = diag(1, e^{-i phi}) J with
+// a_ij = r e^{i phi} (real data: G = J with r = a_ij, signed) and J = [[c, s], [-s, c]] the
+// symmetric Schur rotation of [[a_ii, r], [r, a_jj]] (tau = (a_jj - a_ii) / 2r,
+// t = sgn(tau) / (|tau| + sqrt(1 + tau^2))).
+// A rotation is skipped when |a_ij| <= eps * ||A||_F; a sweep without rotations ends the
+// iteration.  Output: eigenvectors in the columns of H (column-major), ascending eigenvalues,
+// as the LAPACK-style solver returns them; info = 1 if max_sweeps did not converge.
+template <typename T>
+struct Eps;
+template <>
+struct Eps<float> {
+    static constexpr float v = 1.1920929e-07f;
+};
+template <>
+struct Eps<double> {
+    static constexpr double v = 2.220446049250313e-16;
+};
+
+size_t jacobi_lds_bytes(int pe, size_t esz, size_t tsz) {
+    const int npair = pe / 2;
+    return 2 * (size_t)pe * pe * esz + (size_t)npair * (4 * tsz + 2 * sizeof(int)) + 64;
+}
+
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, int max_sweeps,
+                                                           int *info) {
+    extern __shared__ __align__(16) unsigned char jlds[];
+    constexpr int C = CPLX ? 2 : 1;
+    const int npair = pe / 2;
+    T *A = reinterpret_cast<T *>(jlds);
+    T *V = A + (size_t)pe * pe * C;
+    T *pc = V + (size_t)pe * pe * C;      // per pair: c, s, cos(phi), sin(phi)
+    T *ps = pc + npair;
+    T *pr = ps + npair;
+    T *pm = pr + npair;
+    int *pi = reinterpret_cast<int *>(pm + npair);
+    int *pj = pi + npair;
+    __shared__ T red[16];
+    __shared__ int flag[3];
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    // load A (H is column-major: (r, c) at c * p + r), V = I, ||A||_F
+    T sq = 0;
+    for (int e = tid; e < pe * pe; e += nt) {
+        const int r = e / pe, c = e % pe;
+        T ar = 0, ai = 0;
+        if (r < p && c < p) {
+            const int64_t h = (int64_t)c * p + r;
+            ar = CPLX ? H[2 * h] : H[h];
+            ai = CPLX ? H[2 * h + 1] : T(0);
+        }
+        sq += ar * ar + ai * ai;
+        A[C * e] = ar;
+        if constexpr (CPLX) A[C * e + 1] = ai;
+        V[C * e] = r == c ? T(1) : T(0);
+        if constexpr (CPLX) V[C * e + 1] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((tid & 63) == 0) red[tid >> 6] = sq;
+    if (tid < 3) flag[tid] = 0;
+    __syncthreads();
+    T norm2 = 0;
+    for (int w = 0; w < (nt + 63) / 64; w++) norm2 += red[w];
+    const T thresh = Eps<T>::v * sqrt(norm2);
+
+    int sweep = 0;
+    bool done = false;
+    for (; sweep < max_sweeps && !done; sweep++) {
+        if (tid == 0) flag[(sweep + 1) % 3] = 0;
+        for (int round = 0; round < pe - 1; round++) {
+            // rotation parameters of this round's pairs
+            if (tid < npair) {
+                const int m0 = tid, m1 = pe - 1 - tid;
+                int i = m0 == 0 ? 0 : 1 + (m0 - 1 + round) % (pe - 1);
+                int j = 1 + (m1 - 1 + round) % (pe - 1);
+                if (i > j) {
+                    const int t = i;
+                    i = j;
+                    j = t;
+                }
+                T c = 1, s = 0, er = 1, ei = 0;
+                if (j < p) {
+                    const T aii = A[C * (i * pe + i)], ajj = A[C * (j * pe + j)];
+                    const T zr = A[C * (i * pe + j)], zi = CPLX ? A[C * (i * pe + j) + 1] : T(0);
+                    // complex: rotate by |a_ij| after the phase factor; real: the signed a_ij
+                    const T r = CPLX ? sqrt(zr * zr + zi * zi) : zr;
+                    if (fabs(r) > thresh) {
+                        const T tau = (ajj - aii) / (2 * r);
+                        const T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + sqrt(1 + tau * tau));
+                        c = 1 / sqrt(1 + t * t);
+                        s = t * c;
+                        if constexpr (CPLX) {
+                            er = zr / r;
+                            ei = zi / r;
+                        }
+                        flag[sweep % 3] = 1;
+                    }
+                }
+                pi[tid] = i;
+                pj[tid] = j;
+                pc[tid] = c;
+                ps[tid] = s;
+                pr[tid] = er;
+                pm[tid] = ei;
+            }
+            __syncthreads();
+            // blocks (P, Q), P <= Q: B' = J_P^T D_P^H B D_Q J_Q, mirrored into (Q, P)
+            for (int w = tid; w < npair * npair; w += nt) {
+                const int P = w / npair, Q = w % npair;
+                if (P > Q) continue;
+                const T cP = pc[P], sP = ps[P], cQ = pc[Q], sQ = ps[Q];
+                if (sP == 0 && sQ == 0) continue;
+                const int iP = pi[P], jP = pj[P], iQ = pi[Q], jQ = pj[Q];
+                const int e00 = iP * pe + iQ, e01 = iP * pe + jQ, e10 = jP * pe + iQ,
+                          e11 = jP * pe + jQ;
+                T b00r = A[C * e00], b01r = A[C * e01], b10r = A[C * e10], b11r = A[C * e11];
+                T b00i = 0, b01i = 0, b10i = 0, b11i = 0;
+                if constexpr (CPLX) {
+                    b00i = A[C * e00 + 1];
+                    b01i = A[C * e01 + 1];
+                    b10i = A[C * e10 + 1];
+                    b11i = A[C * e11 + 1];
+                    // M = D_P^H B D_Q: column 1 times e^{-i phiQ}, row 1 times e^{i phiP}
+                    const T qr = pr[Q], qi = -pm[Q], pR = pr[P], pI = pm[P];
+                    T tr = b01r * qr - b01i * qi, ti = b01r * qi + b01i * qr;
+                    b01r = tr;
+                    b01i = ti;
+                    tr = b11r * qr - b11i * qi;
+                    ti = b11r * qi + b11i * qr;
+                    b11r = tr;
+                    b11i = ti;
+                    tr = pR * b10r - pI * b10i;
+                    ti = pR * b10i + pI * b10r;
+                    b10r = tr;
+                    b10i = ti;
+                    tr = pR * b11r - pI * b11i;
+                    ti = pR * b11i + pI * b11r;
+                    b11r = tr;
+                    b11i = ti;
+                }
+                // N = M J_Q
+                const T n00r = cQ * b00r - sQ * b01r, n01r = sQ * b00r + cQ * b01r;
+                const T n10r = cQ * b10r - sQ * b11r, n11r = sQ * b10r + cQ * b11r;
+                const T n00i = cQ * b00i - sQ * b01i, n01i = sQ * b00i + cQ * b01i;
+                const T n10i = cQ * b10i - sQ * b11i, n11i = sQ * b10i + cQ * b11i;
+                // B' = J_P^T N
+                T o00r = cP * n00r - sP * n10r, o01r = cP * n01r - sP * n11r;
+                T o10r = sP * n00r + cP * n10r, o11r = sP * n01r + cP * n11r;
+                T o00i = cP * n00i - sP * n10i, o01i = cP * n01i - sP * n11i;
+                T o10i = sP * n00i + cP * n10i, o11i = sP * n01i + cP * n11i;
+                if (P == Q) {   // the rotated pair itself: real diagonal, exact zero off it
+                    o01r = o01i = o10r = o10i = 0;
+                    o00i = o11i = 0;
+                }
+                A[C * e00] = o00r;
+                A[C * e01] = o01r;
+                A[C * e10] = o10r;
+                A[C * e11] = o11r;
+                if constexpr (CPLX) {
+                    A[C * e00 + 1] = o00i;
+                    A[C * e01 + 1] = o01i;
+                    A[C * e10 + 1] = o10i;
+                    A[C * e11 + 1] = o11i;
+                }
+                if (P != Q) {   // Hermitian mirror: (Q, P) block = conj transpose
+                    const int f00 = iQ * pe + iP, f01 = iQ * pe + jP, f10 = jQ * pe + iP,
+                              f11 = jQ * pe + jP;
+                    A[C * f00] = o00r;
+                    A[C * f10] = o01r;
+                    A[C * f01] = o10r;
+                    A[C * f11] = o11r;
+                    if constexpr (CPLX) {
+                        A[C * f00 + 1] = -o00i;
+                        A[C * f10 + 1] = -o01i;
+                        A[C * f01 + 1] = -o10i;
+                        A[C * f11 + 1] = -o11i;
+                    }
+                }
+            }
+            // V' = V G for every pair and row
+            for (int w = tid; w < pe * npair; w += nt) {
+                const int row = w / npair, P = w % npair;
+                const T cP = pc[P], sP = ps[P];
+                if (sP == 0) continue;
+                const int ei0 = row * pe + pi[P], ej0 = row * pe + pj[P];
+                const T vir = V[C * ei0], vjr0 = V[C * ej0];
+                T vjr = vjr0, vji = 0, vii = 0;
+                if constexpr (CPLX) {
+                    vii = V[C * ei0 + 1];
+                    const T vji0 = V[C * ej0 + 1];
+                    const T qr = pr[P], qi = -pm[P];   // times e^{-i phi}
+                    vjr = vjr0 * qr - vji0 * qi;
+                    vji = vjr0 * qi + vji0 * qr;
+                }
+                V[C * ei0] = cP * vir - sP * vjr;
+                V[C * ej0] = sP * vir + cP * vjr;
+                if constexpr (CPLX) {
+                    V[C * ei0 + 1] = cP * vii - sP * vji;
+                    V[C * ej0 + 1] = sP * vii + cP * vji;
+                }
+            }
+            __syncthreads();
+        }
+        done = flag[sweep % 3] == 0;
+    }
+
+    // ascending order of the eigenvalues; ties by index
+    for (int w = tid; w < p * p; w += nt) {
+        const int row = w % p, k = w / p;
+        const T dk = A[C * (k * pe + k)];
+        int rank = 0;
+        for (int l = 0; l < p; l++) {
+            const T dl = A[C * (l * pe + l)];
+            rank += (dl < dk || (dl == dk && l < k)) ? 1 : 0;
+        }
+        const int64_t h = (int64_t)rank * p + row;
+        H[C * h] = V[C * (row * pe + k)];
+        if constexpr (CPLX) H[C * h + 1] = V[C * (row * pe + k) + 1];
+    }
+    if (tid == 0) info[0] = done ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------------------
 // Projection
 // ---------------------------------------------------------------------------------------
 // 256 threads = 4 waves, each wave 16 projection vectors j = j0 + jl.  Coefficient tiles
@@ -568,6 +796,38 @@ ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x
         gram_dispatch<float, true>(st, pl, x, part, H);
     else
         gram_dispatch<float, false>(st, pl, x, part, H);
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+bool rank_jacobi_fits(const RankPlan &pl, int lds_bytes) {
+    const int pe = (pl.p + 1) / 2 * 2;
+    const size_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
+    // the kernel's static LDS (reduction slots, flags) stays well under 512 bytes
+    return pl.p >= 2 && jacobi_lds_bytes(pe, esz, pl.dbl ? 8 : 4) + 512 <= (size_t)lds_bytes;
+}
+
+ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, int *info,
+                                 int max_sweeps) {
+    const int pe = (pl.p + 1) / 2 * 2;
+    const size_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
+    const size_t lds = jacobi_lds_bytes(pe, esz, pl.dbl ? 8 : 4);
+    ProfScope ps(st, "rank_jacobi");
+#define FFDAS_RANK_JACOBI(T, C)                                                               \
+    do {                                                                                     \
+        (void)hipFuncSetAttribute((const void *)rank_jacobi_kernel<T, C>,                    \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
+        hipLaunchKernelGGL((rank_jacobi_kernel<T, C>), dim3(1), dim3(1024), lds, st, (T *)H, \
+                           pl.p, pe, max_sweeps, info);                                      \
+    } while (0)
+    if (pl.dbl && pl.cplx)
+        FFDAS_RANK_JACOBI(double, true);
+    else if (pl.dbl)
+        FFDAS_RANK_JACOBI(double, false);
+    else if (pl.cplx)
+        FFDAS_RANK_JACOBI(float, true);
+    else
+        FFDAS_RANK_JACOBI(float, false);
+#undef FFDAS_RANK_JACOBI
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
 }
 

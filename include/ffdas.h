@@ -308,8 +308,10 @@ FFDAS_API int ffdas_tune_count(void);
  * dtype (dense, row- or column-major, may be x itself).  With p = min(m, n), out keeps the
  * singular components [start, stop) of x counted from the largest, 0 <= start < stop <= p
  * (else FFDAS_ERROR_INVALID_ARGUMENT).  The eigendecomposition of the p x p Gram matrix runs
- * on rocSOLVER (FFDAS_ERROR_CUSOLVER if it is unavailable or fails); the call synchronises
- * the handle's stream once, after the eigendecomposition.
+ * in one workgroup (Jacobi, FFDAS_ERROR_CONVERGENCE_FAILED if 30 sweeps do not converge)
+ * for real data that fits in LDS, else on rocSOLVER (FFDAS_ERROR_CUSOLVER if it is
+ * unavailable or fails); the call synchronises the handle's stream once, after the
+ * eigendecomposition.
  * Replaces include/ffdas.h:178-186. */
 FFDAS_API ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
                                             const void *x, int64_t start, int64_t stop,
@@ -319,7 +321,7 @@ FFDAS_API ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_
  * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
  * "das_bwd_x", "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
- * "transpose_gout", "rank_gram", "rank_project").
+ * "transpose_gout", "rank_gram", "rank_jacobi", "rank_project").
  * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);

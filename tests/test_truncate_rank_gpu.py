@@ -181,3 +181,36 @@ def test_matlab_gateway(dev):
         mex_truncate_rank(x, 0)
     with pytest.raises(MexError):
         mex_truncate_rank(x, 5, 4)
+
+
+@pytest.mark.parametrize("solver", ["jacobi", "rocsolver"])
+@pytest.mark.parametrize("dtype", ["float32", "complex64", "float64", "complex128"])
+@pytest.mark.parametrize("shape, start, stop", [((33, 500), 2, 20), ((64, 2000), 5, None),
+                                                 ((2, 50), 1, None)])
+def test_both_eigensolvers(dev, monkeypatch, solver, dtype, shape, start, stop):
+    """The one-workgroup Jacobi kernel (odd p padded with an inert index) and rocSOLVER give
+    the oracle's result."""
+    import torch
+    monkeypatch.setenv("FFDAS_RANK_EIG", solver)
+    x = _rand(shape, getattr(torch, dtype), 16).to(dev)
+    _check(x, start, stop, TOL[dtype])
+
+
+def test_jacobi_is_the_default_for_small_real_p(dev, monkeypatch):
+    import ctypes
+    import torch
+    import ffdas_amd
+    from ffdas_amd._lib import lib
+    monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
+    L = lib()
+    x = _rand((64, 4096), torch.float32, 17).to(dev)
+    L.ffdas_profile_reset()
+    L.ffdas_profile_enable(1)
+    try:
+        ffdas_amd.truncate_rank(x, 3)
+        torch.cuda.synchronize()
+    finally:
+        L.ffdas_profile_enable(0)
+    t, c = ctypes.c_double(), ctypes.c_int64()
+    L.ffdas_profile_query(b"rank_jacobi", ctypes.byref(t), ctypes.byref(c))
+    assert c.value == 1
