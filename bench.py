@@ -12,6 +12,7 @@ counted once per forward+adjoint step.  Prints ONE JSON line.
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--storage fp16|bf16|fp32]
   python bench.py --profile-only N   # N steps + a 256 MiB copy, for rocprofv3 --pmc runs
   python bench.py --config 5         # batched sparse compounding, three probes, one graph
+  python bench.py --op truncate_rank # clutter filter (rank truncation) of a frame ensemble
 
 roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
 profiles/das_config4_<storage>_traffic.json, written by ffdas_amd.utils.pmc from two
@@ -50,6 +51,9 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--config", type=int, default=4, choices=[4, 5],
                     help="4: fwd+adjoint headline (default); 5: batched sparse compounding")
+    ap.add_argument("--op", default="das", choices=["das", "truncate_rank"],
+                    help="das (default): the delay-and-sum benchmarks; truncate_rank: the "
+                         "clutter filter on a beamformed frame ensemble")
     ap.add_argument("--c5-frames", type=int, default=200, help="config 5 ensemble size")
     ap.add_argument("--c5-probes", default="linear64,linear256,matrix1024")
     ap.add_argument("--no-autotune", action="store_true",
@@ -293,12 +297,107 @@ def bench_config5(a):
     print(json.dumps(line), flush=True)
 
 
+def bench_truncate_rank(a):
+    """The clutter filter after beamforming: truncate_rank of a frame ensemble, removing the
+    strongest singular components (tissue).  Workload: the reference's clutter-filter example
+    size, 64 frames of a 64^3 complex64 volume, start = 5; also timed, and reported in
+    `config.also`: 200 frames of a 96^3 FP32 volume (the volumetric sparse-compounding
+    ensemble), start = 3.  Data: standard normal on the device.  Each call synchronises once
+    (after the eigendecomposition), so calls are timed by wall clock after a device sync;
+    kernel durations come from the library's hipEvent profiling on the launch stream."""
+    import ctypes
+    import torch
+    import ffdas_amd
+    from ffdas_amd._lib import lib
+    dev = torch.device("cuda:0")
+    L = lib()
+
+    def kernel_ms(name):
+        t, c = ctypes.c_double(), ctypes.c_int64()
+        L.ffdas_profile_query(name.encode(), ctypes.byref(t), ctypes.byref(c))
+        return t.value / max(c.value, 1) if c.value else 0.0
+
+    def run(shape, dtype, start, steps, warmup):
+        g = torch.Generator(device=dev).manual_seed(5)
+        x = torch.randn(shape, dtype=dtype, device=dev, generator=g)
+        out = torch.empty_like(x)
+        for _ in range(warmup):
+            ffdas_amd.truncate_rank(x, start, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ffdas_amd.truncate_rank(x, start, out=out)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        L.ffdas_profile_reset()
+        L.ffdas_profile_enable(1)
+        for _ in range(max(3, steps // 4)):
+            ffdas_amd.truncate_rank(x, start, out=out)
+        torch.cuda.synchronize()
+        L.ffdas_profile_enable(0)
+        kt = {k: kernel_ms(k) for k in ("rank_gram", "rank_jacobi", "rank_project")}
+        return x, ms, kt
+
+    steps = max(5, a.steps)
+    m, n = 64, 64 ** 3
+    x, ms, kt = run((m, 64, 64, 64), torch.complex64, 5, steps, a.warmup)
+    value = m * n / (ms * 1e-3) / 1e9
+    log(f"truncate_rank (64, 64^3) complex64, start 5: {ms:.3f} ms/call, kernels {kt}")
+    xb = x.numel() * x.element_size()
+    p = m
+    # Gram: the p(p+1)/2 distinct entries, 8 real flops per complex multiply-add
+    gram_flops = p * (p + 1) // 2 * n * 8
+    gram = {"kernel": "rank_gram", "bound": "mfma", "ms": round(kt["rank_gram"], 4),
+            "achieved": round(gram_flops / (kt["rank_gram"] * 1e-3) / 1e12, 2),
+            "peak": 157.3, "unit": "TFLOP/s"}
+    gram["frac"] = round(gram["achieved"] / gram["peak"], 4)
+    proj = {"kernel": "rank_project", "bound": "hbm", "ms": round(kt["rank_project"], 4),
+            "achieved": round(2 * xb / (kt["rank_project"] * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": 2 * xb}
+    proj["frac"] = round(proj["achieved"] / proj["peak"], 4)
+
+    m2, n2 = 200, 96 ** 3
+    x2, ms2, kt2 = run((m2, 96, 96, 96), torch.float32, 3, max(3, steps // 2), 1)
+    log(f"truncate_rank (200, 96^3) float32, start 3: {ms2:.3f} ms/call, kernels {kt2}")
+    del x2
+
+    cpu = None
+    if not a.no_cpu:
+        import oracle.rank as R
+        # one numpy (LAPACK) SVD filter of a quarter of the volume (64 x 65536 complex64)
+        xs = x.reshape(m, n)[:, : n // 4].cpu().numpy()
+        _, secs = R.svd_filter_timed(xs, 5)
+        cpu = {"value": round(m * (n // 4) / secs / 1e9, 4), "unit": "Gsample/s",
+               "cores": os.cpu_count(), "kind": "port",
+               "sample": "numpy SVD filter (LAPACK, all host threads) of 64 frames x 65536 "
+                         "voxels complex64, start 5"}
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "Gsample/s truncate_rank (frame x voxel elements filtered per second)",
+        "value": round(value, 3), "unit": "Gsample/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": "truncate_rank clutter filter: 64 frames x 64^3 voxels "
+                               "complex64, start 5",
+                   "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+                   "also": {"workload": "200 frames x 96^3 voxels float32, start 3",
+                            "ms_per_call": round(ms2, 4),
+                            "value": round(m2 * n2 / (ms2 * 1e-3) / 1e9, 3),
+                            "kernel_ms": {k: round(v, 4) for k, v in kt2.items()}}},
+        "roofline": proj, "roofline_gram": gram, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "described in DESIGN.md but not built", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.op == "truncate_rank":
+        bench_truncate_rank(a)
+        return
     if a.config == 5:
         bench_config5(a)
         return
