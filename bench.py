@@ -335,7 +335,8 @@ def bench_truncate_rank(a):
             ffdas_amd.truncate_rank(x, start, out=out)
         torch.cuda.synchronize()
         L.ffdas_profile_enable(0)
-        kt = {k: kernel_ms(k) for k in ("rank_gram", "rank_jacobi", "rank_project")}
+        kt = {k: kernel_ms(k) for k in ("rank_gram", "rank_gram_reduce", "rank_jacobi",
+                                        "rank_project")}
         return x, ms, kt
 
     steps = max(5, a.steps)

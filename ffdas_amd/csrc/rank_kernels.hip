@@ -17,8 +17,8 @@
 // products):
 //   rank_gram_kernel      split-K Gram: one wave per (upper-triangle block, k-range),
 //                         block partials written in full, no atomics;
-//   rank_gram_reduce      sums the partials in a fixed order, scales by 1/n, mirrors the
-//                         lower triangle as the conjugate of the upper, column-major out;
+//   rank_gram_reduce      sums the partials in a fixed order, scales by 1/n, writes each
+//                         upper entry and its conjugate mirror, column-major out;
 //   rank_select_kernel    gathers the chosen eigenvector columns into Qh (r x p, conjugated)
 //                         and Qn (p x r), zero padded to 16-multiples;
 //   rank_project_kernel   per wave 16 projection vectors: C = Qh w (MFMA, k over p), then
@@ -226,8 +226,10 @@ __global__ __launch_bounds__(64) void rank_gram_kernel(const T *__restrict__ x, 
             }
 }
 
-// H (column-major p x p) from the block partials: 1024 threads = 16 waves per 64 entries,
-// wave w sums the splits [w*S/16, (w+1)*S/16) in order, then the 16 sums are added in order.
+// H (column-major p x p) from the block partials.  Lanes run over the entries of the stored
+// blocks in storage order (coalesced reads); 1024 threads = 16 waves per 64 entries, wave w
+// sums the splits [w*S/16, (w+1)*S/16) in order, then the 16 sums are added in order.  Each
+// upper entry (a <= b) is written twice, as H[a][b] and as its conjugate H[b][a].
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(1024) void rank_gram_reduce_kernel(const T *__restrict__ part,
                                                                 int p, int BS, int nb, int nblk,
@@ -235,28 +237,32 @@ __global__ __launch_bounds__(1024) void rank_gram_reduce_kernel(const T *__restr
                                                                 T *__restrict__ H) {
     __shared__ T red[16][64][2];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
-    const bool ok = i < (int64_t)p * p;
-    T sr = 0, si = 0;
-    int r = 0, c = 0;
-    bool lower = false;
+    const int64_t blkel = (int64_t)BS * BS;
+    const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+    bool ok = e < nblk * blkel;
+    int a = 0, b = 0;
     if (ok) {
-        r = (int)(i % p);
-        c = (int)(i / p);
-        lower = r > c;
-        const int a = lower ? c : r, b = lower ? r : c;
-        const int bi = a / BS, bj = b / BS;
-        const int blk = bi * nb - bi * (bi - 1) / 2 + (bj - bi);
-        const int64_t blkel = (int64_t)BS * BS;
-        const int64_t off = blk * blkel + (a % BS) * BS + (b % BS);
+        int idx = (int)(e / blkel), bi = 0;
+        while (idx >= nb - bi) {
+            idx -= nb - bi;
+            bi++;
+        }
+        const int bj = bi + idx;
+        const int le = (int)(e % blkel);
+        a = bi * BS + le / BS;
+        b = bj * BS + le % BS;
+        ok = a <= b && b < p;
+    }
+    T sr = 0, si = 0;
+    if (ok) {
         const int s0 = (int)((int64_t)splits * w / 16), s1 = (int)((int64_t)splits * (w + 1) / 16);
         for (int s = s0; s < s1; s++) {
-            const int64_t e = (int64_t)s * nblk * blkel + off;
+            const int64_t q = (int64_t)s * nblk * blkel + e;
             if constexpr (CPLX) {
-                sr += part[2 * e];
-                si += part[2 * e + 1];
+                sr += part[2 * q];
+                si += part[2 * q + 1];
             } else {
-                sr += part[e];
+                sr += part[q];
             }
         }
     }
@@ -271,14 +277,16 @@ __global__ __launch_bounds__(1024) void rank_gram_reduce_kernel(const T *__restr
         ti += red[k][lane][1];
     }
     tr *= scale;
-    ti *= scale;
-    if (lower) ti = -ti;  // H[r][c] = conj(H[c][r])
-    if (r == c) ti = 0;
+    ti = a == b ? T(0) : ti * scale;
+    const int64_t up = (int64_t)b * p + a, lo = (int64_t)a * p + b;   // (a, b) and (b, a)
     if constexpr (CPLX) {
-        H[2 * i] = tr;
-        H[2 * i + 1] = ti;
+        H[2 * up] = tr;
+        H[2 * up + 1] = ti;
+        H[2 * lo] = tr;
+        H[2 * lo + 1] = -ti;
     } else {
-        H[i] = tr;
+        H[up] = tr;
+        H[lo] = tr;
     }
 }
 
@@ -701,12 +709,15 @@ __global__ void rank_copy_kernel(const T *__restrict__ x, int64_t xs0, int64_t x
 template <typename T, bool CPLX, int WT>
 void gram_launch(hipStream_t st, const RankPlan &pl, const void *x, void *part, void *H) {
     const int nblk = pl.nb * (pl.nb + 1) / 2;
-    ProfScope ps(st, "rank_gram");
-    hipLaunchKernelGGL((rank_gram_kernel<T, CPLX, WT>), dim3(nblk * pl.splits), dim3(64), 0, st,
-                       (const T *)x, pl.gsa, pl.gsk, pl.p, pl.L, pl.kchunk, pl.nb, pl.gvec,
-                       (T *)part);
-    const int64_t pe = (int64_t)pl.p * pl.p;
-    hipLaunchKernelGGL((rank_gram_reduce_kernel<T, CPLX>), dim3((unsigned)((pe + 63) / 64)),
+    {
+        ProfScope ps(st, "rank_gram");
+        hipLaunchKernelGGL((rank_gram_kernel<T, CPLX, WT>), dim3(nblk * pl.splits), dim3(64), 0,
+                           st, (const T *)x, pl.gsa, pl.gsk, pl.p, pl.L, pl.kchunk, pl.nb,
+                           pl.gvec, (T *)part);
+    }
+    const int64_t ne = (int64_t)nblk * pl.bs * pl.bs;
+    ProfScope ps(st, "rank_gram_reduce");
+    hipLaunchKernelGGL((rank_gram_reduce_kernel<T, CPLX>), dim3((unsigned)((ne + 63) / 64)),
                        dim3(1024), 0, st, (const T *)part, pl.p, pl.bs, pl.nb, nblk, pl.splits,
                        (T)(1.0 / (double)pl.n), (T *)H);
 }

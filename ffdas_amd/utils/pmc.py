@@ -111,7 +111,10 @@ def kernel_stats(run_dir: str, top: int = 20) -> str:
     out = ["| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
     for r in rows[:top]:
         name = r["Name"]
-        name = name.split("(")[0] if "ffdas::" in name else name[:90]
+        if "ffdas::" in name:
+            name = name.replace("(anonymous namespace)::", "").split("(")[0]
+        else:
+            name = name[:90]
         out.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                    f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
                    f"{float(r['Percentage']):.2f} |")

@@ -321,7 +321,7 @@ FFDAS_API ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_
  * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
  * "das_bwd_x", "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
- * "transpose_gout", "rank_gram", "rank_jacobi", "rank_project").
+ * "transpose_gout", "rank_gram", "rank_gram_reduce", "rank_jacobi", "rank_project").
  * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);
