@@ -3,7 +3,8 @@
 Inputs are small seeded problems (ffdas_amd.workloads.small_random plus two hand-made
 geometries); expected outputs come from oracle/brute.py, the numpy FP64 restatement of the
 reference's definition (the reference itself is CUDA-only and cannot run here, and it ships
-no golden vectors of its own).  Run:  python tests/golden/make_golden.py
+no golden vectors of its own).  Also writes SHA256SUMS of the fixtures.
+Run:  python tests/golden/make_golden.py
 """
 from pathlib import Path
 import sys
@@ -37,6 +38,11 @@ def main():
     save("complex", W.small_random(seed=102, B=2, C=4, T=2, S=64, N=(3, 3), complex_iq=True))
     save("sparse", W.small_random(seed=103, B=2, C=4, T=6, K=3, S=72, N=(5, 2), sparse=True))
     save("scale", W.small_random(seed=104, B=1, C=4, T=3, S=64, N=(7,)), scale=1.04)
+    # checksums of what was written (tests/test_oracle.py checks them)
+    import hashlib
+    lines = [f"{hashlib.sha256(f.read_bytes()).hexdigest()}  {f.name}"
+             for f in sorted(OUT.glob("*.npz"))]
+    (OUT / "SHA256SUMS").write_text("\n".join(lines) + "\n")
 
 
 if __name__ == "__main__":

@@ -326,3 +326,14 @@ def test_fp32_oracle_error_is_bounded():
     a = oracle.das(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, bits=64)
     b = oracle.das(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, bits=32)
     assert oracle.rel_l2(b, a) < 5e-5
+
+
+def test_golden_fixture_checksums():
+    """The committed fixtures are the ones tests/golden/make_golden.py wrote (SHA256SUMS)."""
+    import hashlib
+    from pathlib import Path
+    gold = Path(__file__).resolve().parent / "golden"
+    lines = [l.split() for l in (gold / "SHA256SUMS").read_text().splitlines() if l.strip()]
+    assert len(lines) == len(list(gold.glob("*.npz")))
+    for digest, name in lines:
+        assert hashlib.sha256((gold / name).read_bytes()).hexdigest() == digest, name
