@@ -2,9 +2,8 @@
 // src/truncate_rank/truncate_rank_impl.cuh:382-531; Python bindings/python/ffdas/
 // truncate_rank.py:28-82).  Gram matrix and projection run on the kernels of
 // rank_kernels.hip; the p x p Hermitian eigendecomposition runs on the one-workgroup Jacobi
-// kernel for real data whose A and V fit in LDS (p up to about 140 FP32 / 98 FP64), otherwise
-// on rocSOLVER (syevd/heevd), loaded on first use so that the library does not pull rocSOLVER
-// in at load time.
+// kernel for p <= 64, otherwise on rocSOLVER (syevd/heevd), loaded on first use so that the
+// library does not pull rocSOLVER in at load time.
 #include <dlfcn.h>
 
 #include <climits>
@@ -61,6 +60,7 @@ const Solver &solver() {
 size_t align_up(size_t b) { return (b + 255) & ~size_t(255); }
 
 constexpr int kJacobiSweeps = 30;
+constexpr int kJacobiMaxP = 64;
 
 // The solver's status word, read on the host: the one synchronisation of the call, as the
 // reference's (truncate_rank_impl.cuh:82-87).
@@ -75,20 +75,21 @@ ffdas_error_t check_info(ffdas_context &h, const int *info, ffdas_error_t fail) 
 // Eigendecomposition of the column-major p x p Hermitian H in place (eigenvectors in the
 // columns, eigenvalues ascending), then a check of the solver's info on the host.
 ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E, int *info) {
-    // The one-workgroup Jacobi kernel (A and V in LDS) for real data that fits, rocSOLVER
-    // otherwise: measured on gfx950 at p = 64, Jacobi 0.98 ms vs rocSOLVER 1.1 ms for real
-    // FP32, 1.42 ms vs 1.2 ms for complex.  FFDAS_RANK_EIG=jacobi|rocsolver overrides; Jacobi
-    // also stands in when rocSOLVER cannot be loaded.
+    // The one-workgroup Jacobi kernel (A and V in LDS) up to p = 64, rocSOLVER above:
+    // measured on gfx950, at p = 64 Jacobi 0.74 ms vs rocSOLVER 1.19 ms for real FP32 and
+    // 1.04 vs 1.30 ms for complex; its p^3 cost per sweep loses above (p = 96 complex
+    // 3.3 ms, p = 128 real 5.4 ms, against rocSOLVER's 1.5-1.8 ms).
+    // FFDAS_RANK_EIG=jacobi (any p whose A and V fit in LDS) or =rocsolver overrides.
     const char *force = std::getenv("FFDAS_RANK_EIG");
     const std::string want = force ? force : "";
-    const bool fits = rank_jacobi_fits(pl, h.lds_bytes);
-    const Solver &s = solver();
-    const bool use_jacobi =
-        fits && (want == "jacobi" || (want != "rocsolver" && (!pl.cplx || !s.ok)));
-    if (use_jacobi) {
+    const bool jacobi = want == "jacobi" ? rank_jacobi_fits(pl, h.lds_bytes)
+                                         : want != "rocsolver" && pl.p <= kJacobiMaxP &&
+                                               rank_jacobi_fits(pl, h.lds_bytes);
+    if (jacobi) {
         FFDAS_TRY(launch_rank_jacobi(h.stream, pl, H, info, kJacobiSweeps));
         return check_info(h, info, FFDAS_ERROR_CONVERGENCE_FAILED);
     }
+    const Solver &s = solver();
     if (!s.ok) return FFDAS_ERROR_CUSOLVER;
     if (!h.blas) {
         rocblas_handle bh = nullptr;

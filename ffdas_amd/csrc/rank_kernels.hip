@@ -344,8 +344,17 @@ struct Eps<double> {
 
 size_t jacobi_lds_bytes(int pe, size_t esz, size_t tsz) {
     const int npair = pe / 2;
-    return 2 * (size_t)pe * pe * esz + (size_t)npair * (4 * tsz + 2 * sizeof(int)) + 64;
+    const size_t ntri = (size_t)npair * (npair + 1) / 2;
+    return 2 * (size_t)pe * pe * esz + (size_t)npair * (4 * tsz + 2 * sizeof(int)) +
+           ntri * sizeof(unsigned short) + 64;
 }
+
+// rotation arithmetic: hardware reciprocal / reciprocal square root (about 1 ulp) in FP32,
+// IEEE operations in FP64
+__device__ __forceinline__ float jdiv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double jdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float jrsqrt(float a) { return rsqrtf(a); }
+__device__ __forceinline__ double jrsqrt(double a) { return 1.0 / sqrt(a); }
 
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, int max_sweeps,
@@ -361,6 +370,9 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
     T *pm = pr + npair;
     int *pi = reinterpret_cast<int *>(pm + npair);
     int *pj = pi + npair;
+    // the blocks (P, Q), P <= Q, of a round: P << 8 | Q
+    unsigned short *tri = reinterpret_cast<unsigned short *>(pj + npair);
+    const int ntri = npair * (npair + 1) / 2;
     __shared__ T red[16];
     __shared__ int flag[3];
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -380,6 +392,14 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
         if constexpr (CPLX) A[C * e + 1] = ai;
         V[C * e] = r == c ? T(1) : T(0);
         if constexpr (CPLX) V[C * e + 1] = 0;
+    }
+    for (int w = tid; w < ntri; w += nt) {
+        int P = 0, idx = w;
+        while (idx >= npair - P) {
+            idx -= npair - P;
+            P++;
+        }
+        tri[w] = (unsigned short)((P << 8) | (P + idx));
     }
     for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
     if ((tid & 63) == 0) red[tid >> 6] = sq;
@@ -411,13 +431,14 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                     // complex: rotate by |a_ij| after the phase factor; real: the signed a_ij
                     const T r = CPLX ? sqrt(zr * zr + zi * zi) : zr;
                     if (fabs(r) > thresh) {
-                        const T tau = (ajj - aii) / (2 * r);
-                        const T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + sqrt(1 + tau * tau));
-                        c = 1 / sqrt(1 + t * t);
+                        const T tau = jdiv(ajj - aii, 2 * r);
+                        const T t = jdiv(tau >= 0 ? T(1) : T(-1), fabs(tau) + sqrt(1 + tau * tau));
+                        c = jrsqrt(1 + t * t);
                         s = t * c;
                         if constexpr (CPLX) {
-                            er = zr / r;
-                            ei = zi / r;
+                            const T ir = jdiv(T(1), r);
+                            er = zr * ir;
+                            ei = zi * ir;
                         }
                         flag[sweep % 3] = 1;
                     }
@@ -431,9 +452,8 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
             }
             __syncthreads();
             // blocks (P, Q), P <= Q: B' = J_P^T D_P^H B D_Q J_Q, mirrored into (Q, P)
-            for (int w = tid; w < npair * npair; w += nt) {
-                const int P = w / npair, Q = w % npair;
-                if (P > Q) continue;
+            for (int w = tid; w < ntri; w += nt) {
+                const int PQ = tri[w], P = PQ >> 8, Q = PQ & 255;
                 const T cP = pc[P], sP = ps[P], cQ = pc[Q], sQ = ps[Q];
                 if (sP == 0 && sQ == 0) continue;
                 const int iP = pi[P], jP = pj[P], iQ = pi[Q], jQ = pj[Q];
@@ -822,12 +842,17 @@ ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, in
     const int pe = (pl.p + 1) / 2 * 2;
     const size_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
     const size_t lds = jacobi_lds_bytes(pe, esz, pl.dbl ? 8 : 4);
+    int threads = 1024;
+    if (const char *e = std::getenv("FFDAS_RANK_JACOBI_THREADS")) {
+        const int v = std::atoi(e);
+        if (v == 256 || v == 512 || v == 1024) threads = v;
+    }
     ProfScope ps(st, "rank_jacobi");
 #define FFDAS_RANK_JACOBI(T, C)                                                               \
     do {                                                                                     \
         (void)hipFuncSetAttribute((const void *)rank_jacobi_kernel<T, C>,                    \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
-        hipLaunchKernelGGL((rank_jacobi_kernel<T, C>), dim3(1), dim3(1024), lds, st, (T *)H, \
+        hipLaunchKernelGGL((rank_jacobi_kernel<T, C>), dim3(1), dim3(threads), lds, st, (T *)H, \
                            pl.p, pe, max_sweeps, info);                                      \
     } while (0)
     if (pl.dbl && pl.cplx)

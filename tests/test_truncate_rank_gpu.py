@@ -196,14 +196,14 @@ def test_both_eigensolvers(dev, monkeypatch, solver, dtype, shape, start, stop):
     _check(x, start, stop, TOL[dtype])
 
 
-def test_jacobi_is_the_default_for_small_real_p(dev, monkeypatch):
+def test_jacobi_is_the_default_for_small_p(dev, monkeypatch):
     import ctypes
     import torch
     import ffdas_amd
     from ffdas_amd._lib import lib
     monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
     L = lib()
-    x = _rand((64, 4096), torch.float32, 17).to(dev)
+    x = _rand((64, 4096), torch.complex64, 17).to(dev)
     L.ffdas_profile_reset()
     L.ffdas_profile_enable(1)
     try:
