@@ -571,13 +571,17 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
 // ---------------------------------------------------------------------------------------
 // 256 threads = 4 waves, each wave 16 projection vectors j = j0 + jl.  Coefficient tiles
 // q0 + [0, 16*RT).  base: 0 -> out = sign*P w, 1 -> out = x + sign*P w, 2 -> out += sign*P w.
-template <typename T, bool CPLX, int RT>
+// NK > 0 (FP32, pp <= 16*NK): the x values a lane loads in the first product are exactly the
+// ones it needs again for base 1 (rows 4g + s of each 16-block), so they stay in registers
+// and x is read from memory once.
+template <typename T, bool CPLX, int RT, int NK>
 __global__ __launch_bounds__(256) void rank_project_kernel(
     const T *x, int64_t xsj, int64_t xsf, bool xvec, T *out, int64_t osj,
     int64_t osf, bool ovec, int64_t L, int p, int pp, int rp, int q0,
     const T *__restrict__ Qh, const T *__restrict__ Qn, int base, T sign) {
     using M = Mfma<T>;
     using V4 = typename M::V4;
+    static_assert(NK == 0 || M::kRowsInRegs, "register reuse needs the FP32 output layout");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int jl = lane & 15, g = lane >> 4;
     const int64_t j = ((int64_t)blockIdx.x * 4 + wave) * 16 + jl;
@@ -591,10 +595,9 @@ __global__ __launch_bounds__(256) void rank_project_kernel(
         ci[t] = V4{0, 0, 0, 0};
     }
     // C[q][j] = sum_f Qh[q][f] w_j[f]
-    for (int kb = 0; kb < pp; kb += 16) {
+    auto product1 = [&](int kb, T (&wr)[4], T (&wi)[4]) {
         const int f0 = kb + 4 * g;
         const int nf = !jok ? 0 : (p - f0 >= 4 ? 4 : (p - f0 > 0 ? p - f0 : 0));
-        T wr[4], wi[4];
         load4<T, CPLX>(x, xo + (int64_t)f0 * xsf, xsf, nf, xvec, wr, wi);
 #pragma unroll
         for (int t = 0; t < RT; t++) {
@@ -610,10 +613,22 @@ __global__ __launch_bounds__(256) void rank_project_kernel(
                 }
             }
         }
+    };
+    T kr[NK > 0 ? NK : 1][4], ki[NK > 0 ? NK : 1][4];
+    if constexpr (NK > 0) {
+#pragma unroll
+        for (int b = 0; b < NK; b++)
+            if (16 * b < pp) product1(16 * b, kr[b], ki[b]);
+    } else {
+        for (int kb = 0; kb < pp; kb += 16) {
+            T wr[4], wi[4];
+            product1(kb, wr, wi);
+        }
     }
     // D[f][j] = sum_q Qn[f][q] C[q][j]; lane holds C[16t + row(g, s)][jl] = c[t][s], the B
-    // operand of k-slot g at step s when the A operand uses the same q = 16t + row(g, s)
-    for (int ft = 0; ft < pp; ft += 16) {
+    // operand of k-slot g at step s when the A operand uses the same q = 16t + row(g, s).
+    // xr/xi: the lane's x values of this 16-block when held in registers.
+    auto product2 = [&](int ft, const T (&xr)[4], const T (&xi)[4], bool held) {
         V4 dr = V4{0, 0, 0, 0}, di = V4{0, 0, 0, 0};
 #pragma unroll
         for (int t = 0; t < RT; t++) {
@@ -655,14 +670,20 @@ __global__ __launch_bounds__(256) void rank_project_kernel(
                     out[oe] = br + sign * dr[v];
                 }
             }
-            continue;
+            return;
         }
         // lane holds D[ft + 4g + v][j]
         const int f0 = ft + 4 * g;
         const int nf = !jok ? 0 : (p - f0 >= 4 ? 4 : (p - f0 > 0 ? p - f0 : 0));
-        if (nf == 0) continue;
+        if (nf == 0) return;
         T br[4], bi[4];
-        if (base == 1) {
+        if (base == 1 && held) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                br[v] = xr[v];
+                bi[v] = xi[v];
+            }
+        } else if (base == 1) {
             load4<T, CPLX>(x, xo + (int64_t)f0 * xsf, xsf, nf, xvec, br, bi);
         } else if (base == 2) {
             load4<T, CPLX>(out, oo + (int64_t)f0 * osf, osf, nf, ovec, br, bi);
@@ -707,6 +728,14 @@ __global__ __launch_bounds__(256) void rank_project_kernel(
                 }
             }
         }
+    };
+    if constexpr (NK > 0) {
+#pragma unroll
+        for (int b = 0; b < NK; b++)
+            if (16 * b < pp) product2(16 * b, kr[b], ki[b], true);
+    } else {
+        const T zr[4] = {0, 0, 0, 0}, zi[4] = {0, 0, 0, 0};
+        for (int ft = 0; ft < pp; ft += 16) product2(ft, zr, zi, false);
     }
 }
 
@@ -751,14 +780,29 @@ void gram_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *part
     }
 }
 
+template <typename T, bool CPLX, int RT, int NK>
+void project_launch_nk(hipStream_t st, const RankPlan &pl, const void *x, void *out, int q0,
+                       const void *Qh, const void *Qn, int base, double sign) {
+    const unsigned blocks = (unsigned)((pl.L + 63) / 64);
+    ProfScope ps(st, "rank_project");
+    hipLaunchKernelGGL((rank_project_kernel<T, CPLX, RT, NK>), dim3(blocks), dim3(256), 0, st,
+                       (const T *)x, pl.xsj, pl.xsf, pl.xvec, (T *)out, pl.osj, pl.osf, pl.ovec,
+                       pl.L, pl.p, pl.pp, pl.rp, q0, (const T *)Qh, (const T *)Qn, base, (T)sign);
+}
+
+// FP32 keeps the lane's x values in registers between the two products when they fit: up to
+// 4 blocks of 16 (p <= 64) for any r, up to 16 blocks (p <= 256) for r <= 32
 template <typename T, bool CPLX, int RT>
 void project_launch(hipStream_t st, const RankPlan &pl, const void *x, void *out, int q0,
                     const void *Qh, const void *Qn, int base, double sign) {
-    const unsigned blocks = (unsigned)((pl.L + 63) / 64);
-    ProfScope ps(st, "rank_project");
-    hipLaunchKernelGGL((rank_project_kernel<T, CPLX, RT>), dim3(blocks), dim3(256), 0, st,
-                       (const T *)x, pl.xsj, pl.xsf, pl.xvec, (T *)out, pl.osj, pl.osf, pl.ovec,
-                       pl.L, pl.p, pl.pp, pl.rp, q0, (const T *)Qh, (const T *)Qn, base, (T)sign);
+    constexpr bool fp32 = sizeof(T) == 4;
+    if (fp32 && pl.pp <= 64 && base == 1)
+        project_launch_nk<T, CPLX, RT, fp32 ? 4 : 0>(st, pl, x, out, q0, Qh, Qn, base, sign);
+    else if (fp32 && RT <= 2 && pl.pp <= 256 && base == 1)
+        project_launch_nk<T, CPLX, RT, fp32 && RT <= 2 ? 16 : 0>(st, pl, x, out, q0, Qh, Qn,
+                                                                 base, sign);
+    else
+        project_launch_nk<T, CPLX, RT, 0>(st, pl, x, out, q0, Qh, Qn, base, sign);
 }
 
 template <typename T, bool CPLX>
