@@ -201,6 +201,7 @@ struct RankPlan {
     int64_t osj = 0, osf = 0;      // same for out
     bool xvec = false, ovec = false;
     bool keep = true;              // project onto the kept set (else subtract the dropped set)
+    bool lds_gram = false;         // LDS-staged Gram kernel (FP32, p <= 256)
     int r = 0;                     // eigenvector columns used
     int q_split = 0, c_first = 0, c_second = 0;
     // launch shape (rank_plan_launch_shape)

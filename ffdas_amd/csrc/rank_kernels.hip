@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <string>
 
 #include "internal.h"
 
@@ -224,6 +225,170 @@ __global__ __launch_bounds__(64) void rank_gram_kernel(const T *__restrict__ x, 
                     dst[e] = accr[a][b][v];
                 }
             }
+}
+
+// Gram over LDS-staged k-tiles (FP32, contiguous 16-byte aligned Gram vectors, p <= 256
+// real / 208 complex):
+// one workgroup of 8 waves per k-range stages X[0:pp, k:k+32] into LDS (double-buffered,
+// each element read from HBM once) and its waves compute every 16 x 16 tile (ta <= tb) of the
+// upper triangle, wave w taking tiles w, w + 8, ...  Operands come from LDS as 16-byte reads
+// of four consecutive k (the lane's k-slot g covers k = 4g..4g+3 of each 16-k group), so
+// both operands use the same k mapping.  Partials: part[split][tile][16][16], the layout of
+// the 16 x 16 block kernel, summed by rank_gram_reduce_kernel.
+constexpr int kGramKT = 32;      // k per stage
+constexpr int kGramWaves = 8;
+
+// MAXT: tiles per wave, ceil(nt (nt + 1) / 2 / 8) for nt = pp / 16 (17 at p = 256)
+template <bool CPLX, int MAXT>
+__global__ __launch_bounds__(512) void rank_gram_lds_kernel(const float *__restrict__ x,
+                                                            int64_t sa, int p, int64_t L,
+                                                            int64_t kchunk,
+                                                            float *__restrict__ part) {
+    using M = Mfma<float>;
+    using V4 = M::V4;
+    constexpr int C = CPLX ? 2 : 1;
+    constexpr int ROWF = kGramKT * C + 4;          // floats per staged row (+4: bank offset)
+    extern __shared__ __align__(16) float glds[];
+    const int pp = (p + 15) & ~15, nt = pp >> 4, ntri = nt * (nt + 1) / 2;
+    float *stage0 = glds, *stage1 = glds + pp * ROWF;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int jl = lane & 15, g = lane >> 4;
+    const int64_t k_lo = (int64_t)blockIdx.x * kchunk;
+    const int64_t k_hi = k_lo + kchunk < L ? k_lo + kchunk : L;
+    const int iters = k_hi > k_lo ? (int)((k_hi - k_lo + kGramKT - 1) / kGramKT) : 0;
+
+    // this wave's tiles, (ta << 8) | tb
+    int tab[MAXT];
+    int ntiles = 0;
+#pragma unroll
+    for (int i = 0; i < MAXT; i++) {
+        const int t = wave + kGramWaves * i;
+        int a = 0, idx = t;
+        while (a < nt && idx >= nt - a) {
+            idx -= nt - a;
+            a++;
+        }
+        tab[i] = (a << 8) | (a + idx);
+        if (t < ntri) ntiles = i + 1;
+    }
+    V4 accr[MAXT], acci[MAXT];
+#pragma unroll
+    for (int i = 0; i < MAXT; i++) {
+        accr[i] = V4{0, 0, 0, 0};
+        acci[i] = V4{0, 0, 0, 0};
+    }
+
+    // staging: per row kGramKT * C floats = 8 * C float4; thread -> (row, float4 slot)
+    constexpr int F4 = kGramKT * C / 4;
+    const int nslots = pp * F4;
+    auto fetch = [&](int64_t k0, int slot, float4 &v) {
+        const int row = slot / F4, q = slot - row * F4;
+        const int64_t k = k0 + (int64_t)q * 4 / C;            // first k of this float4
+        v = float4{0, 0, 0, 0};
+        if (row >= p) return;
+        const float *src = x + ((int64_t)row * sa + k) * C;
+        if (k + 4 / C <= k_hi) {
+            v = *reinterpret_cast<const float4 *>(src);
+        } else {
+            if (k < k_hi) v.x = src[0];
+            if (k + 1 / C < k_hi) v.y = src[1];
+            if (k + 2 / C < k_hi) v.z = src[2];
+            if (k + 3 / C < k_hi) v.w = src[3];
+        }
+    };
+    auto put = [&](float *stg, int slot, const float4 &v) {
+        const int row = slot / F4, q = slot - row * F4;
+        *reinterpret_cast<float4 *>(stg + row * ROWF + q * 4) = v;
+    };
+    constexpr int kMaxSlotsPerThread = (256 * (kGramKT * 2 / 4) + 511) / 512;   // 8
+    float4 pre[kMaxSlotsPerThread];
+    if (iters > 0) {
+#pragma unroll
+        for (int i = 0; i < kMaxSlotsPerThread; i++) {
+            const int slot = tid + 512 * i;
+            if (slot < nslots) {
+                float4 v;
+                fetch(k_lo, slot, v);
+                put(stage0, slot, v);
+            }
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it < iters; it++) {
+        float *cur = (it & 1) ? stage1 : stage0;
+        float *nxt = (it & 1) ? stage0 : stage1;
+        const bool more = it + 1 < iters;
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < kMaxSlotsPerThread; i++) {
+                const int slot = tid + 512 * i;
+                if (slot < nslots) fetch(k_lo + (int64_t)(it + 1) * kGramKT, slot, pre[i]);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kGramKT; kk += 16) {
+            // lane's k = kk + 4g + s, s = 0..3
+#pragma unroll
+            for (int i = 0; i < MAXT; i++) {
+                if (i >= ntiles) continue;
+                const float *ra = cur + ((tab[i] >> 8) * 16 + jl) * ROWF + (kk + 4 * g) * C;
+                const float *rb = cur + ((tab[i] & 255) * 16 + jl) * ROWF + (kk + 4 * g) * C;
+                if constexpr (!CPLX) {
+                    const float4 a = *reinterpret_cast<const float4 *>(ra);
+                    const float4 b = *reinterpret_cast<const float4 *>(rb);
+                    accr[i] = M::mma(a.x, b.x, accr[i]);
+                    accr[i] = M::mma(a.y, b.y, accr[i]);
+                    accr[i] = M::mma(a.z, b.z, accr[i]);
+                    accr[i] = M::mma(a.w, b.w, accr[i]);
+                } else {
+                    const float4 a0 = *reinterpret_cast<const float4 *>(ra);
+                    const float4 a1 = *reinterpret_cast<const float4 *>(ra + 4);
+                    const float4 b0 = *reinterpret_cast<const float4 *>(rb);
+                    const float4 b1 = *reinterpret_cast<const float4 *>(rb + 4);
+                    const float ar[4] = {a0.x, a0.z, a1.x, a1.z}, ai[4] = {a0.y, a0.w, a1.y, a1.w};
+                    const float br[4] = {b0.x, b0.z, b1.x, b1.z}, bi[4] = {b0.y, b0.w, b1.y, b1.w};
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        // v_a conj(v_b) = (ar br + ai bi) + i (ai br - ar bi)
+                        accr[i] = M::mma(ar[s], br[s], accr[i]);
+                        accr[i] = M::mma(ai[s], bi[s], accr[i]);
+                        acci[i] = M::mma(ai[s], br[s], acci[i]);
+                        acci[i] = M::mma(-ar[s], bi[s], acci[i]);
+                    }
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < kMaxSlotsPerThread; i++) {
+                const int slot = tid + 512 * i;
+                if (slot < nslots) put(nxt, slot, pre[i]);
+            }
+        }
+        __syncthreads();
+    }
+    // partials of this k-range: tile t = wave + 8 i, element (4g + v, jl)
+#pragma unroll
+    for (int i = 0; i < MAXT; i++) {
+        if (i >= ntiles) continue;
+        const int t = wave + kGramWaves * i;
+        float *dst = part + ((int64_t)blockIdx.x * ntri + t) * 256 * C;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int e = (4 * g + v) * 16 + jl;
+            if constexpr (CPLX) {
+                dst[2 * e] = accr[i][v];
+                dst[2 * e + 1] = acci[i][v];
+            } else {
+                dst[e] = accr[i][v];
+            }
+        }
+    }
+}
+
+size_t gram_lds_bytes(int p, bool cplx) {
+    const int pp = (p + 15) & ~15;
+    return 2 * (size_t)pp * (kGramKT * (cplx ? 2 : 1) + 4) * sizeof(float);
 }
 
 // H (column-major p x p) from the block partials.  Lanes run over the entries of the stored
@@ -773,6 +938,42 @@ void gram_launch(hipStream_t st, const RankPlan &pl, const void *x, void *part, 
 
 template <typename T, bool CPLX>
 void gram_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *part, void *H) {
+    if constexpr (sizeof(T) == 4) {
+        if (pl.lds_gram) {
+            const int nt = pl.pp / 16, ntri = nt * (nt + 1) / 2;
+            const size_t lds = gram_lds_bytes(pl.p, CPLX);
+            {
+                ProfScope ps(st, "rank_gram");
+                const int per_wave = (ntri + kGramWaves - 1) / kGramWaves;
+#define FFDAS_GRAM_LDS(MT)                                                                     \
+    do {                                                                                      \
+        (void)hipFuncSetAttribute((const void *)rank_gram_lds_kernel<CPLX, MT>,               \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);      \
+        hipLaunchKernelGGL((rank_gram_lds_kernel<CPLX, MT>), dim3(pl.splits), dim3(512), lds, \
+                           st, (const float *)x, pl.gsa, pl.p, pl.L, pl.kchunk,               \
+                           (float *)part);                                                    \
+    } while (0)
+                if (per_wave <= 2)
+                    FFDAS_GRAM_LDS(2);
+                else if (per_wave <= 5)
+                    FFDAS_GRAM_LDS(5);
+                else if (per_wave <= 8)
+                    FFDAS_GRAM_LDS(8);
+                else if (per_wave <= 12)
+                    FFDAS_GRAM_LDS(12);
+                else
+                    FFDAS_GRAM_LDS(17);
+#undef FFDAS_GRAM_LDS
+            }
+            ProfScope ps(st, "rank_gram_reduce");
+            const int64_t ne = (int64_t)ntri * 256;
+            hipLaunchKernelGGL((rank_gram_reduce_kernel<float, CPLX>),
+                               dim3((unsigned)((ne + 63) / 64)), dim3(1024), 0, st,
+                               (const float *)part, pl.p, 16, nt, ntri, pl.splits,
+                               (float)(1.0 / (double)pl.n), (float *)H);
+            return;
+        }
+    }
     switch (pl.bs) {
         case 16: gram_launch<T, CPLX, 1>(st, pl, x, part, H); break;
         case 32: gram_launch<T, CPLX, 2>(st, pl, x, part, H); break;
@@ -825,8 +1026,34 @@ void project_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *o
 }  // namespace
 
 void rank_plan_launch_shape(RankPlan &pl, int cu_count) {
+    pl.pp = (pl.p + 15) / 16 * 16;
+    // FP32 with contiguous aligned Gram vectors and p <= 256: the LDS-staged kernel
+    // (FFDAS_RANK_GRAM=wave selects the per-wave block kernel instead)
+    const char *gk = std::getenv("FFDAS_RANK_GRAM");
+    // (complex up to p = 208: its 17-tile variant would spill registers)
+    pl.lds_gram = !pl.dbl && pl.gvec && pl.p > 16 && pl.pp <= (pl.cplx ? 208 : 256) &&
+                  !(gk && std::string(gk) == "wave");
+    // it needs at least one workgroup per CU with 256 k each (measured: at 200 x 36864
+    // complex the block kernel is faster, at 128 x 131072 real the LDS kernel)
+    if (pl.lds_gram && pl.L < 256LL * std::max(cu_count, 1)) pl.lds_gram = false;
+    if (pl.lds_gram) {
+        const int nt = pl.pp / 16;
+        const int64_t ntri = (int64_t)nt * (nt + 1) / 2;
+        const int64_t esz = pl.cplx ? 8 : 4;
+        // two workgroups of 8 waves per CU, whole 32-k stages, at least 256 k each
+        int64_t splits = std::max<int64_t>(1, 2LL * std::max(cu_count, 1));
+        splits = std::min<int64_t>(splits, std::max<int64_t>(1, pl.L / 256));
+        int64_t kchunk = (pl.L + splits - 1) / splits;
+        kchunk = (kchunk + 31) / 32 * 32;
+        pl.kchunk = kchunk;
+        pl.splits = (int)((pl.L + kchunk - 1) / kchunk);
+        pl.bs = 16;
+        pl.nb = nt;
+        pl.part_bytes = (size_t)(pl.splits * ntri * 256 * esz);
+    }
     // measured on gfx950 (64 and 200 frames, real and complex FP32): single-tile blocks with
     // many waves for p <= 64, 32 x 32 blocks above; 64 x 64 blocks were slower everywhere
+    if (!pl.lds_gram) {
     pl.bs = pl.p <= 64 ? 16 : 32;
     if (const char *e = std::getenv("FFDAS_RANK_BS")) {
         const int v = std::atoi(e);
@@ -849,7 +1076,7 @@ void rank_plan_launch_shape(RankPlan &pl, int cu_count) {
     // a multiple of 8 (one k-range per XCD in turn); trailing k-ranges may be empty
     pl.splits = (int)(((pl.L + kchunk - 1) / kchunk + 7) / 8 * 8);
     pl.part_bytes = (size_t)(pl.splits * nblk * pl.bs * pl.bs * esz);
-    pl.pp = (pl.p + 15) / 16 * 16;
+    }
     const int tiles = (pl.r + 15) / 16;
     if (tiles <= 8) {
         pl.rt = tiles <= 1 ? 1 : (tiles <= 2 ? 2 : (tiles <= 4 ? 4 : 8));

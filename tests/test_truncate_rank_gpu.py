@@ -214,3 +214,20 @@ def test_jacobi_is_the_default_for_small_p(dev, monkeypatch):
     t, c = ctypes.c_double(), ctypes.c_int64()
     L.ffdas_profile_query(b"rank_jacobi", ctypes.byref(t), ctypes.byref(c))
     assert c.value == 1
+
+
+@pytest.mark.parametrize("gram", ["lds", "wave"])
+@pytest.mark.parametrize("dtype", ["float32", "complex64"])
+@pytest.mark.parametrize("shape, start", [((40, 70001), 2), ((200, 66000), 3),
+                                          ((17, 66001), 1)])
+def test_both_gram_kernels(dev, monkeypatch, gram, dtype, shape, start):
+    """The LDS-staged Gram kernel (FP32 default for contiguous rows, p <= 256 / 208 complex,
+    at least 256 k per CU) and the per-wave block kernel, with k extents that are not a
+    multiple of the 32-k stage."""
+    import torch
+    if gram == "wave":
+        monkeypatch.setenv("FFDAS_RANK_GRAM", "wave")
+    else:
+        monkeypatch.delenv("FFDAS_RANK_GRAM", raising=False)
+    x = _rand(shape, getattr(torch, dtype), 18).to(dev)
+    _check(x, start, None, TOL[dtype])
