@@ -6,7 +6,9 @@
 // library does not pull rocSOLVER in at load time.
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <string>
@@ -86,7 +88,15 @@ ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void 
                                          : want != "rocsolver" && pl.p <= kJacobiMaxP &&
                                                rank_jacobi_fits(pl, h.lds_bytes);
     if (jacobi) {
-        FFDAS_TRY(launch_rank_jacobi(h.stream, pl, H, info, kJacobiSweeps));
+        int sweeps = kJacobiSweeps;
+        if (const char *e = std::getenv("FFDAS_RANK_SWEEPS")) sweeps = std::max(1, std::atoi(e));
+        FFDAS_TRY(launch_rank_jacobi(h.stream, pl, H, info, sweeps));
+        if (std::getenv("FFDAS_RANK_DEBUG")) {
+            int hi[2] = {0, 0};
+            if (hipMemcpyAsync(hi, info, sizeof(hi), hipMemcpyDeviceToHost, h.stream) == hipSuccess &&
+                hipStreamSynchronize(h.stream) == hipSuccess)
+                std::fprintf(stderr, "ffdas: Jacobi p=%d sweeps=%d info=%d\n", pl.p, hi[1], hi[0]);
+        }
         return check_info(h, info, FFDAS_ERROR_CONVERGENCE_FAILED);
     }
     const Solver &s = solver();

@@ -521,22 +521,57 @@ __device__ __forceinline__ double jdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float jrsqrt(float a) { return rsqrtf(a); }
 __device__ __forceinline__ double jrsqrt(double a) { return 1.0 / sqrt(a); }
 
+template <typename T>
+struct JVec;
+template <>
+struct JVec<float> {
+    using V2 = float2;
+    using V4 = float4;
+};
+template <>
+struct JVec<double> {
+    using V2 = double2;
+    using V4 = double4;
+};
+
+// LDS element (r, i) of a complex or real matrix: one 8/16-byte access per element
+template <typename T, bool CPLX>
+__device__ __forceinline__ void jld(const T *M, int e, T &re, T &im) {
+    if constexpr (CPLX) {
+        const typename JVec<T>::V2 v = reinterpret_cast<const typename JVec<T>::V2 *>(M)[e];
+        re = v.x;
+        im = v.y;
+    } else {
+        re = M[e];
+        im = T(0);
+    }
+}
+template <typename T, bool CPLX>
+__device__ __forceinline__ void jst(T *M, int e, T re, T im) {
+    if constexpr (CPLX) {
+        typename JVec<T>::V2 v;
+        v.x = re;
+        v.y = im;
+        reinterpret_cast<typename JVec<T>::V2 *>(M)[e] = v;
+    } else {
+        M[e] = re;
+    }
+}
+
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, int max_sweeps,
                                                            int *info) {
+    using V4 = typename JVec<T>::V4;
     extern __shared__ __align__(16) unsigned char jlds[];
     constexpr int C = CPLX ? 2 : 1;
     const int npair = pe / 2;
     T *A = reinterpret_cast<T *>(jlds);
     T *V = A + (size_t)pe * pe * C;
-    T *pc = V + (size_t)pe * pe * C;      // per pair: c, s, cos(phi), sin(phi)
-    T *ps = pc + npair;
-    T *pr = ps + npair;
-    T *pm = pr + npair;
-    int *pi = reinterpret_cast<int *>(pm + npair);
-    int *pj = pi + npair;
+    // per pair: (c, s, cos(phi), sin(phi)) and (i, j), one vector access each
+    V4 *prm = reinterpret_cast<V4 *>(V + (size_t)pe * pe * C);
+    int2 *pij = reinterpret_cast<int2 *>(prm + npair);
     // the blocks (P, Q), P <= Q, of a round: P << 8 | Q
-    unsigned short *tri = reinterpret_cast<unsigned short *>(pj + npair);
+    unsigned short *tri = reinterpret_cast<unsigned short *>(pij + npair);
     const int ntri = npair * (npair + 1) / 2;
     __shared__ T red[16];
     __shared__ int flag[3];
@@ -553,10 +588,8 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
             ai = CPLX ? H[2 * h + 1] : T(0);
         }
         sq += ar * ar + ai * ai;
-        A[C * e] = ar;
-        if constexpr (CPLX) A[C * e + 1] = ai;
-        V[C * e] = r == c ? T(1) : T(0);
-        if constexpr (CPLX) V[C * e + 1] = 0;
+        jst<T, CPLX>(A, e, ar, ai);
+        jst<T, CPLX>(V, e, r == c ? T(1) : T(0), T(0));
     }
     for (int w = tid; w < ntri; w += nt) {
         int P = 0, idx = w;
@@ -591,8 +624,10 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                 }
                 T c = 1, s = 0, er = 1, ei = 0;
                 if (j < p) {
-                    const T aii = A[C * (i * pe + i)], ajj = A[C * (j * pe + j)];
-                    const T zr = A[C * (i * pe + j)], zi = CPLX ? A[C * (i * pe + j) + 1] : T(0);
+                    T aii, ajj, zr, zi, dummy;
+                    jld<T, CPLX>(A, i * pe + i, aii, dummy);
+                    jld<T, CPLX>(A, j * pe + j, ajj, dummy);
+                    jld<T, CPLX>(A, i * pe + j, zr, zi);
                     // complex: rotate by |a_ij| after the phase factor; real: the signed a_ij
                     const T r = CPLX ? sqrt(zr * zr + zi * zi) : zr;
                     if (fabs(r) > thresh) {
@@ -608,31 +643,33 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                         flag[sweep % 3] = 1;
                     }
                 }
-                pi[tid] = i;
-                pj[tid] = j;
-                pc[tid] = c;
-                ps[tid] = s;
-                pr[tid] = er;
-                pm[tid] = ei;
+                V4 q;
+                q.x = c;
+                q.y = s;
+                q.z = er;
+                q.w = ei;
+                prm[tid] = q;
+                pij[tid] = int2{i, j};
             }
             __syncthreads();
             // blocks (P, Q), P <= Q: B' = J_P^T D_P^H B D_Q J_Q, mirrored into (Q, P)
             for (int w = tid; w < ntri; w += nt) {
                 const int PQ = tri[w], P = PQ >> 8, Q = PQ & 255;
-                const T cP = pc[P], sP = ps[P], cQ = pc[Q], sQ = ps[Q];
+                const V4 qP = prm[P], qQ = prm[Q];
+                const T cP = qP.x, sP = qP.y, cQ = qQ.x, sQ = qQ.y;
                 if (sP == 0 && sQ == 0) continue;
-                const int iP = pi[P], jP = pj[P], iQ = pi[Q], jQ = pj[Q];
+                const int2 ijP = pij[P], ijQ = pij[Q];
+                const int iP = ijP.x, jP = ijP.y, iQ = ijQ.x, jQ = ijQ.y;
                 const int e00 = iP * pe + iQ, e01 = iP * pe + jQ, e10 = jP * pe + iQ,
                           e11 = jP * pe + jQ;
-                T b00r = A[C * e00], b01r = A[C * e01], b10r = A[C * e10], b11r = A[C * e11];
-                T b00i = 0, b01i = 0, b10i = 0, b11i = 0;
+                T b00r, b01r, b10r, b11r, b00i, b01i, b10i, b11i;
+                jld<T, CPLX>(A, e00, b00r, b00i);
+                jld<T, CPLX>(A, e01, b01r, b01i);
+                jld<T, CPLX>(A, e10, b10r, b10i);
+                jld<T, CPLX>(A, e11, b11r, b11i);
                 if constexpr (CPLX) {
-                    b00i = A[C * e00 + 1];
-                    b01i = A[C * e01 + 1];
-                    b10i = A[C * e10 + 1];
-                    b11i = A[C * e11 + 1];
                     // M = D_P^H B D_Q: column 1 times e^{-i phiQ}, row 1 times e^{i phiP}
-                    const T qr = pr[Q], qi = -pm[Q], pR = pr[P], pI = pm[P];
+                    const T qr = qQ.z, qi = -qQ.w, pR = qP.z, pI = qP.w;
                     T tr = b01r * qr - b01i * qi, ti = b01r * qi + b01i * qr;
                     b01r = tr;
                     b01i = ti;
@@ -663,52 +700,36 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                     o01r = o01i = o10r = o10i = 0;
                     o00i = o11i = 0;
                 }
-                A[C * e00] = o00r;
-                A[C * e01] = o01r;
-                A[C * e10] = o10r;
-                A[C * e11] = o11r;
-                if constexpr (CPLX) {
-                    A[C * e00 + 1] = o00i;
-                    A[C * e01 + 1] = o01i;
-                    A[C * e10 + 1] = o10i;
-                    A[C * e11 + 1] = o11i;
-                }
+                jst<T, CPLX>(A, e00, o00r, o00i);
+                jst<T, CPLX>(A, e01, o01r, o01i);
+                jst<T, CPLX>(A, e10, o10r, o10i);
+                jst<T, CPLX>(A, e11, o11r, o11i);
                 if (P != Q) {   // Hermitian mirror: (Q, P) block = conj transpose
-                    const int f00 = iQ * pe + iP, f01 = iQ * pe + jP, f10 = jQ * pe + iP,
-                              f11 = jQ * pe + jP;
-                    A[C * f00] = o00r;
-                    A[C * f10] = o01r;
-                    A[C * f01] = o10r;
-                    A[C * f11] = o11r;
-                    if constexpr (CPLX) {
-                        A[C * f00 + 1] = -o00i;
-                        A[C * f10 + 1] = -o01i;
-                        A[C * f01 + 1] = -o10i;
-                        A[C * f11 + 1] = -o11i;
-                    }
+                    jst<T, CPLX>(A, iQ * pe + iP, o00r, -o00i);
+                    jst<T, CPLX>(A, jQ * pe + iP, o01r, -o01i);
+                    jst<T, CPLX>(A, iQ * pe + jP, o10r, -o10i);
+                    jst<T, CPLX>(A, jQ * pe + jP, o11r, -o11i);
                 }
             }
             // V' = V G for every pair and row
             for (int w = tid; w < pe * npair; w += nt) {
                 const int row = w / npair, P = w % npair;
-                const T cP = pc[P], sP = ps[P];
+                const V4 qP = prm[P];
+                const T cP = qP.x, sP = qP.y;
                 if (sP == 0) continue;
-                const int ei0 = row * pe + pi[P], ej0 = row * pe + pj[P];
-                const T vir = V[C * ei0], vjr0 = V[C * ej0];
-                T vjr = vjr0, vji = 0, vii = 0;
+                const int2 ijP = pij[P];
+                const int ei0 = row * pe + ijP.x, ej0 = row * pe + ijP.y;
+                T vir, vii, vjr, vji;
+                jld<T, CPLX>(V, ei0, vir, vii);
+                jld<T, CPLX>(V, ej0, vjr, vji);
                 if constexpr (CPLX) {
-                    vii = V[C * ei0 + 1];
-                    const T vji0 = V[C * ej0 + 1];
-                    const T qr = pr[P], qi = -pm[P];   // times e^{-i phi}
-                    vjr = vjr0 * qr - vji0 * qi;
-                    vji = vjr0 * qi + vji0 * qr;
+                    const T qr = qP.z, qi = -qP.w;   // times e^{-i phi}
+                    const T tr = vjr * qr - vji * qi;
+                    vji = vjr * qi + vji * qr;
+                    vjr = tr;
                 }
-                V[C * ei0] = cP * vir - sP * vjr;
-                V[C * ej0] = sP * vir + cP * vjr;
-                if constexpr (CPLX) {
-                    V[C * ei0 + 1] = cP * vii - sP * vji;
-                    V[C * ej0 + 1] = sP * vii + cP * vji;
-                }
+                jst<T, CPLX>(V, ei0, cP * vir - sP * vjr, cP * vii - sP * vji);
+                jst<T, CPLX>(V, ej0, sP * vir + cP * vjr, sP * vii + cP * vji);
             }
             __syncthreads();
         }
@@ -718,17 +739,24 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
     // ascending order of the eigenvalues; ties by index
     for (int w = tid; w < p * p; w += nt) {
         const int row = w % p, k = w / p;
-        const T dk = A[C * (k * pe + k)];
+        T dk, dummy;
+        jld<T, CPLX>(A, k * pe + k, dk, dummy);
         int rank = 0;
         for (int l = 0; l < p; l++) {
-            const T dl = A[C * (l * pe + l)];
+            T dl;
+            jld<T, CPLX>(A, l * pe + l, dl, dummy);
             rank += (dl < dk || (dl == dk && l < k)) ? 1 : 0;
         }
+        T vr, vi;
+        jld<T, CPLX>(V, row * pe + k, vr, vi);
         const int64_t h = (int64_t)rank * p + row;
-        H[C * h] = V[C * (row * pe + k)];
-        if constexpr (CPLX) H[C * h + 1] = V[C * (row * pe + k) + 1];
+        H[C * h] = vr;
+        if constexpr (CPLX) H[C * h + 1] = vi;
     }
-    if (tid == 0) info[0] = done ? 0 : 1;
+    if (tid == 0) {
+        info[0] = done ? 0 : 1;
+        info[1] = sweep;   // sweeps run (diagnostics)
+    }
 }
 
 // ---------------------------------------------------------------------------------------
