@@ -15,10 +15,14 @@
 //
 // Kernels (all matrix products on the 16x16x4 MFMA, FP32 or FP64, complex as four real
 // products):
-//   rank_gram_kernel      split-K Gram: one wave per (upper-triangle block, k-range),
-//                         block partials written in full, no atomics;
+//   rank_gram_lds_kernel  FP32 split-K Gram over LDS-staged k-tiles: one workgroup per
+//                         k-range computes every tile of the upper triangle;
+//   rank_gram_kernel      split-K Gram for the other cases: one wave per (upper-triangle
+//                         block, k-range); both write block partials in full, no atomics;
 //   rank_gram_reduce      sums the partials in a fixed order, scales by 1/n, writes each
 //                         upper entry and its conjugate mirror, column-major out;
+//   rank_jacobi_kernel    eigendecomposition of H for p <= 64 in one workgroup (parallel
+//                         cyclic Jacobi in LDS); larger p go to rocSOLVER (rank_api.cpp);
 //   rank_select_kernel    gathers the chosen eigenvector columns into Qh (r x p, conjugated)
 //                         and Qn (p x r), zero padded to 16-multiples;
 //   rank_project_kernel   per wave 16 projection vectors: C = Qh w (MFMA, k over p), then

@@ -231,3 +231,30 @@ def test_both_gram_kernels(dev, monkeypatch, gram, dtype, shape, start):
         monkeypatch.delenv("FFDAS_RANK_GRAM", raising=False)
     x = _rand(shape, getattr(torch, dtype), 18).to(dev)
     _check(x, start, None, TOL[dtype])
+
+
+def test_abi_error_codes(dev):
+    """ffdas_truncate_rank's argument checks through the C-ABI: range, dtype, rank, shape."""
+    import torch
+    from ffdas_amd._core import element_code, handle_for
+    from ffdas_amd._lib import TensorDesc, lib
+    INVALID_ARGUMENT, INVALID_DIMS, UNSUPPORTED_TYPE = 1, 2, 3     # include/ffdas.h
+    L = lib()
+    x = torch.randn(8, 32, device=dev)
+    out = torch.empty_like(x)
+    h = handle_for(x)
+
+    def call(xt, ot, start, stop, xdims=None, odims=None):
+        xd = TensorDesc(xdims or tuple(xt.shape), None, element_code(xt))
+        od = TensorDesc(odims or tuple(ot.shape), None, element_code(ot))
+        return L.ffdas_truncate_rank(h.h, xd.ptr, xt.data_ptr(), start, stop, od.ptr,
+                                     ot.data_ptr())
+
+    assert call(x, out, 0, 8) == 0
+    for start, stop in [(-1, 8), (8, 8), (3, 3), (2, 9)]:
+        assert call(x, out, start, stop) == INVALID_ARGUMENT, (start, stop)
+    xh = x.half()
+    assert call(xh, torch.empty_like(xh), 0, 4) == UNSUPPORTED_TYPE
+    assert call(x, out.double(), 0, 4) == UNSUPPORTED_TYPE
+    assert call(x, out, 0, 4, xdims=(8, 4, 8), odims=(8, 4, 8)) == INVALID_DIMS
+    assert call(x, torch.empty(8, 16, device=dev), 0, 4) == INVALID_DIMS
