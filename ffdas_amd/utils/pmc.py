@@ -12,6 +12,9 @@ writes ``<name>.json`` (per-kernel bytes, used by bench.py for ``roofline.traffi
 
 stats: the kernel table of a ``rocprofv3 --kernel-trace --stats --output-format csv`` run.
 
+counters: per-kernel utilisation (VALU, LDS bank conflicts, L1/L2 hit rates, TA) from one
+``rocprofv3 --pmc`` run per pass of COUNTER_PASSES, each under <base>/<pass>/.
+
   python -m ffdas_amd.utils.pmc stats <run_dir> --out profiles/<name> --command "..."
 """
 from __future__ import annotations
@@ -121,6 +124,46 @@ def kernel_stats(run_dir: str, top: int = 20) -> str:
     return "\n".join(out) + "\n"
 
 
+# counter passes of `counters` (at most 8 SQ, 4 TCC, 4 TCP, 2 TA counters per pass)
+COUNTER_PASSES = {
+    "sq1": "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_BUSY_CYCLES SQ_WAVE_CYCLES SQ_INSTS_SALU "
+           "SQ_INSTS_VMEM_RD GRBM_GUI_ACTIVE",
+    "sq2": "SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_LDS",
+    "cache": "TCC_HIT_sum TCC_MISS_sum TCP_TOTAL_CACHE_ACCESSES_sum TCP_TCC_READ_REQ_sum "
+             "TA_BUSY_avr",
+}
+
+
+def counter_table(base: str, cus: int = 256, simds: int = 4, xcds: int = 8) -> str:
+    """Per-kernel utilisation from the passes of COUNTER_PASSES under base/<pass>/ (the last
+    dispatch of each kernel): duration from GRBM_GUI_ACTIVE (summed over the XCDs), VALU
+    busy = SQ_INSTS_VALU * 4 cycles / (CUs * SIMDs) / cycles, the share of cycles with an LDS
+    bank conflict per CU, L2 (TCC) and L1 (TCP) hit rates, TA busy."""
+    vals = collections.defaultdict(dict)
+    for name, counters in COUNTER_PASSES.items():
+        for c in counters.split():
+            for kern, v in read_counter(os.path.join(base, name), c).items():
+                vals[kern][c] = v[-1]
+    lines = ["| kernel | us | VALU busy | LDS instr/CU | LDS bank-conflict cycles | "
+             "L2 hit | L1 hit | TA busy |", "|---|---|---|---|---|---|---|---|"]
+    for kern in sorted(vals):
+        v = vals[kern]
+        if "GRBM_GUI_ACTIVE" not in v or not kern.endswith("kernel"):
+            continue
+        cyc = v["GRBM_GUI_ACTIVE"] / xcds
+        valu = v.get("SQ_INSTS_VALU", 0.0) * 4 / (cus * simds) / cyc
+        lds = v.get("SQ_INSTS_LDS", 0.0) / cus
+        bank = v.get("SQ_LDS_BANK_CONFLICT", 0.0) / cus / cyc
+        hit, miss = v.get("TCC_HIT_sum", 0.0), v.get("TCC_MISS_sum", 0.0)
+        l2 = hit / (hit + miss) if hit + miss else float("nan")
+        acc = v.get("TCP_TOTAL_CACHE_ACCESSES_sum", 0.0)
+        l1 = 1.0 - v.get("TCP_TCC_READ_REQ_sum", 0.0) / acc if acc else float("nan")
+        ta = v.get("TA_BUSY_avr", 0.0) / cyc
+        lines.append(f"| {kern} | {cyc / 2.4e3:.0f} | {valu:.2f} | {lds:.3g} | {bank:.2f} | "
+                     f"{l2:.2f} | {l1:.2f} | {ta:.2f} |")
+    return "\n".join(lines) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -134,7 +177,22 @@ def main():
     k.add_argument("--out", required=True, help="output path without extension")
     k.add_argument("--command", default="", help="the profiled command, for the summary")
     k.add_argument("--title", default="rocprofv3 kernel statistics")
+    c = sub.add_parser("counters")
+    c.add_argument("base", help="directory holding one run per pass: sq1/ sq2/ cache/")
+    c.add_argument("--out", required=True, help="output path without extension")
+    c.add_argument("--command", default="", help="the profiled command, for the summary")
     a = ap.parse_args()
+    if a.cmd == "counters":
+        text = (f"# Counter summary per kernel\n\nCommand per pass: `{a.command}` with "
+                f"`--pmc` set to each of:\n\n" +
+                "".join(f"- {k}: `{v}`\n" for k, v in COUNTER_PASSES.items()) +
+                "\nDuration from GRBM_GUI_ACTIVE at 2.4 GHz; VALU busy = VALU instructions x 4 "
+                "cycles over 256 CUs x 4 SIMDs; LDS bank-conflict cycles as a share of the "
+                "kernel's cycles per CU.\n\n" + counter_table(a.base))
+        with open(a.out + ".md", "w") as fh:
+            fh.write(text)
+        print(text)
+        return
     if a.cmd == "traffic":
         s = summarize(a.fetch_dir, a.write_dir)
         with open(a.out + ".json", "w") as fh:
