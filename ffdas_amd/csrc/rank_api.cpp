@@ -181,10 +181,20 @@ extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor
     const size_t rsz = pl.dbl ? 8 : 4;
     hipStream_t st = handle->stream;
     const int64_t kept = stop - start, dropped = p - kept;
+    const size_t xspan = (size_t)((m - 1) * xs0 + (n - 1) * xs1 + 1) * esz;
+    const size_t ospan = (size_t)((m - 1) * os0 + (n - 1) * os1 + 1) * esz;
+    const bool same_layout = x == out && xs0 == os0 && xs1 == os1;
+    // out overlapping x in another layout: every kernel below would read x after writing it
+    const bool alias = !same_layout && overlaps(x, xspan, out, ospan);
     if (dropped == 0) {
         // every component kept: the reconstruction is x itself
-        if (x == out && xs0 == os0 && xs1 == os1) return FFDAS_SUCCESS;
-        return launch_rank_copy(st, pl.dbl, pl.cplx, x, xs0, xs1, out, os0, os1, m, n);
+        if (same_layout) return FFDAS_SUCCESS;
+        if (!alias) return launch_rank_copy(st, pl.dbl, pl.cplx, x, xs0, xs1, out, os0, os1, m, n);
+        void *tmp = nullptr;
+        FFDAS_TRY(handle->workspace((size_t)m * n * esz, &tmp));
+        handle->xcopy.valid = false;
+        FFDAS_TRY(launch_rank_copy(st, pl.dbl, pl.cplx, x, xs0, xs1, tmp, n, 1, m, n));
+        return launch_rank_copy(st, pl.dbl, pl.cplx, tmp, n, 1, out, os0, os1, m, n);
     }
 
     pl.m = m;
@@ -216,10 +226,10 @@ extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor
     }
     rank_plan_launch_shape(pl, handle->cu_count);
 
-    // x is read again by later passes: an output that overlaps it needs a private copy
-    const size_t xspan = (size_t)((m - 1) * xs0 + (n - 1) * xs1 + 1) * esz;
-    const size_t ospan = (size_t)((m - 1) * os0 + (n - 1) * os1 + 1) * esz;
-    const bool need_copy = pl.passes > 1 && overlaps(x, xspan, out, ospan);
+    // x is read again by later passes, and by other lanes when out aliases it in another
+    // layout: such an output needs a private copy of x (out == x in the same layout and one
+    // pass is safe: each lane reads its elements before it writes them)
+    const bool need_copy = alias || (same_layout && pl.passes > 1);
 
     const size_t b_part = align_up(pl.part_bytes);
     const size_t b_H = align_up((size_t)p * p * esz);

@@ -258,3 +258,24 @@ def test_abi_error_codes(dev):
     assert call(x, out.double(), 0, 4) == UNSUPPORTED_TYPE
     assert call(x, out, 0, 4, xdims=(8, 4, 8), odims=(8, 4, 8)) == INVALID_DIMS
     assert call(x, torch.empty(8, 16, device=dev), 0, 4) == INVALID_DIMS
+
+
+def test_out_aliasing_x_in_another_layout(dev):
+    """out is x's own memory seen transposed: the call works from a private copy of x."""
+    import torch
+    from ffdas_amd._core import element_code, handle_for
+    from ffdas_amd._lib import TensorDesc, lib
+    import ffdas_amd
+    L = lib()
+    for start, stop in [(3, 40), (0, 48)]:          # subtract mode, and keep-all (a copy)
+        base = _rand((48, 48), torch.float32, 19).to(dev)
+        want = ffdas_amd.truncate_rank(base.clone(), start, stop)
+        x = base
+        out_view = base.t()                           # same memory, strides (1, 48)
+        xd = TensorDesc((48, 48), (48, 1), element_code(x))
+        od = TensorDesc((48, 48), (1, 48), element_code(x))
+        h = handle_for(x)
+        assert L.ffdas_truncate_rank(h.h, xd.ptr, x.data_ptr(), start, stop, od.ptr,
+                                     out_view.data_ptr()) == 0
+        torch.cuda.synchronize()
+        assert float((out_view - want).norm() / want.norm()) < 1e-5, (start, stop)
