@@ -828,6 +828,75 @@ __global__ __launch_bounds__(256) void sparse_fill_kernel(const int *__restrict_
     list[pos] = (r * ax.D0 + d0) * K + j;
 }
 
+// Block-ordered inverted index (T <= kMaxSparseT): the axial voxels are cut into blocks of
+// kSpBlock; sparse_bcount_kernel histograms each block's entries per transmit in LDS,
+// sparse_bscan_kernel scans each transmit's block counts (and sparse_scan_kernel the
+// transmit totals), and sparse_bfill_kernel writes every block's entries of transmit t
+// contiguously at that block's offset.  Each transmit's list is thus ordered by voxel block
+// (order inside a block is free), so a row of the scatter visits neighbouring voxels
+// together, as the dense rows do.  sp is the sparse table in axial order, (K, N).
+constexpr int kSpBlock = 1024;
+
+__global__ __launch_bounds__(256) void sparse_bcount_kernel(const int *__restrict__ sp, int K,
+                                                            int N, int T, int nblk,
+                                                            unsigned *__restrict__ tc) {
+    __shared__ unsigned h[kMaxSparseT];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const int q0 = blockIdx.x * kSpBlock;
+    const int q1 = min(N, q0 + kSpBlock);
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x)
+        for (int j = 0; j < K; j++) {
+            const int t = sp[(int64_t)j * N + q];
+            if ((unsigned)t < (unsigned)T) atomicAdd(h + t, 1u);
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T; i += blockDim.x) tc[(int64_t)i * nblk + blockIdx.x] = h[i];
+}
+
+// one block per transmit t: exclusive scan of tc[t][0..nblk) in place, tot[t] = the sum
+__global__ __launch_bounds__(256) void sparse_bscan_kernel(unsigned *__restrict__ tc, int nblk,
+                                                           unsigned *__restrict__ tot) {
+    __shared__ unsigned part[256];
+    unsigned *row = tc + (int64_t)blockIdx.x * nblk;
+    unsigned carry = 0;
+    for (int b0 = 0; b0 < nblk; b0 += 256) {
+        const int b = b0 + (int)threadIdx.x;
+        const unsigned v = b < nblk ? row[b] : 0u;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {     // inclusive Hillis-Steele scan
+            const unsigned add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (b < nblk) row[b] = carry + part[threadIdx.x] - v;
+        carry += part[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void sparse_bfill_kernel(const int *__restrict__ sp, int K,
+                                                           int N, int T, int nblk,
+                                                           const unsigned *__restrict__ tc,
+                                                           const unsigned *__restrict__ start,
+                                                           int *__restrict__ list) {
+    __shared__ unsigned h[kMaxSparseT];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const int q0 = blockIdx.x * kSpBlock;
+    const int q1 = min(N, q0 + kSpBlock);
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x)
+        for (int j = 0; j < K; j++) {
+            const int t = sp[(int64_t)j * N + q];
+            if ((unsigned)t >= (unsigned)T) continue;
+            const unsigned rank = atomicAdd(h + t, 1u);
+            list[start[t] + tc[(int64_t)t * nblk + blockIdx.x] + rank] = q * K + j;
+        }
+}
+
 // LDS slot of sample k: within each 32-sample block the low five bits are XORed with the
 // block number, so samples a multiple of 4 or 8 apart (neighbouring voxels along depth) land
 // in different LDS banks.  A bijection on [0, 32 * ceil(S / 32)).
@@ -1752,14 +1821,26 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
     ga.offsets = perm + 3 * (int64_t)g.N;
     ga.weights = ga.offsets + (int64_t)g.K * g.N;
     ga.sparse = g.sparse ? reinterpret_cast<const int *>(ga.weights + (int64_t)g.K * g.N) : nullptr;
-    // sparse: inverted index [count T][start T+1][cursor T][list K*N] after the geometry
+    // sparse: inverted index after the geometry, [count T][start T+1][list K*N][cursor T or
+    // block counts T*nblk]
     unsigned *sp_start = nullptr;
     int *sp_list = nullptr;
-    if (g.sparse) {
+    if (g.sparse && g.T <= dev::kMaxSparseT) {
+        unsigned *tot = reinterpret_cast<unsigned *>(perm + (3 + 3 * (int64_t)g.K) * g.N);
+        sp_start = tot + g.T;
+        sp_list = reinterpret_cast<int *>(sp_start + g.T + 1);
+        unsigned *tc = reinterpret_cast<unsigned *>(sp_list + (int64_t)g.K * g.N);
+        const int nblk = (g.N + dev::kSpBlock - 1) / dev::kSpBlock;
+        dev::sparse_bcount_kernel<<<nblk, 256, 0, st>>>(ga.sparse, g.K, g.N, g.T, nblk, tc);
+        dev::sparse_bscan_kernel<<<g.T, 256, 0, st>>>(tc, nblk, tot);
+        dev::sparse_scan_kernel<<<1, 64, 0, st>>>(tot, g.T, sp_start);
+        dev::sparse_bfill_kernel<<<nblk, 256, 0, st>>>(ga.sparse, g.K, g.N, g.T, nblk, tc,
+                                                        sp_start, sp_list);
+    } else if (g.sparse) {
         unsigned *count = reinterpret_cast<unsigned *>(perm + (3 + 3 * (int64_t)g.K) * g.N);
         sp_start = count + g.T;
-        unsigned *cursor = sp_start + g.T + 1;
-        sp_list = reinterpret_cast<int *>(cursor + g.T);
+        sp_list = reinterpret_cast<int *>(sp_start + g.T + 1);
+        unsigned *cursor = reinterpret_cast<unsigned *>(sp_list + (int64_t)g.K * g.N);
         const int64_t entries = (int64_t)g.K * g.N;
         dev::fill_zero_kernel<<<(g.T + 255) / 256, 256, 0, st>>>(reinterpret_cast<float *>(count), g.T);
         dev::fill_zero_kernel<<<(g.T + 255) / 256, 256, 0, st>>>(reinterpret_cast<float *>(cursor), g.T);
@@ -2155,7 +2236,11 @@ size_t backward_workspace(const DasProblem &p) {
     const size_t esz = 8 * (p.cplx ? 2 : 1);   // pre-scaled double grad_out
     const size_t nchunks = (size_t)(p.B + fs - 1) / fs;
     const size_t perm = (size_t)p.N * (3 + 3 * (size_t)p.K) * sizeof(float);   // axial geometry
-    const size_t inv = p.sparse ? ((3 * (size_t)p.T + 1) + (size_t)p.K * p.N) * 4 : 0;
+    // [count T][start T+1][list K*N][cursor T or block counts T * ceil(N / 1024)]
+    const size_t nblk = ((size_t)p.N + 1023) / 1024;
+    const size_t inv =
+        p.sparse ? ((2 * (size_t)p.T + 1) + (size_t)p.K * p.N + (size_t)p.T * std::max<size_t>(nblk, 1)) * 4
+                 : 0;
     return 256 + nchunks * (size_t)fs * (size_t)p.N * esz + perm + inv;
 }
 
