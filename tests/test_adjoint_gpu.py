@@ -224,11 +224,13 @@ def test_reuse_x_matches_a_fresh_copy(dev):
         assert torch.equal(a, c)
 
 
-@pytest.mark.parametrize("K,T", [(3, 5), (1, 4), (6, 2)])
-def test_sparse_adjoint_inverted_index(dev, K, T):
-    """The sparse grad_x visits each transmit's entries through an inverted index; indices
-    outside [0, T) select nothing, in the forward and in every gradient."""
-    w = W.small_random(seed=70 + K, B=5, C=4, T=T, K=K, S=110, N=(9, 8), sparse=True)
+@pytest.mark.parametrize("K,T,N", [(3, 5, (9, 8)), (1, 4, (9, 8)), (6, 2, (9, 8)),
+                                   (3, 300, (40, 60)), (2, 4100, (9, 8))])
+def test_sparse_adjoint_inverted_index(dev, K, T, N):
+    """The sparse grad_x visits each transmit's entries through an inverted index (block-
+    ordered for T <= 4096, several 1024-voxel blocks at 40 x 60; atomically filled above);
+    indices outside [0, T) select nothing, in the forward and in every gradient."""
+    w = W.small_random(seed=70 + K, B=5, C=4, T=T, K=K, S=110, N=N, sparse=True)
     bad = np.random.default_rng(K).random(w.sparse_indices.shape) < 0.15
     w.sparse_indices[bad] = np.where(np.random.default_rng(1).random(bad.sum()) < 0.5, -1, T)
     rng = np.random.default_rng(11)
