@@ -453,14 +453,14 @@ def main():
     dom = max((k for k in kt if k in bytes_per_u), key=kt.get)
     achieved = U * bytes_per_u[dom] / (kt[dom] * 1e-3) / 1e9
     # on-chip figure of the dominant kernel: grad_x is bound by 64-bit LDS atomics (per tap
-    # and frame chunk of FS frames: FS/2 frame pairs at each of the two samples, plus two
-    # 32-bit per-sample counts, i.e. 1 + 2/FS per unit),
+    # and frame chunk of FS frames: FS/2 frame pairs at each of the two samples, plus one
+    # 32-bit count, i.e. 1 + 1/FS per unit),
     # measured on gfx950 at 8.9 lane-ops/clk/CU without bank conflicts; the gathers by the
     # vector L1 (64 B/clk/CU), 2 x sizeof(storage) bytes per unit
     clk, cus = 2.4e9, 256
     if dom == "das_bwd_x":
         fs = bwdx_frames_used(a, tuned)
-        ops = U * (1.0 + 2.0 / fs)
+        ops = U * (1.0 + 1.0 / fs)
         onchip = {"resource": "LDS 64-bit atomics", "unit": "lane-ops/clk/CU",
                   "achieved": round(ops / (kt[dom] * 1e-3) / (clk * cus), 2), "peak": 8.9}
     else:

@@ -1073,10 +1073,9 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
             const int s0 = lds_slot(tp.k), s1 = lds_slot(tp.k + 1);
             U64 *l0 = lds + s0;
             U64 *l1 = lds + s1;
-            if (counting) {
-                atomicAdd(cnt + s0, 1u);
-                atomicAdd(cnt + s1, 1u);
-            }
+            // one count per tap, at its first sample: sample s then received
+            // cnt(s) + cnt(s - 1) contributions
+            if (counting) atomicAdd(cnt + s0, 1u);
             if constexpr (PACK) {
                 // FP32: weights scaled by 2^E, the FMA rounds each product to its integer
                 const float tf = tp.t;
@@ -1176,7 +1175,8 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
         if (!counting) break;
         // the row's largest contribution count decides whether pass 0's sums are valid
         unsigned cmax = 0;
-        for (int i = threadIdx.x; i < S32; i += blockDim.x) cmax = max(cmax, cnt[i]);
+        for (int i = threadIdx.x; i < g.S; i += blockDim.x)
+            cmax = max(cmax, cnt[lds_slot(i)] + (i > 0 ? cnt[lds_slot(i - 1)] : 0u));
         cmax = block_max_u32(cmax, red);
         if (cmax <= (1u << kLogCnt0)) break;
         int logc = kLogCnt0;
@@ -1188,6 +1188,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
     const double inv = ldexp(1.0, -E);
     for (int k = threadIdx.x; k < g.S; k += blockDim.x) {
         const U64 *wp = lds + lds_slot(k);
+        const unsigned nk = PACK ? cnt[lds_slot(k)] + (k > 0 ? cnt[lds_slot(k - 1)] : 0u) : 0u;
         A *dst = gx + ((int64_t)b0 * gx_b + (int64_t)m * gx_c + (int64_t)s * gx_t + k) * CW;
         if constexpr (PACK) {
 #pragma unroll
@@ -1195,7 +1196,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
 #pragma unroll
                 for (int c = 0; c < CW; c++) {
                     long long v0, v1;
-                    fx_unpack_pair(wp[(size_t)(p * CW + c) * S32], cnt[lds_slot(k)], v0, v1);
+                    fx_unpack_pair(wp[(size_t)(p * CW + c) * S32], nk, v0, v1);
                     if (2 * p < nf) dst[(int64_t)(2 * p) * gx_b * CW + c] = (A)((double)v0 * inv);
                     if (2 * p + 1 < nf) dst[(int64_t)(2 * p + 1) * gx_b * CW + c] = (A)((double)v1 * inv);
                 }
