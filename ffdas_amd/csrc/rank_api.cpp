@@ -75,8 +75,10 @@ ffdas_error_t check_info(ffdas_context &h, const int *info, ffdas_error_t fail) 
 }
 
 // Eigendecomposition of the column-major p x p Hermitian H in place (eigenvectors in the
-// columns, eigenvalues ascending), then a check of the solver's info on the host.
-ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E, int *info) {
+// columns, eigenvalues ascending), queued on the stream; *fail is the error to report if the
+// solver's status word (read by the caller once the call's work is done) is not zero.
+ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E, int *info,
+                   ffdas_error_t *fail) {
     // The one-workgroup Jacobi kernel (A and V in LDS) up to p = 64, rocSOLVER above:
     // measured on gfx950, at p = 64 Jacobi 0.74 ms vs rocSOLVER 1.19 ms for real FP32 and
     // 1.04 vs 1.30 ms for complex; its p^3 cost per sweep loses above (p = 96 complex
@@ -97,7 +99,8 @@ ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void 
                 hipStreamSynchronize(h.stream) == hipSuccess)
                 std::fprintf(stderr, "ffdas: Jacobi p=%d sweeps=%d info=%d\n", pl.p, hi[1], hi[0]);
         }
-        return check_info(h, info, FFDAS_ERROR_CONVERGENCE_FAILED);
+        *fail = FFDAS_ERROR_CONVERGENCE_FAILED;
+        return FFDAS_SUCCESS;
     }
     const Solver &s = solver();
     if (!s.ok) return FFDAS_ERROR_CUSOLVER;
@@ -122,7 +125,8 @@ ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void 
     else
         st = s.ssyevd(bh, ev, up, pl.p, (float *)H, pl.p, (float *)D, (float *)E, info);
     if (st != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
-    return check_info(h, info, FFDAS_ERROR_CUSOLVER);
+    *fail = FFDAS_ERROR_CUSOLVER;
+    return FFDAS_SUCCESS;
 }
 
 bool overlaps(const void *a, size_t na, const void *b, size_t nb) {
@@ -260,6 +264,10 @@ extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor
     pl.ovec = pl.osf == 1 && aligned(out, pl.osj);
 
     FFDAS_TRY(launch_rank_gram(st, pl, x, part, H));
-    FFDAS_TRY(eigh(*handle, pl, H, D, E, info));
-    return launch_rank_project(st, pl, H, xs, out, Qh, Qn);
+    // the projection is queued behind the eigensolver; its status word is read once at the
+    // end (a failed solve leaves out undefined and returns the solver's error)
+    ffdas_error_t fail = FFDAS_SUCCESS;
+    FFDAS_TRY(eigh(*handle, pl, H, D, E, info, &fail));
+    FFDAS_TRY(launch_rank_project(st, pl, H, xs, out, Qh, Qn));
+    return check_info(*handle, info, fail);
 }

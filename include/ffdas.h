@@ -309,9 +309,10 @@ FFDAS_API int ffdas_tune_count(void);
  * singular components [start, stop) of x counted from the largest, 0 <= start < stop <= p
  * (else FFDAS_ERROR_INVALID_ARGUMENT).  The eigendecomposition of the p x p Gram matrix runs
  * in one workgroup (Jacobi, FFDAS_ERROR_CONVERGENCE_FAILED if 30 sweeps do not converge)
- * for p <= 64, else on rocSOLVER
- * (FFDAS_ERROR_CUSOLVER if it is unavailable or fails); the call synchronises the handle's
- * stream once, after the eigendecomposition.
+ * for p <= 64, else on rocSOLVER (FFDAS_ERROR_CUSOLVER if it is unavailable or fails); on
+ * such a failure out is undefined.  Unlike every other entry point, this one synchronises
+ * the handle's stream, once, at its end, to read the solver's status (the reference's also
+ * synchronises).
  * Replaces include/ffdas.h:178-186. */
 FFDAS_API ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
                                             const void *x, int64_t start, int64_t stop,
