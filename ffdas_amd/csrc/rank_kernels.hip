@@ -511,10 +511,12 @@ struct Eps<double> {
     static constexpr double v = 2.220446049250313e-16;
 };
 
+// A and V are kept row-major with a row stride of pe + 1 elements, so a column of either
+// (the Hermitian mirror stores, the diagonal) falls on distinct LDS banks
 size_t jacobi_lds_bytes(int pe, size_t esz, size_t tsz) {
     const int npair = pe / 2;
     const size_t ntri = (size_t)npair * (npair + 1) / 2;
-    return 2 * (size_t)pe * pe * esz + (size_t)npair * (4 * tsz + 2 * sizeof(int)) +
+    return 2 * (size_t)pe * (pe + 1) * esz + (size_t)npair * (4 * tsz + 2 * sizeof(int)) +
            ntri * sizeof(unsigned short) + 64;
 }
 
@@ -568,11 +570,11 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
     using V4 = typename JVec<T>::V4;
     extern __shared__ __align__(16) unsigned char jlds[];
     constexpr int C = CPLX ? 2 : 1;
-    const int npair = pe / 2;
+    const int npair = pe / 2, ld = pe + 1;
     T *A = reinterpret_cast<T *>(jlds);
-    T *V = A + (size_t)pe * pe * C;
+    T *V = A + (size_t)pe * ld * C;
     // per pair: (c, s, cos(phi), sin(phi)) and (i, j), one vector access each
-    V4 *prm = reinterpret_cast<V4 *>(V + (size_t)pe * pe * C);
+    V4 *prm = reinterpret_cast<V4 *>(V + (size_t)pe * ld * C);
     int2 *pij = reinterpret_cast<int2 *>(prm + npair);
     // the blocks (P, Q), P <= Q, of a round: P << 8 | Q
     unsigned short *tri = reinterpret_cast<unsigned short *>(pij + npair);
@@ -580,6 +582,7 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
     __shared__ T red[16];
     __shared__ int flag[3];
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int vrows = nt >= npair ? nt / npair : 1;
 
     // load A (H is column-major: (r, c) at c * p + r), V = I, ||A||_F
     T sq = 0;
@@ -592,8 +595,8 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
             ai = CPLX ? H[2 * h + 1] : T(0);
         }
         sq += ar * ar + ai * ai;
-        jst<T, CPLX>(A, e, ar, ai);
-        jst<T, CPLX>(V, e, r == c ? T(1) : T(0), T(0));
+        jst<T, CPLX>(A, r * ld + c, ar, ai);
+        jst<T, CPLX>(V, r * ld + c, r == c ? T(1) : T(0), T(0));
     }
     for (int w = tid; w < ntri; w += nt) {
         int P = 0, idx = w;
@@ -629,9 +632,9 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                 T c = 1, s = 0, er = 1, ei = 0;
                 if (j < p) {
                     T aii, ajj, zr, zi, dummy;
-                    jld<T, CPLX>(A, i * pe + i, aii, dummy);
-                    jld<T, CPLX>(A, j * pe + j, ajj, dummy);
-                    jld<T, CPLX>(A, i * pe + j, zr, zi);
+                    jld<T, CPLX>(A, i * ld + i, aii, dummy);
+                    jld<T, CPLX>(A, j * ld + j, ajj, dummy);
+                    jld<T, CPLX>(A, i * ld + j, zr, zi);
                     // complex: rotate by |a_ij| after the phase factor; real: the signed a_ij
                     const T r = CPLX ? sqrt(zr * zr + zi * zi) : zr;
                     if (fabs(r) > thresh) {
@@ -664,8 +667,8 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                 if (sP == 0 && sQ == 0) continue;
                 const int2 ijP = pij[P], ijQ = pij[Q];
                 const int iP = ijP.x, jP = ijP.y, iQ = ijQ.x, jQ = ijQ.y;
-                const int e00 = iP * pe + iQ, e01 = iP * pe + jQ, e10 = jP * pe + iQ,
-                          e11 = jP * pe + jQ;
+                const int e00 = iP * ld + iQ, e01 = iP * ld + jQ, e10 = jP * ld + iQ,
+                          e11 = jP * ld + jQ;
                 T b00r, b01r, b10r, b11r, b00i, b01i, b10i, b11i;
                 jld<T, CPLX>(A, e00, b00r, b00i);
                 jld<T, CPLX>(A, e01, b01r, b01i);
@@ -709,31 +712,33 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
                 jst<T, CPLX>(A, e10, o10r, o10i);
                 jst<T, CPLX>(A, e11, o11r, o11i);
                 if (P != Q) {   // Hermitian mirror: (Q, P) block = conj transpose
-                    jst<T, CPLX>(A, iQ * pe + iP, o00r, -o00i);
-                    jst<T, CPLX>(A, jQ * pe + iP, o01r, -o01i);
-                    jst<T, CPLX>(A, iQ * pe + jP, o10r, -o10i);
-                    jst<T, CPLX>(A, jQ * pe + jP, o11r, -o11i);
+                    jst<T, CPLX>(A, iQ * ld + iP, o00r, -o00i);
+                    jst<T, CPLX>(A, jQ * ld + iP, o01r, -o01i);
+                    jst<T, CPLX>(A, iQ * ld + jP, o10r, -o10i);
+                    jst<T, CPLX>(A, jQ * ld + jP, o11r, -o11i);
                 }
             }
-            // V' = V G for every pair and row
-            for (int w = tid; w < pe * npair; w += nt) {
-                const int row = w / npair, P = w % npair;
+            // V' = V G for every pair and row: thread tid < npair * vrows keeps pair
+            // tid % npair, loads its parameters once and updates rows tid / npair + k * vrows
+            if (tid < npair * vrows) {
+                const int P = tid % npair;
                 const V4 qP = prm[P];
                 const T cP = qP.x, sP = qP.y;
-                if (sP == 0) continue;
                 const int2 ijP = pij[P];
-                const int ei0 = row * pe + ijP.x, ej0 = row * pe + ijP.y;
-                T vir, vii, vjr, vji;
-                jld<T, CPLX>(V, ei0, vir, vii);
-                jld<T, CPLX>(V, ej0, vjr, vji);
-                if constexpr (CPLX) {
-                    const T qr = qP.z, qi = -qP.w;   // times e^{-i phi}
-                    const T tr = vjr * qr - vji * qi;
-                    vji = vjr * qi + vji * qr;
-                    vjr = tr;
+                for (int row = tid / npair; sP != 0 && row < pe; row += vrows) {
+                    const int ei0 = row * ld + ijP.x, ej0 = row * ld + ijP.y;
+                    T vir, vii, vjr, vji;
+                    jld<T, CPLX>(V, ei0, vir, vii);
+                    jld<T, CPLX>(V, ej0, vjr, vji);
+                    if constexpr (CPLX) {
+                        const T qr = qP.z, qi = -qP.w;   // times e^{-i phi}
+                        const T tr = vjr * qr - vji * qi;
+                        vji = vjr * qi + vji * qr;
+                        vjr = tr;
+                    }
+                    jst<T, CPLX>(V, ei0, cP * vir - sP * vjr, cP * vii - sP * vji);
+                    jst<T, CPLX>(V, ej0, sP * vir + cP * vjr, sP * vii + cP * vji);
                 }
-                jst<T, CPLX>(V, ei0, cP * vir - sP * vjr, cP * vii - sP * vji);
-                jst<T, CPLX>(V, ej0, sP * vir + cP * vjr, sP * vii + cP * vji);
             }
             __syncthreads();
         }
@@ -744,15 +749,15 @@ __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, 
     for (int w = tid; w < p * p; w += nt) {
         const int row = w % p, k = w / p;
         T dk, dummy;
-        jld<T, CPLX>(A, k * pe + k, dk, dummy);
+        jld<T, CPLX>(A, k * ld + k, dk, dummy);
         int rank = 0;
         for (int l = 0; l < p; l++) {
             T dl;
-            jld<T, CPLX>(A, l * pe + l, dl, dummy);
+            jld<T, CPLX>(A, l * ld + l, dl, dummy);
             rank += (dl < dk || (dl == dk && l < k)) ? 1 : 0;
         }
         T vr, vi;
-        jld<T, CPLX>(V, row * pe + k, vr, vi);
+        jld<T, CPLX>(V, row * ld + k, vr, vi);
         const int64_t h = (int64_t)rank * p + row;
         H[C * h] = vr;
         if constexpr (CPLX) H[C * h + 1] = vi;
