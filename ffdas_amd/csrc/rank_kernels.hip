@@ -490,16 +490,15 @@ __global__ void rank_select_kernel(const T *__restrict__ V, int p, int pp, int r
 // ---------------------------------------------------------------------------------------
 // Eigendecomposition of a small Hermitian matrix: parallel cyclic Jacobi in one workgroup
 // ---------------------------------------------------------------------------------------
-// A and V (pe x pe, pe = p rounded up to even, row-major, complex as pairs) live in LDS.
-// Each round pairs the indices by the round-robin tournament (pe - 1 rounds visit every pair
-// once); the pe/2 rotations of a round are disjoint, so every 2x2 block (P, Q) of A is
-// updated independently as G_P^H B G_Q, and V's rows as V G.  G = diag(1, e^{-i phi}) J with
-// a_ij = r e^{i phi} (real data: G = J with r = a_ij, signed) and J = [[c, s], [-s, c]] the
-// symmetric Schur rotation of [[a_ii, r], [r, a_jj]] (tau = (a_jj - a_ii) / 2r,
-// t = sgn(tau) / (|tau| + sqrt(1 + tau^2))).
-// A rotation is skipped when |a_ij| <= eps * ||A||_F; a sweep without rotations ends the
-// iteration.  Output: eigenvectors in the columns of H (column-major), ascending eigenvalues,
-// as the LAPACK-style solver returns them; info = 1 if max_sweeps did not converge.
+// A (pe x pe, pe = p rounded up to even, upper triangle, complex as pairs) lives in LDS, V in
+// registers.  Each round pairs the indices by the round-robin tournament (pe - 1 rounds visit
+// every pair once); the pe/2 rotations of a round are disjoint, so every 2x2 block (P, Q) of
+// A is updated independently as G_P^H B G_Q, and V's rows as V G, with G the unitary Schur
+// rotation of the pair (its form is given at the kernel).  A rotation is skipped when
+// |a_ij| <= eps * ||A||_F, and the iteration ends after the sweep that leaves no larger
+// off-diagonal element.  Output: eigenvectors in the columns of H (column-major), ascending
+// eigenvalues, as the LAPACK-style solver returns them; info = 1 if max_sweeps did not
+// converge.
 template <typename T>
 struct Eps;
 template <>
@@ -516,16 +515,32 @@ struct Eps<double> {
 size_t jacobi_lds_bytes(int pe, size_t esz, size_t tsz) {
     const int npair = pe / 2;
     const size_t ntri = (size_t)npair * (npair + 1) / 2;
-    return 2 * (size_t)pe * (pe + 1) * esz + (size_t)npair * (4 * tsz + 2 * sizeof(int)) +
+    return 2 * (size_t)pe * (pe + 1) * esz + (size_t)npair * 4 * tsz +
            ntri * sizeof(unsigned short) + 64;
 }
 
-// rotation arithmetic: hardware reciprocal / reciprocal square root (about 1 ulp) in FP32,
-// IEEE operations in FP64
+// rotation arithmetic: hardware reciprocal, square root and reciprocal square root (about
+// 1 ulp) in FP32, IEEE operations in FP64
 __device__ __forceinline__ float jdiv(float a, float b) { return __fdividef(a, b); }
 __device__ __forceinline__ double jdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float jrsqrt(float a) { return rsqrtf(a); }
 __device__ __forceinline__ double jrsqrt(double a) { return 1.0 / sqrt(a); }
+__device__ __forceinline__ float jsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double jsqrt(double a) { return sqrt(a); }
+
+// the pair P of round `round` of the round-robin tournament over pe (even) indices: slot 0
+// holds index 0, slots 1 .. pe-1 hold 1 + (slot - 1 + round) mod (pe - 1); pair P takes
+// slots P and pe-1-P; returned as i < j
+__device__ __forceinline__ int2 jslots(int P, int round, int pe) {
+    int a = P - 1 + round, b = pe - 2 - P + round;   // both below 2 (pe - 1)
+    if (a >= pe - 1) a -= pe - 1;
+    if (b >= pe - 1) b -= pe - 1;
+    return int2{P == 0 ? 0 : 1 + a, 1 + b};   // indices in slots P and pe-1-P
+}
+__device__ __forceinline__ int2 jpair(int P, int round, int pe) {
+    const int2 s = jslots(P, round, pe);
+    return s.x < s.y ? s : int2{s.y, s.x};
+}
 
 template <typename T>
 struct JVec;
@@ -564,186 +579,289 @@ __device__ __forceinline__ void jst(T *M, int e, T re, T im) {
     }
 }
 
+// Upper-triangle storage: element (r, c) of the Hermitian A lives at U[min][max], conjugated
+// when r > c; the diagonal block of a pair is written as a real diagonal and an exact zero.
 template <typename T, bool CPLX>
+__device__ __forceinline__ void jget(const T *A, int ld, int r, int c, T &re, T &im) {
+    const bool up = r <= c;   // one access either way: address and sign are selected
+    jld<T, CPLX>(A, up ? r * ld + c : c * ld + r, re, im);
+    if constexpr (CPLX) im = up ? im : -im;
+}
+template <typename T, bool CPLX>
+__device__ __forceinline__ void jput(T *A, int ld, int r, int c, T re, T im) {
+    const bool up = r < c;
+    jst<T, CPLX>(A, up ? r * ld + c : c * ld + r, re, up ? im : -im);
+}
+
+// One-lane moves across the whole wave on the VALU (DPP): jnext gives lane l the value of
+// lane l + 1 (wave_shl:1), jprev that of lane l - 1 (wave_shr:1); the lane without a source
+// keeps its own value.
+__device__ __forceinline__ int jdpp_i(int v, bool next) {
+    return next ? __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false)
+                : __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float jdpp(float v, bool next) {
+    return __int_as_float(jdpp_i(__float_as_int(v), next));
+}
+__device__ __forceinline__ double jdpp(double v, bool next) {
+    const long long b = __double_as_longlong(v);
+    const int lo = jdpp_i((int)(unsigned)b, next), hi = jdpp_i((int)(b >> 32), next);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// V rows per thread at 1024 threads: 2 with 32-lane row slots (pe <= 64); with 64-lane slots
+// (16 row slots) the most pe whose A and V fit in 160 KB of LDS allows: 138 (4-byte
+// elements, 9 rows), 98 (8-byte, 7 rows), 70 (16-byte, 5 rows)
+constexpr int jacobi_vrows(int sw, size_t esz) {
+    return sw == 32 ? 2 : (esz >= 16 ? 5 : (esz >= 8 ? 7 : 9));
+}
+
+// SW: lanes per V row slot (32 for pe <= 64, 64 above); 1024 threads.
+// Each round uses G = [[c, sigma], [-conj(sigma), c]] with sigma = s e^{i phi}, a_ij = r e^{i phi}:
+// G^H [[a_ii, a_ij], [conj(a_ij), a_jj]] G is diagonal for t = s / c = 2 r sgn(d) / (|d| + h),
+// d = a_jj - a_ii, h = sqrt(d^2 + 4 r^2), so c = 1 / sqrt(1 + 4 r^2 / (|d| + h)^2) and
+// sigma = 2 sgn(d) c a_ij / (|d| + h): no square root of |a_ij| and no phase division.
+// (Real data: the same with sigma = s, a_ij signed.)
+template <typename T, bool CPLX, int SW>
 __global__ __launch_bounds__(1024) void rank_jacobi_kernel(T *H, int p, int pe, int max_sweeps,
                                                            int *info) {
     using V4 = typename JVec<T>::V4;
+    constexpr int VR = jacobi_vrows(SW, sizeof(T) * (CPLX ? 2 : 1));   // >= ceil(pe / (1024 / SW))
     extern __shared__ __align__(16) unsigned char jlds[];
     constexpr int C = CPLX ? 2 : 1;
     const int npair = pe / 2, ld = pe + 1;
     T *A = reinterpret_cast<T *>(jlds);
     T *V = A + (size_t)pe * ld * C;
-    // per pair: (c, s, cos(phi), sin(phi)) and (i, j), one vector access each
+    // per pair: (c, Re sigma, Im sigma, 0), one vector access
     V4 *prm = reinterpret_cast<V4 *>(V + (size_t)pe * ld * C);
-    int2 *pij = reinterpret_cast<int2 *>(prm + npair);
     // the blocks (P, Q), P <= Q, of a round: P << 8 | Q
-    unsigned short *tri = reinterpret_cast<unsigned short *>(pij + npair);
+    unsigned short *tri = reinterpret_cast<unsigned short *>(prm + npair);
     const int ntri = npair * (npair + 1) / 2;
-    __shared__ T red[16];
-    __shared__ int flag[3];
+    __shared__ double red[16];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int vrows = nt >= npair ? nt / npair : 1;
+    // V stays in registers: lane k = tid % SW of row slot rs = tid / SW holds, for rows
+    // rs + v * nrs, the elements of its pair's two tournament slots, a = slot k and
+    // b = slot pe-1-k; after each round the values move with the slots' indices (slot m
+    // takes slot m+1's index, slot 0 keeps index 0), which is a one-lane shift.
+    const int k = tid % SW, rs = tid / SW, nrs = nt / SW;
+    const bool vown = k < npair;
 
-    // load A (H is column-major: (r, c) at c * p + r), V = I, ||A||_F
-    T sq = 0;
+    // ||H||_F in FP64; A = H 2^-e with ||A||_F near 1.  Power-of-two scaling is exact and the
+    // rotations do not depend on scale, so d^2 + 4 r^2 stays far from overflow
+    double sq = 0;
+    for (int e = tid; e < p * p; e += nt) {
+        const double ar = CPLX ? H[2 * e] : H[e], ai = CPLX ? H[2 * e + 1] : T(0);
+        sq += ar * ar + ai * ai;
+    }
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((tid & 63) == 0) red[tid >> 6] = sq;
+    __syncthreads();
+    double norm2 = 0;
+    for (int w = 0; w < nt / 64; w++) norm2 += red[w];
+    int ex = 0;
+    if (norm2 > 0) frexp(sqrt(norm2), &ex);
+    ex = max(-126, min(126, ex));
+    const T sc = (T)ldexp(1.0, -ex);
+    const T thresh = (T)(Eps<T>::v * sqrt(norm2) * ldexp(1.0, -ex));
+    const T thresh2 = thresh * thresh;
+
+    // load A (H is column-major: (r, c) at c * p + r)
     for (int e = tid; e < pe * pe; e += nt) {
         const int r = e / pe, c = e % pe;
         T ar = 0, ai = 0;
         if (r < p && c < p) {
             const int64_t h = (int64_t)c * p + r;
-            ar = CPLX ? H[2 * h] : H[h];
-            ai = CPLX ? H[2 * h + 1] : T(0);
+            ar = (CPLX ? H[2 * h] : H[h]) * sc;
+            ai = CPLX ? H[2 * h + 1] * sc : T(0);
         }
-        sq += ar * ar + ai * ai;
         jst<T, CPLX>(A, r * ld + c, ar, ai);
-        jst<T, CPLX>(V, r * ld + c, r == c ? T(1) : T(0), T(0));
     }
+    int myPQ = 0;   // this thread's first block, kept in a register (the only one if nt >= ntri)
     for (int w = tid; w < ntri; w += nt) {
         int P = 0, idx = w;
         while (idx >= npair - P) {
             idx -= npair - P;
             P++;
         }
-        tri[w] = (unsigned short)((P << 8) | (P + idx));
+        const int PQ = (P << 8) | (P + idx);
+        tri[w] = (unsigned short)PQ;
+        if (w == tid) myPQ = PQ;
     }
-    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
-    if ((tid & 63) == 0) red[tid >> 6] = sq;
-    if (tid < 3) flag[tid] = 0;
+    // V = I: slot k holds index k, slot pe-1-k index pe-1-k at round 0
+    T var[VR], vai[VR], vbr[VR], vbi[VR];
+#pragma unroll
+    for (int v = 0; v < VR; v++) {
+        const int row = rs + v * nrs;
+        var[v] = row == k ? T(1) : T(0);
+        vbr[v] = row == pe - 1 - k ? T(1) : T(0);
+        vai[v] = vbi[v] = T(0);
+    }
     __syncthreads();
-    T norm2 = 0;
-    for (int w = 0; w < (nt + 63) / 64; w++) norm2 += red[w];
-    const T thresh = Eps<T>::v * sqrt(norm2);
 
+    // A sweep rotates the pairs whose |a_ij|^2 exceeds thresh^2.  The iteration ends once no
+    // off-diagonal element does (checked after each sweep: the next sweep would rotate
+    // nothing, so skipping it changes no result).
+    auto converged = [&]() {
+        int any = 0;
+        for (int e = tid; e < p * p; e += nt) {
+            const int r = e / p, c = e % p;
+            if (r < c) {
+                T zr, zi;
+                jld<T, CPLX>(A, r * ld + c, zr, zi);
+                any |= (CPLX ? zr * zr + zi * zi : zr * zr) > thresh2;
+            }
+        }
+        return __syncthreads_or(any) == 0;
+    };
     int sweep = 0;
-    bool done = false;
+    bool done = converged();
     for (; sweep < max_sweeps && !done; sweep++) {
-        if (tid == 0) flag[(sweep + 1) % 3] = 0;
         for (int round = 0; round < pe - 1; round++) {
             // rotation parameters of this round's pairs
             if (tid < npair) {
-                const int m0 = tid, m1 = pe - 1 - tid;
-                int i = m0 == 0 ? 0 : 1 + (m0 - 1 + round) % (pe - 1);
-                int j = 1 + (m1 - 1 + round) % (pe - 1);
-                if (i > j) {
-                    const int t = i;
-                    i = j;
-                    j = t;
-                }
-                T c = 1, s = 0, er = 1, ei = 0;
+                const int2 ij = jpair(tid, round, pe);
+                const int i = ij.x, j = ij.y;
+                T c = 1, sr = 0, si = 0;
                 if (j < p) {
                     T aii, ajj, zr, zi, dummy;
                     jld<T, CPLX>(A, i * ld + i, aii, dummy);
                     jld<T, CPLX>(A, j * ld + j, ajj, dummy);
                     jld<T, CPLX>(A, i * ld + j, zr, zi);
-                    // complex: rotate by |a_ij| after the phase factor; real: the signed a_ij
-                    const T r = CPLX ? sqrt(zr * zr + zi * zi) : zr;
-                    if (fabs(r) > thresh) {
-                        const T tau = jdiv(ajj - aii, 2 * r);
-                        const T t = jdiv(tau >= 0 ? T(1) : T(-1), fabs(tau) + sqrt(1 + tau * tau));
-                        c = jrsqrt(1 + t * t);
-                        s = t * c;
-                        if constexpr (CPLX) {
-                            const T ir = jdiv(T(1), r);
-                            er = zr * ir;
-                            ei = zi * ir;
-                        }
-                        flag[sweep % 3] = 1;
+                    const T r2 = CPLX ? zr * zr + zi * zi : zr * zr;
+                    if (r2 > thresh2) {
+                        const T d = ajj - aii;
+                        const T inv = jdiv(T(1), fabs(d) + jsqrt(d * d + 4 * r2));
+                        c = jrsqrt(1 + 4 * r2 * (inv * inv));
+                        const T f = (d >= 0 ? T(2) : T(-2)) * c * inv;
+                        sr = f * zr;
+                        si = CPLX ? f * zi : T(0);
                     }
                 }
                 V4 q;
                 q.x = c;
-                q.y = s;
-                q.z = er;
-                q.w = ei;
+                q.y = sr;
+                q.z = si;
+                q.w = 0;
                 prm[tid] = q;
-                pij[tid] = int2{i, j};
             }
             __syncthreads();
-            // blocks (P, Q), P <= Q: B' = J_P^T D_P^H B D_Q J_Q, mirrored into (Q, P)
+            // blocks (P, Q), P <= Q: B' = G_P^H B G_Q in upper-triangle storage
             for (int w = tid; w < ntri; w += nt) {
-                const int PQ = tri[w], P = PQ >> 8, Q = PQ & 255;
-                const V4 qP = prm[P], qQ = prm[Q];
-                const T cP = qP.x, sP = qP.y, cQ = qQ.x, sQ = qQ.y;
-                if (sP == 0 && sQ == 0) continue;
-                const int2 ijP = pij[P], ijQ = pij[Q];
+                const int PQ = w == tid ? myPQ : (int)tri[w], P = PQ >> 8, Q = PQ & 255;
+                const int2 ijP = jpair(P, round, pe), ijQ = jpair(Q, round, pe);
                 const int iP = ijP.x, jP = ijP.y, iQ = ijQ.x, jQ = ijQ.y;
-                const int e00 = iP * ld + iQ, e01 = iP * ld + jQ, e10 = jP * ld + iQ,
-                          e11 = jP * ld + jQ;
+                const V4 qP = prm[P], qQ = prm[Q];
                 T b00r, b01r, b10r, b11r, b00i, b01i, b10i, b11i;
-                jld<T, CPLX>(A, e00, b00r, b00i);
-                jld<T, CPLX>(A, e01, b01r, b01i);
-                jld<T, CPLX>(A, e10, b10r, b10i);
-                jld<T, CPLX>(A, e11, b11r, b11i);
+                jget<T, CPLX>(A, ld, iP, iQ, b00r, b00i);
+                jget<T, CPLX>(A, ld, iP, jQ, b01r, b01i);
+                jget<T, CPLX>(A, ld, jP, iQ, b10r, b10i);
+                jget<T, CPLX>(A, ld, jP, jQ, b11r, b11i);
+                const T cP = qP.x, sPr = qP.y, sPi = qP.z, cQ = qQ.x, sQr = qQ.y, sQi = qQ.z;
+                if (sPr == 0 && sPi == 0 && sQr == 0 && sQi == 0) continue;
+                // N = B G_Q: n_x0 = cQ b_x0 - conj(sQ) b_x1, n_x1 = sQ b_x0 + cQ b_x1
+                T n00r, n00i, n01r, n01i, n10r, n10i, n11r, n11i;
                 if constexpr (CPLX) {
-                    // M = D_P^H B D_Q: column 1 times e^{-i phiQ}, row 1 times e^{i phiP}
-                    const T qr = qQ.z, qi = -qQ.w, pR = qP.z, pI = qP.w;
-                    T tr = b01r * qr - b01i * qi, ti = b01r * qi + b01i * qr;
-                    b01r = tr;
-                    b01i = ti;
-                    tr = b11r * qr - b11i * qi;
-                    ti = b11r * qi + b11i * qr;
-                    b11r = tr;
-                    b11i = ti;
-                    tr = pR * b10r - pI * b10i;
-                    ti = pR * b10i + pI * b10r;
-                    b10r = tr;
-                    b10i = ti;
-                    tr = pR * b11r - pI * b11i;
-                    ti = pR * b11i + pI * b11r;
-                    b11r = tr;
-                    b11i = ti;
+                    n00r = cQ * b00r - (sQr * b01r + sQi * b01i);
+                    n00i = cQ * b00i - (sQr * b01i - sQi * b01r);
+                    n01r = (sQr * b00r - sQi * b00i) + cQ * b01r;
+                    n01i = (sQr * b00i + sQi * b00r) + cQ * b01i;
+                    n10r = cQ * b10r - (sQr * b11r + sQi * b11i);
+                    n10i = cQ * b10i - (sQr * b11i - sQi * b11r);
+                    n11r = (sQr * b10r - sQi * b10i) + cQ * b11r;
+                    n11i = (sQr * b10i + sQi * b10r) + cQ * b11i;
+                } else {
+                    n00r = cQ * b00r - sQr * b01r;
+                    n01r = sQr * b00r + cQ * b01r;
+                    n10r = cQ * b10r - sQr * b11r;
+                    n11r = sQr * b10r + cQ * b11r;
+                    n00i = n01i = n10i = n11i = 0;
                 }
-                // N = M J_Q
-                const T n00r = cQ * b00r - sQ * b01r, n01r = sQ * b00r + cQ * b01r;
-                const T n10r = cQ * b10r - sQ * b11r, n11r = sQ * b10r + cQ * b11r;
-                const T n00i = cQ * b00i - sQ * b01i, n01i = sQ * b00i + cQ * b01i;
-                const T n10i = cQ * b10i - sQ * b11i, n11i = sQ * b10i + cQ * b11i;
-                // B' = J_P^T N
-                T o00r = cP * n00r - sP * n10r, o01r = cP * n01r - sP * n11r;
-                T o10r = sP * n00r + cP * n10r, o11r = sP * n01r + cP * n11r;
-                T o00i = cP * n00i - sP * n10i, o01i = cP * n01i - sP * n11i;
-                T o10i = sP * n00i + cP * n10i, o11i = sP * n01i + cP * n11i;
+                // B' = G_P^H N: o_0y = cP n_0y - sP n_1y, o_1y = conj(sP) n_0y + cP n_1y
+                T o00r, o00i, o01r, o01i, o10r, o10i, o11r, o11i;
+                if constexpr (CPLX) {
+                    o00r = cP * n00r - (sPr * n10r - sPi * n10i);
+                    o00i = cP * n00i - (sPr * n10i + sPi * n10r);
+                    o01r = cP * n01r - (sPr * n11r - sPi * n11i);
+                    o01i = cP * n01i - (sPr * n11i + sPi * n11r);
+                    o10r = (sPr * n00r + sPi * n00i) + cP * n10r;
+                    o10i = (sPr * n00i - sPi * n00r) + cP * n10i;
+                    o11r = (sPr * n01r + sPi * n01i) + cP * n11r;
+                    o11i = (sPr * n01i - sPi * n01r) + cP * n11i;
+                } else {
+                    o00r = cP * n00r - sPr * n10r;
+                    o01r = cP * n01r - sPr * n11r;
+                    o10r = sPr * n00r + cP * n10r;
+                    o11r = sPr * n01r + cP * n11r;
+                    o00i = o01i = o10i = o11i = 0;
+                }
                 if (P == Q) {   // the rotated pair itself: real diagonal, exact zero off it
-                    o01r = o01i = o10r = o10i = 0;
-                    o00i = o11i = 0;
-                }
-                jst<T, CPLX>(A, e00, o00r, o00i);
-                jst<T, CPLX>(A, e01, o01r, o01i);
-                jst<T, CPLX>(A, e10, o10r, o10i);
-                jst<T, CPLX>(A, e11, o11r, o11i);
-                if (P != Q) {   // Hermitian mirror: (Q, P) block = conj transpose
-                    jst<T, CPLX>(A, iQ * ld + iP, o00r, -o00i);
-                    jst<T, CPLX>(A, jQ * ld + iP, o01r, -o01i);
-                    jst<T, CPLX>(A, iQ * ld + jP, o10r, -o10i);
-                    jst<T, CPLX>(A, jQ * ld + jP, o11r, -o11i);
+                    jst<T, CPLX>(A, iP * ld + iP, o00r, T(0));
+                    jst<T, CPLX>(A, jP * ld + jP, o11r, T(0));
+                    jst<T, CPLX>(A, iP * ld + jP, T(0), T(0));
+                } else {
+                    jput<T, CPLX>(A, ld, iP, iQ, o00r, o00i);
+                    jput<T, CPLX>(A, ld, iP, jQ, o01r, o01i);
+                    jput<T, CPLX>(A, ld, jP, iQ, o10r, o10i);
+                    jput<T, CPLX>(A, ld, jP, jQ, o11r, o11i);
                 }
             }
-            // V' = V G for every pair and row: thread tid < npair * vrows keeps pair
-            // tid % npair, loads its parameters once and updates rows tid / npair + k * vrows
-            if (tid < npair * vrows) {
-                const int P = tid % npair;
-                const V4 qP = prm[P];
-                const T cP = qP.x, sP = qP.y;
-                const int2 ijP = pij[P];
-                for (int row = tid / npair; sP != 0 && row < pe; row += vrows) {
-                    const int ei0 = row * ld + ijP.x, ej0 = row * ld + ijP.y;
-                    T vir, vii, vjr, vji;
-                    jld<T, CPLX>(V, ei0, vir, vii);
-                    jld<T, CPLX>(V, ej0, vjr, vji);
-                    if constexpr (CPLX) {
-                        const T qr = qP.z, qi = -qP.w;   // times e^{-i phi}
-                        const T tr = vjr * qr - vji * qi;
-                        vji = vjr * qi + vji * qr;
-                        vjr = tr;
+            // V' = V G on the registers: v_i' = c v_i - conj(sigma) v_j, v_j' = sigma v_i + c v_j.
+            // With (a, b) the lane's two slots: a' = c a + s1 b, b' = c b + s2 a, where
+            // (s1, s2) = (-conj(sigma), sigma) if a holds i, else (sigma, -conj(sigma)).
+            if (vown) {
+                const V4 q = prm[k];
+                if (q.y != 0 || q.z != 0) {
+                    const int2 sl = jslots(k, round, pe);
+                    const T c = q.x, si = q.z;
+                    const T s1r = sl.x < sl.y ? -q.y : q.y, s2r = -s1r;
+#pragma unroll
+                    for (int v = 0; v < VR; v++) {
+                        const T ar = var[v], ai = vai[v], br = vbr[v], bi = vbi[v];
+                        if constexpr (CPLX) {
+                            var[v] = c * ar + (s1r * br - si * bi);
+                            vai[v] = c * ai + (s1r * bi + si * br);
+                            vbr[v] = c * br + (s2r * ar - si * ai);
+                            vbi[v] = c * bi + (s2r * ai + si * ar);
+                        } else {
+                            var[v] = c * ar + s1r * br;
+                            vbr[v] = c * br + s2r * ar;
+                        }
                     }
-                    jst<T, CPLX>(V, ei0, cP * vir - sP * vjr, cP * vii - sP * vji);
-                    jst<T, CPLX>(V, ej0, sP * vir + cP * vjr, sP * vii + cP * vji);
                 }
+            }
+            // move to the next round's slots: a <- slot k+1 (lane k+1's a; the last pair its
+            // own b), b <- slot pe-k (lane k-1's b; pair 0 slot 1, lane 1's a or, for pe = 2,
+            // its own b).  Every lane takes part in the moves (all lanes are active here).
+#pragma unroll
+            for (int v = 0; v < VR; v++) {
+                const T dar = jdpp(var[v], true), ubr = jdpp(vbr[v], false);
+                const T dai = CPLX ? jdpp(vai[v], true) : T(0);
+                const T ubi = CPLX ? jdpp(vbi[v], false) : T(0);
+                const T nar = k == 0 ? var[v] : (k == npair - 1 ? vbr[v] : dar);
+                const T nai = k == 0 ? vai[v] : (k == npair - 1 ? vbi[v] : dai);
+                const T nbr = k == 0 ? (npair > 1 ? dar : vbr[v]) : ubr;
+                const T nbi = k == 0 ? (npair > 1 ? dai : vbi[v]) : ubi;
+                var[v] = nar;
+                vai[v] = nai;
+                vbr[v] = nbr;
+                vbi[v] = nbi;
             }
             __syncthreads();
         }
-        done = flag[sweep % 3] == 0;
+        done = converged();
     }
+    // every sweep ends where it began: slot k holds index k, slot pe-1-k index pe-1-k
+    if (vown) {
+#pragma unroll
+        for (int v = 0; v < VR; v++) {
+            const int row = rs + v * nrs;
+            if (row < pe) {
+                jst<T, CPLX>(V, row * ld + k, var[v], vai[v]);
+                jst<T, CPLX>(V, row * ld + pe - 1 - k, vbr[v], vbi[v]);
+            }
+        }
+    }
+    __syncthreads();
 
     // ascending order of the eigenvalues; ties by index
     for (int w = tid; w < p * p; w += nt) {
@@ -1141,8 +1259,11 @@ ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x
 bool rank_jacobi_fits(const RankPlan &pl, int lds_bytes) {
     const int pe = (pl.p + 1) / 2 * 2;
     const size_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
-    // the kernel's static LDS (reduction slots, flags) stays well under 512 bytes
-    return pl.p >= 2 && jacobi_lds_bytes(pe, esz, pl.dbl ? 8 : 4) + 512 <= (size_t)lds_bytes;
+    // the kernel's static LDS (reduction slots, flags) stays well under 512 bytes; its V
+    // registers hold ceil(pe / row slots) rows per thread
+    const int slots = pe <= 64 ? 32 : 16;
+    return pl.p >= 2 && (pe + slots - 1) / slots <= jacobi_vrows(pe <= 64 ? 32 : 64, esz) &&
+           jacobi_lds_bytes(pe, esz, pl.dbl ? 8 : 4) + 512 <= (size_t)lds_bytes;
 }
 
 ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, int *info,
@@ -1150,18 +1271,20 @@ ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, in
     const int pe = (pl.p + 1) / 2 * 2;
     const size_t esz = (pl.dbl ? 8 : 4) * (pl.cplx ? 2 : 1);
     const size_t lds = jacobi_lds_bytes(pe, esz, pl.dbl ? 8 : 4);
-    int threads = 1024;
-    if (const char *e = std::getenv("FFDAS_RANK_JACOBI_THREADS")) {
-        const int v = std::atoi(e);
-        if (v == 256 || v == 512 || v == 1024) threads = v;
-    }
     ProfScope ps(st, "rank_jacobi");
 #define FFDAS_RANK_JACOBI(T, C)                                                               \
     do {                                                                                     \
-        (void)hipFuncSetAttribute((const void *)rank_jacobi_kernel<T, C>,                    \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
-        hipLaunchKernelGGL((rank_jacobi_kernel<T, C>), dim3(1), dim3(threads), lds, st, (T *)H, \
-                           pl.p, pe, max_sweeps, info);                                      \
+        if (pe <= 64) {                                                                      \
+            (void)hipFuncSetAttribute((const void *)rank_jacobi_kernel<T, C, 32>,            \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+            hipLaunchKernelGGL((rank_jacobi_kernel<T, C, 32>), dim3(1), dim3(1024), lds, st,   \
+                               (T *)H, pl.p, pe, max_sweeps, info);                          \
+        } else {                                                                             \
+            (void)hipFuncSetAttribute((const void *)rank_jacobi_kernel<T, C, 64>,            \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+            hipLaunchKernelGGL((rank_jacobi_kernel<T, C, 64>), dim3(1), dim3(1024), lds, st,   \
+                               (T *)H, pl.p, pe, max_sweeps, info);                          \
+        }                                                                                    \
     } while (0)
     if (pl.dbl && pl.cplx)
         FFDAS_RANK_JACOBI(double, true);
