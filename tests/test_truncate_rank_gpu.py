@@ -279,3 +279,66 @@ def test_out_aliasing_x_in_another_layout(dev):
                                      out_view.data_ptr()) == 0
         torch.cuda.synchronize()
         assert float((out_view - want).norm() / want.norm()) < 1e-5, (start, stop)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "complex64"])
+def test_power_of_two_scaling_is_exact(dev, monkeypatch, dtype):
+    """Scaling x by 2^k scales every product and sum of the Gram matrix, the Jacobi
+    eigensolver's input and the projection exactly (no overflow or denormals at 2^+-30), so
+    the result scales exactly: truncate_rank(2^k x) == 2^k truncate_rank(x), bit for bit.
+    This pins the Jacobi kernel's power-of-two prescaling of the Gram matrix.  (rocSOLVER
+    rescales large or small matrices by other factors, so the property is the Jacobi
+    kernel's.)"""
+    import torch
+    import ffdas_amd
+    monkeypatch.setenv("FFDAS_RANK_EIG", "jacobi")
+    x = _rand((40, 3000), getattr(torch, dtype), 21).to(dev)
+    base = ffdas_amd.truncate_rank(x, 3)
+    for k in (30, -30):
+        got = ffdas_amd.truncate_rank(x * 2.0 ** k, 3)
+        assert torch.equal(got, base * 2.0 ** k), k
+
+
+@pytest.mark.parametrize("dtype", ["float32", "complex64", "float64"])
+@pytest.mark.parametrize("scale", [1e12, 1e-12])
+def test_extreme_magnitudes(dev, dtype, scale):
+    """Magnitudes far from 1 (Gram entries near 1e24 x n or 1e-24): the oracle's result to the
+    usual tolerance."""
+    import torch
+    x = (_rand((33, 700), getattr(torch, dtype), 22) * scale).to(dev)
+    _check(x, 2, 20, TOL[dtype])
+
+
+@pytest.mark.parametrize("dtype", ["float32", "complex128"])
+def test_zero_input(dev, dtype):
+    """All-zero data: no rotation, no component; the filtered result is zero in both the
+    subtract (start 3) and keep (stop 4) forms."""
+    import torch
+    import ffdas_amd
+    x = torch.zeros((24, 500), dtype=getattr(torch, dtype), device=dev)
+    for start, stop in [(3, None), (0, 4), (1, 3)]:
+        got = ffdas_amd.truncate_rank(x, start, stop)
+        assert torch.equal(got, torch.zeros_like(x)), (start, stop)
+
+
+@pytest.mark.parametrize("solver", ["jacobi", "rocsolver"])
+def test_repeated_singular_values(dev, monkeypatch, solver):
+    """Singular values with ties inside the removed and the kept sets (a gap only at the
+    boundary): the truncation is still unique, x minus its top-3 part."""
+    import torch
+    import ffdas_amd
+    monkeypatch.setenv("FFDAS_RANK_EIG", solver)
+    m, n = 30, 4000
+    g = torch.Generator().manual_seed(23)
+    u, _ = torch.linalg.qr(torch.randn(m, m, dtype=torch.complex128, generator=g))
+    v, _ = torch.linalg.qr(torch.randn(n, m, dtype=torch.complex128, generator=g))
+    s = torch.tensor([50.0] * 3 + [5.0] * 10 + [1.0] * (m - 13), dtype=torch.complex128)
+    x = ((u * s) @ v.conj().T).to(dev)
+    want = ((u[:, 3:] * s[3:]) @ v[:, 3:].conj().T).to(dev)
+    got = ffdas_amd.truncate_rank(x.to(torch.complex64), 3)
+    assert float((got.to(torch.complex128) - want).norm() / want.norm()) < 1e-4
+    got64 = ffdas_amd.truncate_rank(x, 3)
+    assert float((got64 - want).norm() / want.norm()) < 1e-9
+    keep = ffdas_amd.truncate_rank(x, 3, 13)      # the tied middle block alone
+    want_keep = ((u[:, 3:13] * s[3:13]) @ v[:, 3:13].conj().T).to(dev)
+    assert float((keep - want_keep).norm() / want_keep.norm()) < 1e-9
