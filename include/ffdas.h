@@ -103,8 +103,17 @@ typedef enum {
     FFDAS_ALG4 = 4
 } ffdas_alg_t;
 
+/* Interpolation mode; values as in the reference (include/ffdas.h:92-95). */
+typedef enum {
+    FFDAS_INTERP_NEAREST = 0,
+    FFDAS_INTERP_LINEAR = 1
+} ffdas_interp_mode_t;
+
 struct ffdas_context;
 typedef struct ffdas_context *ffdas_handle_t;
+
+struct ffdas_interpolation_plan;
+typedef struct ffdas_interpolation_plan *ffdas_interpolation_plan_t;
 
 struct ffdas_tensor_desc;
 typedef struct ffdas_tensor_desc *ffdas_tensor_desc_t;
@@ -318,11 +327,48 @@ FFDAS_API ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_
                                             const void *x, int64_t start, int64_t stop,
                                             ffdas_tensor_desc_t out_desc, void *out);
 
+/* Interpolation from a structured (nz, ny, nx) grid of arbitrary vertex positions to
+ * scattered query points.  The plan splits every grid cell into the reference's six
+ * tetrahedra and bins them over the grid's bounding box (128 bins per axis); a query is
+ * located in the lowest-numbered tetrahedron of its bin whose FP32 barycentric test passes
+ * (the reference's result), and evaluates to the vertex of largest weight (NEAREST) or the
+ * barycentric combination of the four vertices (LINEAR), or to *fill_value outside the grid.
+ * gridpos: device, nz*ny*nx packed float triples, copied into the plan.
+ * Replaces include/ffdas.h:248-254. */
+FFDAS_API ffdas_error_t ffdas_create_interpolation_plan(ffdas_handle_t handle,
+                                                        ffdas_interpolation_plan_t *plan,
+                                                        int64_t nx, int64_t ny, int64_t nz,
+                                                        const float *gridpos,
+                                                        ffdas_interp_mode_t mode);
+
+/* Locates num_querypos points (device, packed float triples) once and keeps the result in
+ * the plan; a later ffdas_interpolation with the same num_querypos uses it and ignores its
+ * querypos argument (as the reference does).  Replaces include/ffdas.h:256-261. */
+FFDAS_API ffdas_error_t ffdas_interpolation_preprocess(ffdas_handle_t handle,
+                                                       ffdas_interpolation_plan_t plan,
+                                                       int64_t num_querypos,
+                                                       const float *querypos);
+
+/* out[b, q] for x ([batch,] nz, ny, nx) per x_desc (R32F, C32F, R64F or C64F, packed) and
+ * out (batch, num_querypos) packed, same dtype; fill_value is a host pointer to one element
+ * of x's type.  querypos may be NULL when the plan was preprocessed for num_querypos points.
+ * Replaces include/ffdas.h:263-272. */
+FFDAS_API ffdas_error_t ffdas_interpolation(ffdas_handle_t handle, ffdas_interpolation_plan_t plan,
+                                            int64_t num_querypos, const float *querypos,
+                                            ffdas_tensor_desc_t x_desc, const void *x, void *out,
+                                            const void *fill_value);
+
+/* Releases the plan's device memory (after the handle's stream has finished with it).
+ * Replaces include/ffdas.h:274-277. */
+FFDAS_API ffdas_error_t ffdas_destroy_interpolation_plan(ffdas_handle_t handle,
+                                                         ffdas_interpolation_plan_t plan);
+
 /* Opt-in kernel timing: while enabled, every kernel the library launches is bracketed by
  * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
  * "das_bwd_x", "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
- * "transpose_gout", "rank_gram", "rank_gram_reduce", "rank_jacobi", "rank_project").
+ * "transpose_gout", "rank_gram", "rank_gram_reduce", "rank_jacobi", "rank_project",
+ * "interp_locate", "interp_eval").
  * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);
