@@ -13,6 +13,7 @@ counted once per forward+adjoint step.  Prints ONE JSON line.
   python bench.py --profile-only N   # N steps + a 256 MiB copy, for rocprofv3 --pmc runs
   python bench.py --config 5         # batched sparse compounding, three probes, one graph
   python bench.py --op truncate_rank # clutter filter (rank truncation) of a frame ensemble
+  python bench.py --op interpolate   # scan conversion of a sector volume to a Cartesian grid
 
 roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
 profiles/das_config4_<storage>_traffic.json, written by ffdas_amd.utils.pmc from two
@@ -51,9 +52,10 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--config", type=int, default=4, choices=[4, 5],
                     help="4: fwd+adjoint headline (default); 5: batched sparse compounding")
-    ap.add_argument("--op", default="das", choices=["das", "truncate_rank"],
+    ap.add_argument("--op", default="das", choices=["das", "truncate_rank", "interpolate"],
                     help="das (default): the delay-and-sum benchmarks; truncate_rank: the "
-                         "clutter filter on a beamformed frame ensemble")
+                         "clutter filter on a beamformed frame ensemble; interpolate: scan "
+                         "conversion of beamformed frames")
     ap.add_argument("--c5-frames", type=int, default=200, help="config 5 ensemble size")
     ap.add_argument("--c5-probes", default="linear64,linear256,matrix1024")
     ap.add_argument("--no-autotune", action="store_true",
@@ -297,6 +299,112 @@ def bench_config5(a):
     print(json.dumps(line), flush=True)
 
 
+def sector_grid(nr, nt, nphi, r0=0.05, r1=1.0, half=0.6):
+    """Vertices (nr, nt, nphi, 3) of a volumetric sector scan: radius r0..r1 along z, two
+    steering angles in [-half, half] rad (FP32)."""
+    import numpy as np
+    r = np.linspace(r0, r1, nr)
+    th = np.linspace(-half, half, nt)
+    ph = np.linspace(-half, half, nphi)
+    R, TH, PH = np.meshgrid(r, th, ph, indexing="ij")
+    return np.stack([R * np.sin(TH) * np.cos(PH), R * np.sin(PH), R * np.cos(TH) * np.cos(PH)],
+                    -1).astype(np.float32)
+
+
+def bench_interpolate(a):
+    """Scan conversion after beamforming: interpolate 16 frames of a 512 x 96 x 96 sector
+    volume (radius x two steering angles, FP32 envelope) onto a 256^3 Cartesian grid over its
+    bounding box, with the plan built and the query points located once (preprocess), as a
+    per-frame display loop runs it.  A step is one evaluation of all 16 frames.  Data:
+    standard normal on the device."""
+    import ctypes
+    import numpy as np
+    import torch
+    import ffdas_amd
+    from ffdas_amd._lib import lib
+    dev = torch.device("cuda:0")
+    L = lib()
+
+    def kernel_ms(name):
+        t, c = ctypes.c_double(), ctypes.c_int64()
+        L.ffdas_profile_query(name.encode(), ctypes.byref(t), ctypes.byref(c))
+        return t.value / max(c.value, 1) if c.value else 0.0
+
+    B, nr, nt, nphi, nq1 = 16, 512, 96, 96, 256
+    g = sector_grid(nr, nt, nphi)
+    flat = g.reshape(-1, 3)
+    ax = [np.linspace(flat[:, k].min(), flat[:, k].max(), nq1, dtype=np.float32) for k in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    q = torch.from_numpy(np.stack([X, Y, Z], -1).astype(np.float32)).to(dev)
+    gd = torch.from_numpy(g).to(dev)
+    gen = torch.Generator(device=dev).manual_seed(6)
+    x = torch.randn((B, nr, nt, nphi), device=dev, generator=gen)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    interp = ffdas_amd.Interpolator(gd, mode="linear")
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    L.ffdas_profile_reset()
+    L.ffdas_profile_enable(1)
+    interp.preprocess(q)
+    torch.cuda.synchronize()
+    L.ffdas_profile_enable(0)
+    locate_ms = kernel_ms("interp_locate")
+    out = torch.empty((B,) + tuple(q.shape[:-1]), device=dev)
+    steps = max(5, a.steps)
+    ms, wall = time_steps(lambda: interp(x, q, out=out), steps, a.warmup)
+    L.ffdas_profile_reset()
+    L.ffdas_profile_enable(1)
+    for _ in range(max(3, steps // 4)):
+        interp(x, q, out=out)
+    torch.cuda.synchronize()
+    L.ffdas_profile_enable(0)
+    eval_ms = kernel_ms("interp_eval")
+    nq, npts = nq1 ** 3, nr * nt * nphi
+    located = float(torch.isfinite(interp(x[:1], q, fill=float("nan"))).float().mean())
+    value = B * nq / (ms * 1e-3) / 1e9
+    log(f"interpolate {B} x {nr}x{nt}x{nphi} -> {nq1}^3: {ms:.3f} ms/step ({value:.1f} G/s), "
+        f"eval kernel {eval_ms:.3f} ms, plan {plan_ms:.1f} ms, locate {locate_ms:.3f} ms, "
+        f"located {located:.3f}")
+    # algorithmic bytes of one evaluation: every frame of x once, every output once, the
+    # lookup (4 ids + 4 weights per point) once per chunk of 8 frames
+    nbytes = B * (nq + npts) * 4 + -(-B // 8) * nq * 32
+    roof = {"kernel": "interp_eval", "bound": "hbm", "ms": round(eval_ms, 4),
+            "achieved": round(nbytes / (eval_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "bytes_per_launch": nbytes, "traffic": None}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    cpu = None
+    if not a.no_cpu:
+        import oracle.interpolation as OI
+        # the oracle end to end (plan, location, evaluation) on a 64 x 24 x 24 sector and
+        # 4000 of its box's points, 16 frames
+        gs = sector_grid(64, 24, 24)
+        fs = gs.reshape(-1, 3)
+        rng = np.random.default_rng(7)
+        qs = rng.uniform(fs.min(0), fs.max(0), (4000, 3)).astype(np.float32)
+        xs = rng.standard_normal((B,) + gs.shape[:3]).astype(np.float32)
+        t0 = time.perf_counter()
+        OI.interpolate(gs, xs, qs, mode="linear")
+        secs = time.perf_counter() - t0
+        cpu = {"value": round(B * qs.shape[0] / secs / 1e9, 6), "unit": "Gsample/s",
+               "cores": 1, "kind": "port",
+               "sample": "oracle interpolate (numpy, one thread) of 16 frames of a 64 x 24 x 24 "
+                         "sector at 4000 points, plan and location included"}
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "Gsample/s interpolate (query points x frames per second, preprocessed plan)",
+        "value": round(value, 3), "unit": "Gsample/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"scan conversion: {B} frames of a {nr} x {nt} x {nphi} sector "
+                               f"volume (FP32) to {nq1}^3 Cartesian points, linear",
+                   "plan_ms": round(plan_ms, 2), "locate_ms": round(locate_ms, 3),
+                   "located_fraction": round(located, 4), "wall_s": round(wall, 3)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_truncate_rank(a):
     """The clutter filter after beamforming: truncate_rank of a frame ensemble, removing the
     strongest singular components (tissue).  Workload: the reference's clutter-filter example
@@ -398,6 +506,9 @@ def main():
         sys.exit(2)
     if a.op == "truncate_rank":
         bench_truncate_rank(a)
+        return
+    if a.op == "interpolate":
+        bench_interpolate(a)
         return
     if a.config == 5:
         bench_config5(a)

@@ -10,8 +10,9 @@
 // Query (once per point set): per point, its bin and the FP32 barycentric test of each
 // candidate, computed without contraction in the reference's operation order; the result is
 // four vertex ids and weights (linear) or one vertex id (nearest).
-// Evaluation (per call, the per-frame part): one thread per query point walks a chunk of
-// frames, loading its lookup once per chunk; x and out are packed.
+// Evaluation (per call, the per-frame part): x is first copied frame-inner (the 8 frames of a
+// chunk contiguous per vertex), so one thread per query point fetches each of its vertices'
+// 8 frames with one vector load; its lookup is read once per chunk; out is packed.
 #include <hip/hip_runtime.h>
 
 #include <cstring>   // before rocPRIM, whose iterators use memset
@@ -262,43 +263,92 @@ struct El {
 
 constexpr int kFramesPerThread = 8;
 
+// the eight frames of one vertex (one chunk), contiguous: one vector load per vertex
 template <typename R, int CW>
-__global__ __launch_bounds__(256) void interp_nearest_kernel(const El<R, CW> *__restrict__ x,
+struct alignas(16) Frames {
+    El<R, CW> f[kFramesPerThread];
+};
+
+// x [frame][vertex] -> xfi [chunk][vertex][frame in chunk], zero past the last frame; each
+// thread reads its vertex from the chunk's frames (coalesced per frame) and writes the
+// vertex's frames as one contiguous block (coalesced across threads)
+template <typename R, int CW>
+__global__ __launch_bounds__(256) void interp_frames_kernel(const El<R, CW> *__restrict__ x,
+                                                            Frames<R, CW> *__restrict__ xfi,
+                                                            int64_t npts, int64_t B) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= npts) return;
+    const int64_t b0 = (int64_t)blockIdx.y * kFramesPerThread;
+    Frames<R, CW> o;
+#pragma unroll
+    for (int f = 0; f < kFramesPerThread; f++) {
+        if (b0 + f < B) {
+            o.f[f] = x[(b0 + f) * npts + v];
+        } else {
+#pragma unroll
+            for (int c = 0; c < CW; c++) o.f[f].c[c] = R(0);
+        }
+    }
+    xfi[blockIdx.y * npts + v] = o;
+}
+
+template <typename R, int CW>
+__global__ __launch_bounds__(256) void interp_nearest_kernel(const Frames<R, CW> *__restrict__ xfi,
                                                              El<R, CW> *__restrict__ out,
                                                              const int *__restrict__ idx, int64_t n,
-                                                             int64_t B, int64_t bs, El<R, CW> fill) {
+                                                             int64_t npts, int64_t B,
+                                                             El<R, CW> fill) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int k = idx[i];
     const int64_t b0 = (int64_t)blockIdx.y * kFramesPerThread;
-    const int64_t b1 = b0 + kFramesPerThread < B ? b0 + kFramesPerThread : B;
-    for (int64_t b = b0; b < b1; b++) out[b * n + i] = k < 0 ? fill : x[b * bs + k];
+    const int nf = (int)(B - b0 < kFramesPerThread ? B - b0 : kFramesPerThread);
+    if (k >= 0) {
+        const Frames<R, CW> v = xfi[blockIdx.y * npts + k];
+#pragma unroll
+        for (int f = 0; f < kFramesPerThread; f++)
+            if (f < nf) out[(b0 + f) * n + i] = v.f[f];
+    } else {
+#pragma unroll
+        for (int f = 0; f < kFramesPerThread; f++)
+            if (f < nf) out[(b0 + f) * n + i] = fill;
+    }
 }
 
 template <typename R, int CW>
-__global__ __launch_bounds__(256) void interp_linear_kernel(const El<R, CW> *__restrict__ x,
+__global__ __launch_bounds__(256) void interp_linear_kernel(const Frames<R, CW> *__restrict__ xfi,
                                                             El<R, CW> *__restrict__ out,
                                                             const int4 *__restrict__ idx,
                                                             const float4 *__restrict__ wt, int64_t n,
-                                                            int64_t B, int64_t bs, El<R, CW> fill) {
+                                                            int64_t npts, int64_t B,
+                                                            El<R, CW> fill) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int4 k = idx[i];
     const float4 w = wt[i];
     const int64_t b0 = (int64_t)blockIdx.y * kFramesPerThread;
-    const int64_t b1 = b0 + kFramesPerThread < B ? b0 + kFramesPerThread : B;
-    for (int64_t b = b0; b < b1; b++) {
+    const int nf = (int)(B - b0 < kFramesPerThread ? B - b0 : kFramesPerThread);
+    const Frames<R, CW> *xc = xfi + blockIdx.y * npts;
+    Frames<R, CW> v0, v1, v2, v3;
+    if (k.x >= 0) {
+        v0 = xc[k.x];
+        v1 = xc[k.y];
+        v2 = xc[k.z];
+        v3 = xc[k.w];
+    }
+#pragma unroll
+    for (int f = 0; f < kFramesPerThread; f++) {
+        if (f >= nf) break;
         El<R, CW> r = fill;
         if (k.x >= 0) {
-            const El<R, CW> *xb = x + b * bs;
-            const El<R, CW> v0 = xb[k.x], v1 = xb[k.y], v2 = xb[k.z], v3 = xb[k.w];
             // w0 v0 + w1 v1 + w2 v2 + w3 v3, left to right (interpolation_kernels.cuh:110-115)
 #pragma unroll
             for (int c = 0; c < CW; c++)
-                r.c[c] = (R)w.x * v0.c[c] + (R)w.y * v1.c[c] + (R)w.z * v2.c[c] + (R)w.w * v3.c[c];
+                r.c[c] = (R)w.x * v0.f[f].c[c] + (R)w.y * v1.f[f].c[c] + (R)w.z * v2.f[f].c[c] +
+                         (R)w.w * v3.f[f].c[c];
         }
-        out[b * n + i] = r;
+        out[(b0 + f) * n + i] = r;
     }
 }
 
@@ -349,20 +399,32 @@ ffdas_error_t alloc_lookup(int mode, int64_t n, int **idx, float **w) {
 
 template <typename R, int CW>
 ffdas_error_t evaluate(ffdas_context &h, int mode, const void *x, void *out, const int *idx,
-                       const float *w, int64_t n, int64_t B, int64_t bs, const void *fill) {
+                       const float *w, int64_t n, int64_t B, int64_t npts, const void *fill) {
     using E = El<R, CW>;
+    using F = Frames<R, CW>;
     E f;
     for (int c = 0; c < CW; c++) f.c[c] = static_cast<const R *>(fill)[c];
-    const dim3 grid(grid1(n), (unsigned)((B + kFramesPerThread - 1) / kFramesPerThread));
+    const int64_t chunks = (B + kFramesPerThread - 1) / kFramesPerThread;
+    // the frame-inner copy of x lives in the handle's workspace (which then no longer holds
+    // the converted x of an earlier delay-and-sum call)
+    void *ws = nullptr;
+    h.xcopy.valid = false;
+    FFDAS_TRY(h.workspace((size_t)chunks * npts * sizeof(F), &ws));
+    F *xfi = static_cast<F *>(ws);
+    {
+        ProfScope ps(h.stream, "interp_frames");
+        hipLaunchKernelGGL((interp_frames_kernel<R, CW>), dim3(grid1(npts), (unsigned)chunks),
+                           dim3(256), 0, h.stream, static_cast<const E *>(x), xfi, npts, B);
+    }
+    const dim3 grid(grid1(n), (unsigned)chunks);
     ProfScope ps(h.stream, "interp_eval");
     if (mode == FFDAS_INTERP_NEAREST)
-        hipLaunchKernelGGL((interp_nearest_kernel<R, CW>), grid, dim3(256), 0, h.stream,
-                           static_cast<const E *>(x), static_cast<E *>(out), idx, n, B, bs, f);
+        hipLaunchKernelGGL((interp_nearest_kernel<R, CW>), grid, dim3(256), 0, h.stream, xfi,
+                           static_cast<E *>(out), idx, n, npts, B, f);
     else
-        hipLaunchKernelGGL((interp_linear_kernel<R, CW>), grid, dim3(256), 0, h.stream,
-                           static_cast<const E *>(x), static_cast<E *>(out),
-                           reinterpret_cast<const int4 *>(idx), reinterpret_cast<const float4 *>(w),
-                           n, B, bs, f);
+        hipLaunchKernelGGL((interp_linear_kernel<R, CW>), grid, dim3(256), 0, h.stream, xfi,
+                           static_cast<E *>(out), reinterpret_cast<const int4 *>(idx),
+                           reinterpret_cast<const float4 *>(w), n, npts, B, f);
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
 }
 

@@ -368,7 +368,7 @@ FFDAS_API ffdas_error_t ffdas_destroy_interpolation_plan(ffdas_handle_t handle,
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
  * "das_bwd_x", "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
  * "transpose_gout", "rank_gram", "rank_gram_reduce", "rank_jacobi", "rank_project",
- * "interp_locate", "interp_eval").
+ * "interp_locate", "interp_frames", "interp_eval").
  * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);
