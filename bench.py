@@ -14,6 +14,7 @@ counted once per forward+adjoint step.  Prints ONE JSON line.
   python bench.py --config 5         # batched sparse compounding, three probes, one graph
   python bench.py --op truncate_rank # clutter filter (rank truncation) of a frame ensemble
   python bench.py --op interpolate   # scan conversion of a sector volume to a Cartesian grid
+  python bench.py --op greens        # frequency-domain Green's function simulation
 
 roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
 profiles/das_config4_<storage>_traffic.json, written by ffdas_amd.utils.pmc from two
@@ -52,10 +53,12 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--config", type=int, default=4, choices=[4, 5],
                     help="4: fwd+adjoint headline (default); 5: batched sparse compounding")
-    ap.add_argument("--op", default="das", choices=["das", "truncate_rank", "interpolate"],
+    ap.add_argument("--op", default="das",
+                    choices=["das", "truncate_rank", "interpolate", "greens"],
                     help="das (default): the delay-and-sum benchmarks; truncate_rank: the "
                          "clutter filter on a beamformed frame ensemble; interpolate: scan "
-                         "conversion of beamformed frames")
+                         "conversion of beamformed frames; greens: frequency-domain "
+                         "simulation")
     ap.add_argument("--c5-frames", type=int, default=200, help="config 5 ensemble size")
     ap.add_argument("--c5-probes", default="linear64,linear256,matrix1024")
     ap.add_argument("--no-autotune", action="store_true",
@@ -405,6 +408,79 @@ def bench_interpolate(a):
     print(json.dumps(line), flush=True)
 
 
+def bench_greens(a):
+    """Frequency-domain simulation step: greens from a 128-element linear array (0.3 mm
+    pitch) to 32768 points of a 128 x 256 image plane, 512 frequency bins around 5 MHz, 8
+    transmit events (batch), complex64.  A step is one call.  Data: standard normal field
+    on the device."""
+    import ctypes
+    import math
+    import numpy as np
+    import torch
+    import ffdas_amd
+    from ffdas_amd._lib import lib
+    dev = torch.device("cuda:0")
+    L = lib()
+    B, ns, nf = 8, 128, 512
+    nz, nx = 256, 128
+    src = np.zeros((ns, 3), np.float32)
+    src[:, 0] = (np.arange(ns) - (ns - 1) / 2) * 0.3e-3
+    zz, xx = np.meshgrid(np.linspace(5e-3, 45e-3, nz), np.linspace(-19e-3, 19e-3, nx),
+                         indexing="ij")
+    dst = np.stack([xx, np.zeros_like(xx), zz], -1).astype(np.float32)
+    freqs = 5e6 + (np.arange(nf) - nf / 2) * (8e6 / nf)
+    k = (-2 * math.pi * freqs / 1540.0).astype(np.float32)
+    gen = torch.Generator(device=dev).manual_seed(8)
+    x = torch.complex(torch.randn((B, ns, nf), device=dev, generator=gen),
+                      torch.randn((B, ns, nf), device=dev, generator=gen))
+    srcd, dstd, kd = (torch.from_numpy(v).to(dev) for v in (src, dst, k))
+    out = torch.empty((B, nz, nx, nf), dtype=torch.complex64, device=dev)
+    steps = max(5, a.steps)
+    ms, wall = time_steps(lambda: ffdas_amd.greens(srcd, kd, x, dstd, out=out), steps, a.warmup)
+    t, c = ctypes.c_double(), ctypes.c_int64()
+    L.ffdas_profile_reset()
+    L.ffdas_profile_enable(1)
+    for _ in range(3):
+        ffdas_amd.greens(srcd, kd, x, dstd, out=out)
+    torch.cuda.synchronize()
+    L.ffdas_profile_enable(0)
+    L.ffdas_profile_query(b"greens", ctypes.byref(t), ctypes.byref(c))
+    kms = t.value / max(c.value, 1)
+    terms = B * nz * nx * ns * nf
+    value = terms / (ms * 1e-3) / 1e9
+    log(f"greens {B} x {ns} sources -> {nz}x{nx} points x {nf} freqs: {ms:.3f} ms/step, "
+        f"{value:.1f} G terms/s, kernel {kms:.3f} ms")
+    # the complex multiply-adds alone (8 flops a term) against the FP32 vector peak; each
+    # (target, source, frequency) also costs a sincos and a reciprocal square root
+    roof = {"kernel": "greens", "bound": "valu", "ms": round(kms, 4),
+            "achieved": round(terms * 8 / (kms * 1e-3) / 1e12, 2), "peak": 78.6,
+            "unit": "TFLOP/s", "traffic": None}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    cpu = None
+    if not a.no_cpu:
+        import oracle.greens as OG
+        # the FP64 oracle (numpy) on 512 of the points, all sources, frequencies and events
+        xs = x.cpu().numpy()
+        t0 = time.perf_counter()
+        OG.greens(src, k, xs, dst.reshape(-1, 3)[:512])
+        secs = time.perf_counter() - t0
+        cpu = {"value": round(B * 512 * ns * nf / secs / 1e9, 4), "unit": "Gterm/s",
+               "cores": 1, "kind": "port",
+               "sample": "oracle greens (numpy FP64) at 512 of the 32768 points"}
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "Gterm/s greens (source x target x frequency x event terms per second)",
+        "value": round(value, 2), "unit": "Gterm/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"greens: {ns}-element array to {nz} x {nx} points, {nf} "
+                               f"frequencies, {B} events, complex64",
+                   "wall_s": round(wall, 3)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_truncate_rank(a):
     """The clutter filter after beamforming: truncate_rank of a frame ensemble, removing the
     strongest singular components (tissue).  Workload: the reference's clutter-filter example
@@ -509,6 +585,9 @@ def main():
         return
     if a.op == "interpolate":
         bench_interpolate(a)
+        return
+    if a.op == "greens":
+        bench_greens(a)
         return
     if a.config == 5:
         bench_config5(a)

@@ -9,6 +9,7 @@ from ._lib import Algorithm, ComputeType, DType, FfdasError
 from .autograd import DasFunction, das_autograd
 from .das import (CapturedCall, DowncastWarning, autotune, clear_tuning, das, das_backward,
                   das_batched, das_sparse, load_tuning, save_tuning, tuning_entries)
+from .greens import greens
 from .interpolation import Interpolator, interpolate
 from .rank import truncate_rank
 from .utils.spatial import angle, cdist, rect_dist
@@ -22,5 +23,5 @@ __all__ = [
     "get_library_handle", "device_count", "device_get", "device_set", "autotune",
     "save_tuning", "load_tuning", "clear_tuning", "tuning_entries", "das_batched",
     "CapturedCall", "cdist", "rect_dist", "angle", "truncate_rank", "Interpolator",
-    "interpolate",
+    "interpolate", "greens",
 ]

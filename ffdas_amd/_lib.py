@@ -146,6 +146,7 @@ def lib():
         "ffdas_interpolation_preprocess": (i32, [vp, vp, i64, vp]),
         "ffdas_interpolation": (i32, [vp, vp, i64, vp, vp, vp, vp, vp]),
         "ffdas_destroy_interpolation_plan": (i32, [vp, vp]),
+        "ffdas_greens": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "ffdas_profile_enable": (i32, [i32]),
         "ffdas_profile_reset": (i32, []),
         "ffdas_profile_query": (i32, [ctypes.c_char_p, P(ctypes.c_double), P(i64)]),

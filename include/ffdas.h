@@ -363,12 +363,24 @@ FFDAS_API ffdas_error_t ffdas_interpolation(ffdas_handle_t handle, ffdas_interpo
 FFDAS_API ffdas_error_t ffdas_destroy_interpolation_plan(ffdas_handle_t handle,
                                                          ffdas_interpolation_plan_t plan);
 
+/* Frequency-domain Green's function summation: out[b, t, f] = sum_s x[b, s, f]
+ * exp(i k_f r) / r with r = |dstpos[t] - srcpos[s]| (r = rsq * rsqrt(rsq), as the reference;
+ * a target on a source gives a non-finite term).  x: (batch, sources, frequencies) C32F or
+ * C16F, packed; out: (batch, targets, frequencies) C32F, packed; srcpos, dstpos: device
+ * packed float triples; wavenums: device floats, one per frequency.  Arithmetic in FP32
+ * (the reference rounds x and the Green's function to FP16 for its tensor-core path).
+ * Replaces include/ffdas.h:279-288. */
+FFDAS_API ffdas_error_t ffdas_greens(ffdas_handle_t handle, const float *srcpos,
+                                     const float *wavenums, ffdas_tensor_desc_t x_desc,
+                                     const void *x, const float *dstpos,
+                                     ffdas_tensor_desc_t out_desc, void *out);
+
 /* Opt-in kernel timing: while enabled, every kernel the library launches is bracketed by
  * hipEvents on its stream; ffdas_profile_query returns the summed duration [ms] and launch
  * count per kernel name ("das_fwd", "das_fwd_fi", "das_bwd_param", "das_bwd_param_fi",
  * "das_bwd_x", "das_bwd_x_plain", "das_fwd_fi_batched", "relayout_fi", "cast_rows",
  * "transpose_gout", "rank_gram", "rank_gram_reduce", "rank_jacobi", "rank_project",
- * "interp_locate", "interp_frames", "interp_eval").
+ * "interp_locate", "interp_frames", "interp_eval", "greens").
  * Counterpart of the reference's NVTX ranges (no reference symbol). */
 FFDAS_API ffdas_error_t ffdas_profile_enable(int enable);
 FFDAS_API ffdas_error_t ffdas_profile_reset(void);
