@@ -370,8 +370,8 @@ def bench_interpolate(a):
         f"eval kernel {eval_ms:.3f} ms, plan {plan_ms:.1f} ms, locate {locate_ms:.3f} ms, "
         f"located {located:.3f}")
     # algorithmic bytes of one evaluation: every frame of x once, every output once, the
-    # lookup (4 ids + 4 weights per point) once per chunk of 8 frames
-    nbytes = B * (nq + npts) * 4 + -(-B // 8) * nq * 32
+    # lookup (4 ids + 4 weights per point) once per chunk of 16 frames (FP32 data)
+    nbytes = B * (nq + npts) * 4 + -(-B // 16) * nq * 32
     roof = {"kernel": "interp_eval", "bound": "hbm", "ms": round(eval_ms, 4),
             "achieved": round(nbytes / (eval_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "bytes_per_launch": nbytes, "traffic": None}

@@ -10,9 +10,10 @@
 // Query (once per point set): per point, its bin and the FP32 barycentric test of each
 // candidate, computed without contraction in the reference's operation order; the result is
 // four vertex ids and weights (linear) or one vertex id (nearest).
-// Evaluation (per call, the per-frame part): x is first copied frame-inner (the 8 frames of a
-// chunk contiguous per vertex), so one thread per query point fetches each of its vertices'
-// 8 frames with one vector load; its lookup is read once per chunk; out is packed.
+// Evaluation (per call, the per-frame part): x is first copied frame-inner (the frames of a
+// chunk, 16 for 4-byte data and 8 otherwise, contiguous per vertex), so one thread per query
+// point fetches each of its vertices' chunk with one vector load; its lookup is read once per
+// chunk; out is packed.
 #include <hip/hip_runtime.h>
 
 #include <cstring>   // before rocPRIM, whose iterators use memset
@@ -261,12 +262,18 @@ struct El {
     R c[CW];
 };
 
-constexpr int kFramesPerThread = 8;
+constexpr int kFramesPerThread = 8;   // frames per chunk: the smallest, for the extent check
 
-// the eight frames of one vertex (one chunk), contiguous: one vector load per vertex
+// frames per chunk: 16 for 4-byte elements (a 64-byte vector load per vertex), else 8
+template <typename R, int CW>
+constexpr int frames_per_chunk() {
+    return sizeof(R) * CW == 4 ? 16 : 8;
+}
+
+// the frames of one vertex in one chunk, contiguous: one vector load per vertex
 template <typename R, int CW>
 struct alignas(16) Frames {
-    El<R, CW> f[kFramesPerThread];
+    El<R, CW> f[frames_per_chunk<R, CW>()];
 };
 
 // x [frame][vertex] -> xfi [chunk][vertex][frame in chunk], zero past the last frame; each
@@ -276,12 +283,13 @@ template <typename R, int CW>
 __global__ __launch_bounds__(256) void interp_frames_kernel(const El<R, CW> *__restrict__ x,
                                                             Frames<R, CW> *__restrict__ xfi,
                                                             int64_t npts, int64_t B) {
+    constexpr int FPC = frames_per_chunk<R, CW>();
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (v >= npts) return;
-    const int64_t b0 = (int64_t)blockIdx.y * kFramesPerThread;
+    const int64_t b0 = (int64_t)blockIdx.y * FPC;
     Frames<R, CW> o;
 #pragma unroll
-    for (int f = 0; f < kFramesPerThread; f++) {
+    for (int f = 0; f < FPC; f++) {
         if (b0 + f < B) {
             o.f[f] = x[(b0 + f) * npts + v];
         } else {
@@ -298,19 +306,20 @@ __global__ __launch_bounds__(256) void interp_nearest_kernel(const Frames<R, CW>
                                                              const int *__restrict__ idx, int64_t n,
                                                              int64_t npts, int64_t B,
                                                              El<R, CW> fill) {
+    constexpr int FPC = frames_per_chunk<R, CW>();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int k = idx[i];
-    const int64_t b0 = (int64_t)blockIdx.y * kFramesPerThread;
-    const int nf = (int)(B - b0 < kFramesPerThread ? B - b0 : kFramesPerThread);
+    const int64_t b0 = (int64_t)blockIdx.y * FPC;
+    const int nf = (int)(B - b0 < FPC ? B - b0 : FPC);
     if (k >= 0) {
         const Frames<R, CW> v = xfi[blockIdx.y * npts + k];
 #pragma unroll
-        for (int f = 0; f < kFramesPerThread; f++)
+        for (int f = 0; f < FPC; f++)
             if (f < nf) out[(b0 + f) * n + i] = v.f[f];
     } else {
 #pragma unroll
-        for (int f = 0; f < kFramesPerThread; f++)
+        for (int f = 0; f < FPC; f++)
             if (f < nf) out[(b0 + f) * n + i] = fill;
     }
 }
@@ -323,32 +332,29 @@ __global__ __launch_bounds__(256) void interp_linear_kernel(const Frames<R, CW> 
                                                             int64_t npts, int64_t B,
                                                             El<R, CW> fill) {
 #pragma clang fp contract(off)
+    constexpr int FPC = frames_per_chunk<R, CW>();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int4 k = idx[i];
     const float4 w = wt[i];
-    const int64_t b0 = (int64_t)blockIdx.y * kFramesPerThread;
-    const int nf = (int)(B - b0 < kFramesPerThread ? B - b0 : kFramesPerThread);
+    const int64_t b0 = (int64_t)blockIdx.y * FPC;
+    const int nf = (int)(B - b0 < FPC ? B - b0 : FPC);
     const Frames<R, CW> *xc = xfi + blockIdx.y * npts;
-    Frames<R, CW> v0, v1, v2, v3;
-    if (k.x >= 0) {
-        v0 = xc[k.x];
-        v1 = xc[k.y];
-        v2 = xc[k.z];
-        v3 = xc[k.w];
-    }
+    // unlocated points read vertex 0 (always present) and write the fill value: the loads
+    // stay unconditional, so the four blocks live in registers
+    const bool in = k.x >= 0;
+    const Frames<R, CW> v0 = xc[in ? k.x : 0], v1 = xc[in ? k.y : 0], v2 = xc[in ? k.z : 0],
+                        v3 = xc[in ? k.w : 0];
 #pragma unroll
-    for (int f = 0; f < kFramesPerThread; f++) {
-        if (f >= nf) break;
-        El<R, CW> r = fill;
-        if (k.x >= 0) {
-            // w0 v0 + w1 v1 + w2 v2 + w3 v3, left to right (interpolation_kernels.cuh:110-115)
+    for (int f = 0; f < FPC; f++) {
+        El<R, CW> r;
+        // w0 v0 + w1 v1 + w2 v2 + w3 v3, left to right (interpolation_kernels.cuh:110-115)
 #pragma unroll
-            for (int c = 0; c < CW; c++)
-                r.c[c] = (R)w.x * v0.f[f].c[c] + (R)w.y * v1.f[f].c[c] + (R)w.z * v2.f[f].c[c] +
-                         (R)w.w * v3.f[f].c[c];
-        }
-        out[(b0 + f) * n + i] = r;
+        for (int c = 0; c < CW; c++)
+            r.c[c] = in ? (R)w.x * v0.f[f].c[c] + (R)w.y * v1.f[f].c[c] + (R)w.z * v2.f[f].c[c] +
+                              (R)w.w * v3.f[f].c[c]
+                        : fill.c[c];
+        if (f < nf) out[(b0 + f) * n + i] = r;
     }
 }
 
@@ -404,7 +410,8 @@ ffdas_error_t evaluate(ffdas_context &h, int mode, const void *x, void *out, con
     using F = Frames<R, CW>;
     E f;
     for (int c = 0; c < CW; c++) f.c[c] = static_cast<const R *>(fill)[c];
-    const int64_t chunks = (B + kFramesPerThread - 1) / kFramesPerThread;
+    constexpr int FPC = frames_per_chunk<R, CW>();
+    const int64_t chunks = (B + FPC - 1) / FPC;
     // the frame-inner copy of x lives in the handle's workspace (which then no longer holds
     // the converted x of an earlier delay-and-sum call)
     void *ws = nullptr;
