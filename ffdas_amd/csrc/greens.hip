@@ -1,10 +1,10 @@
 // Frequency-domain Green's function summation (ffdas_greens):
 //   out[b, t, f] = sum_s x[b, s, f] e^{i k_f r_ts} / r_ts,   r_ts = |dst_t - src_s|.
-// One lane per target: its position and, per source, its distance stay in registers; the
-// source position and the x values of a (source, frequency chunk, batch chunk) are the same
-// for the whole wave, so they come through the scalar unit.  Each thread keeps FPT
-// frequencies x BT batch items of complex sums.  Every term costs a sincos and two products
-// on the VALU whatever the batch, so the kernel is VALU-bound; FP32 throughout (the input
+// Fewer than 8 events (greens_kernel): one lane per target, its position and per-source
+// distance in registers; the source position and the x values of a (source, frequency
+// chunk, event chunk) are the same for the whole wave, so they come through the scalar unit;
+// each thread keeps FPT frequencies x BT events of complex sums on the VALU.  8 or more
+// events (greens_mfma_kernel): the sums on the MFMA, see there.  FP32 throughout (the input
 // may be stored as FP16).
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -87,9 +87,97 @@ __global__ __launch_bounds__(256) void greens_kernel(const float *__restrict__ s
     }
 }
 
+// Many events: the multiply-adds on the MFMA.  Per wave a 16-target x 16-event tile of
+// kGreensFPW frequencies; per step of 4 sources, lane (jl, g) = (lane % 16, lane / 16)
+// generates the Green's function of target t0 + jl and source s0 + g (the A operand of
+// v_mfma_f32_16x16x4f32) and loads x of event b0 + jl, source s0 + g (the B operand); the
+// complex product is four real MFMAs (A_r B_r - A_i B_i, A_r B_i + A_i B_r).  Output register
+// v of lane (jl, g) holds target t0 + 4 g + v, event b0 + jl.  One sincos per (target,
+// source, frequency) is all the VALU does per term.
+constexpr int kGreensFPW = 8;
+
+typedef float GV4 __attribute__((ext_vector_type(4)));
+
+template <typename XT>
+__global__ __launch_bounds__(256) void greens_mfma_kernel(const float *__restrict__ src,
+                                                          const float *__restrict__ wav,
+                                                          const XT *__restrict__ x,
+                                                          const float *__restrict__ dst,
+                                                          float2 *__restrict__ out, int nsrc,
+                                                          int nfreq, int ndst, int B) {
+    const int lane = threadIdx.x & 63, jl = lane & 15, g = lane >> 4;
+    const int t0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int f0 = blockIdx.y * kGreensFPW;
+    const int b0 = blockIdx.z * 16;
+    const int nf = min(kGreensFPW, nfreq - f0);
+    const int ta = t0 + jl;                  // this lane's target in the A operand
+    const bool tok = ta < ndst;
+    float px = 0, py = 0, pz = 0;
+    if (tok) {
+        px = dst[3 * (int64_t)ta];
+        py = dst[3 * (int64_t)ta + 1];
+        pz = dst[3 * (int64_t)ta + 2];
+    }
+    const int be = b0 + jl;                  // this lane's event in the B operand
+    const bool bok = be < B;
+    float k[kGreensFPW];
+#pragma unroll
+    for (int f = 0; f < kGreensFPW; f++) k[f] = f < nf ? wav[f0 + f] : 0.0f;
+    GV4 cr[kGreensFPW], ci[kGreensFPW];
+#pragma unroll
+    for (int f = 0; f < kGreensFPW; f++) {
+        cr[f] = GV4{0.0f, 0.0f, 0.0f, 0.0f};
+        ci[f] = GV4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    for (int s0 = 0; s0 < nsrc; s0 += 4) {
+        const int s = s0 + g;
+        const bool sok = s < nsrc;
+        const bool aok = tok && sok;
+        float rinv = 0.0f, r = 0.0f;
+        if (aok) {
+            const float dx = px - src[3 * s], dy = py - src[3 * s + 1], dz = pz - src[3 * s + 2];
+            const float rsq = dx * dx + dy * dy + dz * dz;
+            rinv = rsqrtf(rsq);
+            r = rsq * rinv;
+        }
+        const XT *xs = x + ((int64_t)(bok ? be : 0) * nsrc + (sok ? s : 0)) * nfreq + f0;
+        const bool xok = bok && sok;
+#pragma unroll
+        for (int f = 0; f < kGreensFPW; f++) {
+            float sn, cs;
+            __sincosf(k[f] * r, &sn, &cs);
+            const float ar = aok ? cs * rinv : 0.0f, ai = aok ? sn * rinv : 0.0f;
+            const float2 v = (xok && f < nf) ? load_x<XT>(xs + f) : float2{0.0f, 0.0f};
+            cr[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, v.x, cr[f], 0, 0, 0);
+            cr[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, v.y, cr[f], 0, 0, 0);
+            ci[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, v.y, ci[f], 0, 0, 0);
+            ci[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, v.x, ci[f], 0, 0, 0);
+        }
+    }
+    if (!bok) return;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int t = t0 + 4 * g + v;
+        if (t >= ndst) continue;
+        float2 *o = out + ((int64_t)be * ndst + t) * nfreq + f0;
+#pragma unroll
+        for (int f = 0; f < kGreensFPW; f++)
+            if (f < nf) o[f] = float2{cr[f][v], ci[f][v]};
+    }
+}
+
 template <typename XT>
 ffdas_error_t launch(hipStream_t st, const float *src, const float *wav, const void *x,
                      const float *dst, void *out, int nsrc, int nfreq, int ndst, int B) {
+    if (B >= 8) {
+        const dim3 g((unsigned)((ndst + 63) / 64), (unsigned)((nfreq + kGreensFPW - 1) / kGreensFPW),
+                     (unsigned)((B + 15) / 16));
+        ProfScope ps(st, "greens");
+        hipLaunchKernelGGL((greens_mfma_kernel<XT>), g, dim3(256), 0, st, src, wav,
+                           static_cast<const XT *>(x), dst, static_cast<float2 *>(out), nsrc,
+                           nfreq, ndst, B);
+        return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+    }
     const dim3 grid((unsigned)((ndst + 255) / 256), (unsigned)((nfreq + kGreensFPT - 1) / kGreensFPT),
                     1);
     ProfScope ps(st, "greens");

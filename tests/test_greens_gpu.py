@@ -1,7 +1,8 @@
 """greens on the GPU against the FP64 oracle (oracle/greens.py): the reference's own cases
 (bindings/python/tests/test_greens.py) at its tolerance (rtol 1e-3, atol 1e-4; the kernel
-computes in FP32, its error is about 1e-6 of the sum's scale), both batch tilings (1 and 4
-items per thread), frequency chunks with a remainder, FP16 input, a 2-D target grid and the
+computes in FP32, its error is about 1e-6 of the sum's scale), all three event tilings (1
+and 4 events per thread on the VALU, 16-event tiles on the MFMA from 8 events up), partial
+tiles of targets, sources, frequencies and events, FP16 input, a 2-D target grid and the
 C-ABI error codes."""
 import ctypes
 
@@ -57,7 +58,8 @@ def test_reference_zero_input(dev):
 
 
 @pytest.mark.parametrize("B, nin, nk, nout", [(1, 37, 13, 300), (4, 64, 16, 257),
-                                              (7, 20, 9, 70), (2, 1, 1, 1)])
+                                              (7, 20, 9, 70), (2, 1, 1, 1), (8, 37, 13, 300),
+                                              (20, 66, 17, 131), (16, 3, 8, 16)])
 def test_parity_with_the_oracle(dev, B, nin, nk, nout):
     import ffdas_amd
     rng = np.random.default_rng(B * 1000 + nin)
@@ -79,10 +81,11 @@ def test_two_dimensional_input_and_fp16(dev):
     assert tuple(got2.shape) == (40, 10)
     np.testing.assert_allclose(got2.cpu().numpy(), G.greens(src, k, x[0], dst), rtol=1e-3,
                                atol=1e-4)
-    xh = _t(x, dev).to(torch.complex32)
-    goth = ffdas_amd.greens(_t(src, dev), _t(k, dev), xh, _t(dst, dev)).cpu().numpy()
-    want = G.greens(src, k, xh.to(torch.complex64).cpu().numpy(), dst)
-    assert np.abs(goth - want).max() < 1e-4 * np.abs(want).max()
+    for xin in (x, np.concatenate([x] * 9)):          # the VALU and the MFMA kernel
+        xh = _t(xin, dev).to(torch.complex32)
+        goth = ffdas_amd.greens(_t(src, dev), _t(k, dev), xh, _t(dst, dev)).cpu().numpy()
+        want = G.greens(src, k, xh.to(torch.complex64).cpu().numpy(), dst)
+        assert np.abs(goth - want).max() < 1e-4 * np.abs(want).max()
 
 
 def test_abi_error_codes(dev):
