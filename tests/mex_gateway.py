@@ -183,3 +183,74 @@ def mex_truncate_rank(x: torch.Tensor, start_1based: int, stop: int | None = Non
         check(lib().ffdas_truncate_rank(h.h, x_desc.ptr, xv.data_ptr(), start, int(stop),
                                         o_desc.ptr, out.data_ptr()), name)
     return out.reshape(x.shape)
+
+
+def mex_interpolate(gridpos, x, querypos, mode: str, fill_value):
+    """interpolate.m + ffdas_interpolate.cpp:12-120: gridpos (3, nx, ny, nz) in MATLAB (the
+    row-major view (nz, ny, nx, 3)), x (nx, ny, nz[, batch...]), querypos (3, ...), a mode
+    string and a one-element host fill value; a plan is created, evaluated and destroyed per
+    call; the result has x's leading (MATLAB-trailing) dims followed by the query dims."""
+    name = "ffdas_interpolate"
+    if gridpos.dim() != 4 or gridpos.shape[3] != 3:
+        raise MexError(f"{name}:error: gridpos must have shape (3, nx, ny, nz)")
+    fill = torch.as_tensor(fill_value, dtype=x.dtype).reshape(-1)
+    if fill.numel() != 1:
+        raise MexError(f"{name}:error: fill_value should contain exactly one element")
+    nz, ny, nx = (int(d) for d in gridpos.shape[:3])
+    if nz < 2 or ny < 2 or nx < 2:
+        raise MexError(f"{name}:error: grid size must be at least (2, 2, 2), got "
+                       f"({nx}, {ny}, {nz})")
+    nv = x.dim()
+    if nv < 3:
+        raise MexError(f"{name}:error: x must have at least 3 dimensions")
+    if tuple(x.shape[nv - 3:]) != (nz, ny, nx):
+        raise MexError(f"{name}:error: x spatial dims must match grid shape ({nx}, {ny}, {nz})")
+    nq = querypos.dim()
+    if nq < 2 or querypos.shape[nq - 1] != 3:
+        raise MexError(f"{name}:error: querypos must have shape (3, ...)")
+    modes = {"nearest": 0, "linear": 1}
+    if mode not in modes:
+        raise MexError(f"{name}:error: unknown interpolation mode: {mode}")
+    num_q = 1
+    for d in querypos.shape[:-1]:
+        num_q *= int(d)
+    out = torch.empty(tuple(x.shape[:nv - 3]) + tuple(querypos.shape[:-1]), dtype=x.dtype,
+                      device=x.device)
+    xd = TensorDesc(tuple(x.shape), tuple(x.stride()), element_code(x))
+    plan = ctypes.c_void_p()
+    L = lib()
+    with torch.cuda.device(x.device):
+        h = handle_for(x)
+        check(L.ffdas_create_interpolation_plan(h.h, ctypes.byref(plan), nx, ny, nz,
+                                                gridpos.data_ptr(), modes[mode]),
+              "ffdas_create_interpolation_plan")
+        err = L.ffdas_interpolation(h.h, plan, num_q, querypos.data_ptr(), xd.ptr, x.data_ptr(),
+                                    out.data_ptr(), fill.data_ptr())
+        L.ffdas_destroy_interpolation_plan(h.h, plan)
+    if err:
+        raise MexError(f"{name}:error: ffdas_interpolation returned error {err}")
+    return matlab_squeeze(out)
+
+
+def mex_greens(srcpos, wavenums, x, dstpos):
+    """greens.m + ffdas_greens.cpp:13-72: x (frequencies, sources[, batch]) in MATLAB, i.e.
+    the row-major (batch, sources, frequencies) view, a squeezed batch re-inserted; the
+    output (frequencies, dst..., batch), squeezed as MATLAB does."""
+    name = "ffdas_greens"
+    if x.dim() == 2:
+        x = x.reshape((1,) + tuple(x.shape))
+    elif x.dim() != 3:
+        raise MexError(f"{name}:error: input must have 2 or 3 dimensions")
+    B, ns, nf = (int(d) for d in x.shape)
+    dims = tuple(int(d) for d in dstpos.shape[:-1])
+    ndst = 1
+    for d in dims:
+        ndst *= d
+    out = torch.empty((B, ndst, nf), dtype=x.dtype, device=x.device)
+    xd = TensorDesc((B, ns, nf), tuple(x.stride()), element_code(x))
+    od = TensorDesc((B, ndst, nf), tuple(out.stride()), element_code(out))
+    with torch.cuda.device(x.device):
+        h = handle_for(x)
+        check(lib().ffdas_greens(h.h, srcpos.data_ptr(), wavenums.data_ptr(), xd.ptr,
+                                 x.data_ptr(), dstpos.data_ptr(), od.ptr, out.data_ptr()), name)
+    return matlab_squeeze(out.reshape((B,) + dims + (nf,)))
