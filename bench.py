@@ -69,6 +69,8 @@ def parse():
                          "configuration)")
     ap.add_argument("--profile-only", type=int, default=0, metavar="N",
                     help="run N steps and a 256 MiB calibration copy, print no JSON")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="end an --op run with the 256 MiB calibration copy (for --pmc runs)")
     return ap.parse_args()
 
 
@@ -176,9 +178,9 @@ def accuracy(p):
             oracle.rel_l2(gx.float().cpu().numpy(), gref))
 
 
-def traffic_for(storage, kernel):
+def traffic_for(storage, kernel, summary=None):
     """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
-    path = os.path.join(ROOT, "profiles", f"das_config4_{storage}_traffic.json")
+    path = os.path.join(ROOT, "profiles", summary or f"das_config4_{storage}_traffic.json")
     try:
         with open(path) as fh:
             k = json.load(fh)["kernels"].get(kernel + "_kernel")
@@ -187,16 +189,21 @@ def traffic_for(storage, kernel):
         return None
 
 
-def profile_only(p, x, n):
-    """A few steps and a 256 MiB device copy (the FETCH/WRITE_SIZE calibration)."""
+def calibration_copy():
+    """A 256 MiB device copy, twice: the FETCH/WRITE_SIZE counters' calibration."""
     import torch
-    for _ in range(n):
-        step(p, x)
-    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=p.dev)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
     for _ in range(2):
         dst.copy_(src)
     torch.cuda.synchronize()
+
+
+def profile_only(p, x, n):
+    """A few steps and a 256 MiB device copy (the FETCH/WRITE_SIZE calibration)."""
+    for _ in range(n):
+        step(p, x)
+    calibration_copy()
     log(f"profile-only: {n} steps and the calibration copy done")
 
 
@@ -374,7 +381,8 @@ def bench_interpolate(a):
     nbytes = B * (nq + npts) * 4 + -(-B // 16) * nq * 32
     roof = {"kernel": "interp_eval", "bound": "hbm", "ms": round(eval_ms, 4),
             "achieved": round(nbytes / (eval_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "bytes_per_launch": nbytes, "traffic": None}
+            "unit": "GB/s", "bytes_per_launch": nbytes,
+            "traffic": traffic_for(None, "interp_linear", "interpolate_scan_conversion_traffic.json")}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     cpu = None
     if not a.no_cpu:
@@ -455,7 +463,8 @@ def bench_greens(a):
     # (target, source, frequency) also costs a sincos on the VALU, overlapped with the MFMA)
     roof = {"kernel": "greens", "bound": "mfma", "ms": round(kms, 4),
             "achieved": round(terms * 8 / (kms * 1e-3) / 1e12, 2), "peak": 157.3,
-            "unit": "TFLOP/s", "traffic": None}
+            "unit": "TFLOP/s",
+            "traffic": traffic_for(None, "greens_mfma", "greens_array_simulation_traffic.json")}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     cpu = None
     if not a.no_cpu:
@@ -581,14 +590,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "described in DESIGN.md but not built", "n_gpus": a.gpus}))
         sys.exit(2)
-    if a.op == "truncate_rank":
-        bench_truncate_rank(a)
-        return
-    if a.op == "interpolate":
-        bench_interpolate(a)
-        return
-    if a.op == "greens":
-        bench_greens(a)
+    ops = {"truncate_rank": bench_truncate_rank, "interpolate": bench_interpolate,
+           "greens": bench_greens}
+    if a.op in ops:
+        ops[a.op](a)
+        if a.calibration_copy:
+            calibration_copy()
         return
     if a.config == 5:
         bench_config5(a)

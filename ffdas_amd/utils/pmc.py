@@ -26,12 +26,13 @@ import glob
 import json
 import os
 import re
+import statistics
 
 COPY_BYTES = 256 * 1024 * 1024
 
 
 def short_name(kernel: str) -> str:
-    m = re.search(r"ffdas::dev::(\w+)", kernel)
+    m = re.search(r"ffdas::(?:\(anonymous namespace\)::)?(?:dev::)?(\w+)", kernel)
     if m:
         return m.group(1)
     m = re.search(r"_ZN5ffdas3dev\d+([A-Za-z_]\w*?)(?:I|E)", kernel)
@@ -75,9 +76,9 @@ def summarize(fetch_dir: str, write_dir: str) -> dict:
     for name in sorted(set(fetch) | set(write)):
         if name == "copyBuffer" or not name.endswith("kernel"):
             continue
-        # the last dispatch of each kernel (after warm-up)
-        fb = fetch[name][-1] * 1024.0 * kf if name in fetch else None
-        wb = write[name][-1] * 1024.0 * kw if name in write else None
+        # the median dispatch of each kernel (warm-ups and odd-sized calls aside)
+        fb = statistics.median(fetch[name]) * 1024.0 * kf if name in fetch else None
+        wb = statistics.median(write[name]) * 1024.0 * kw if name in write else None
         kernels[name] = {"fetch_bytes": fb, "write_bytes": wb,
                          "traffic_bytes": (fb or 0.0) + (wb or 0.0),
                          "dispatches": len(fetch.get(name, []))}
