@@ -270,6 +270,27 @@ constexpr int frames_per_chunk() {
     return sizeof(R) * CW == 4 ? 16 : 8;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id.  This bijection hands each
+// XCD groups of kXcdGroup consecutive (query block, chunk) tiles instead of single ones, so
+// neighbouring query blocks, which share vertices, are served by one L2, while the XCDs
+// still take turns over the whole grid (the located points are unevenly spread: whole
+// contiguous eighths would leave some XCDs with most of the work).  Tiles past the last
+// full round of 8 groups keep their own id.  Returns the tile as (x, y).
+constexpr int kXcdGroup = 8;
+
+__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
+    const int total = gridDim.x * gridDim.y;
+    const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+    constexpr int R = 8 * kXcdGroup;
+    int L = lin;
+    if (lin < total / R * R) {
+        const int base = lin / R * R, o = lin - base;   // o = j * 8 + xcd within the round
+        L = base + (o % 8) * kXcdGroup + o / 8;
+    }
+    bx = L % gridDim.x;
+    by = L / gridDim.x;
+}
+
 // the frames of one vertex in one chunk, contiguous: one vector load per vertex
 template <typename R, int CW>
 struct alignas(16) Frames {
@@ -307,13 +328,15 @@ __global__ __launch_bounds__(256) void interp_nearest_kernel(const Frames<R, CW>
                                                              int64_t npts, int64_t B,
                                                              El<R, CW> fill) {
     constexpr int FPC = frames_per_chunk<R, CW>();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int bx, by;
+    xcd_tile(bx, by);
+    const int64_t i = (int64_t)bx * 256 + threadIdx.x;
     if (i >= n) return;
     const int k = idx[i];
-    const int64_t b0 = (int64_t)blockIdx.y * FPC;
+    const int64_t b0 = (int64_t)by * FPC;
     const int nf = (int)(B - b0 < FPC ? B - b0 : FPC);
     if (k >= 0) {
-        const Frames<R, CW> v = xfi[blockIdx.y * npts + k];
+        const Frames<R, CW> v = xfi[by * npts + k];
 #pragma unroll
         for (int f = 0; f < FPC; f++)
             if (f < nf) out[(b0 + f) * n + i] = v.f[f];
@@ -333,13 +356,15 @@ __global__ __launch_bounds__(256) void interp_linear_kernel(const Frames<R, CW> 
                                                             El<R, CW> fill) {
 #pragma clang fp contract(off)
     constexpr int FPC = frames_per_chunk<R, CW>();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int bx, by;
+    xcd_tile(bx, by);
+    const int64_t i = (int64_t)bx * 256 + threadIdx.x;
     if (i >= n) return;
     const int4 k = idx[i];
     const float4 w = wt[i];
-    const int64_t b0 = (int64_t)blockIdx.y * FPC;
+    const int64_t b0 = (int64_t)by * FPC;
     const int nf = (int)(B - b0 < FPC ? B - b0 : FPC);
-    const Frames<R, CW> *xc = xfi + blockIdx.y * npts;
+    const Frames<R, CW> *xc = xfi + by * npts;
     // unlocated points read vertex 0 (always present) and write the fill value: the loads
     // stay unconditional, so the four blocks live in registers
     const bool in = k.x >= 0;
