@@ -188,6 +188,27 @@ __global__ __launch_bounds__(256) void interp_fill_kernel(Box b, const float *__
             }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id.  This bijection hands each
+// XCD groups of kXcdGroup consecutive (query block, chunk) tiles instead of single ones, so
+// neighbouring query blocks, which share vertices, are served by one L2, while the XCDs
+// still take turns over the whole grid (the located points are unevenly spread: whole
+// contiguous eighths would leave some XCDs with most of the work).  Tiles past the last
+// full round of 8 groups keep their own id.  Returns the tile as (x, y).
+constexpr int kXcdGroup = 8;
+
+__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
+    const int total = gridDim.x * gridDim.y;
+    const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+    constexpr int R = 8 * kXcdGroup;
+    int L = lin;
+    if (lin < total / R * R) {
+        const int base = lin / R * R, o = lin - base;   // o = j * 8 + xcd within the round
+        L = base + (o % 8) * kXcdGroup + o / 8;
+    }
+    bx = L % gridDim.x;
+    by = L / gridDim.x;
+}
+
 // FP32 barycentric coordinates of p in (v0, v1, v2, v3) and whether all are >= 0, in the
 // reference's operation order (spatial_hash_kernels.cuh:62-90), with no FMA contraction
 __device__ bool barycentric(const float *p, const float *v0, const float *v1, const float *v2,
@@ -268,27 +289,6 @@ constexpr int kFramesPerThread = 8;   // frames per chunk: the smallest, for the
 template <typename R, int CW>
 constexpr int frames_per_chunk() {
     return sizeof(R) * CW == 4 ? 16 : 8;
-}
-
-// Workgroups are dealt to the 8 XCDs round-robin by linear id.  This bijection hands each
-// XCD groups of kXcdGroup consecutive (query block, chunk) tiles instead of single ones, so
-// neighbouring query blocks, which share vertices, are served by one L2, while the XCDs
-// still take turns over the whole grid (the located points are unevenly spread: whole
-// contiguous eighths would leave some XCDs with most of the work).  Tiles past the last
-// full round of 8 groups keep their own id.  Returns the tile as (x, y).
-constexpr int kXcdGroup = 8;
-
-__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
-    const int total = gridDim.x * gridDim.y;
-    const int lin = blockIdx.y * gridDim.x + blockIdx.x;
-    constexpr int R = 8 * kXcdGroup;
-    int L = lin;
-    if (lin < total / R * R) {
-        const int base = lin / R * R, o = lin - base;   // o = j * 8 + xcd within the round
-        L = base + (o % 8) * kXcdGroup + o / 8;
-    }
-    bx = L % gridDim.x;
-    by = L / gridDim.x;
 }
 
 // the frames of one vertex in one chunk, contiguous: one vector load per vertex
