@@ -71,6 +71,29 @@ __device__ __forceinline__ int4 tet_vertices(long long tid, int nx, int ny) {
     return int4{v[0], v[1], v[2], v[3]};
 }
 
+// A tetrahedron in the bin lists: key = base vertex * 8 + t, base = cx + cy nx + cz ny nx of
+// its cell.  Keys order exactly as the ids cell * 6 + t do (cells in (cz, cy, cx) order, then
+// t), so the lowest passing id is the lowest passing key, and the vertices follow from the
+// key without divisions.
+__device__ __forceinline__ int tet_key(long long tid, int nx, int ny) {
+    const long long cell = tid / 6;
+    const int t = (int)(tid - cell * 6);
+    const long long cx = cell % (nx - 1), cy = (cell / (nx - 1)) % (ny - 1),
+                    cz = cell / ((long long)(nx - 1) * (ny - 1));
+    return (int)((cx + cy * nx + cz * (long long)ny * nx) * 8 + t);
+}
+
+__device__ __forceinline__ int4 key_vertices(int key, int nx, int ny) {
+    const int base = key >> 3, t = key & 7;
+    int v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int code = kTet[t][c];
+        v[c] = base + (code & 1) + ((code >> 1) & 1) * nx + ((code >> 2) & 1) * ny * nx;
+    }
+    return int4{v[0], v[1], v[2], v[3]};
+}
+
 struct Box {
     float mn[3], sc[3];
 };
@@ -184,7 +207,7 @@ __global__ __launch_bounds__(256) void interp_fill_kernel(Box b, const float *__
             for (int x = lo[0]; x <= hi[0]; x++) {
                 const int64_t bin = x + kBins * (y + kBins * z);
                 const long long k = (long long)atomicAdd((unsigned long long *)cursor + bin, 1ull);
-                list[start[bin] + k] = (int)t;
+                list[start[bin] + k] = tet_key(t, nx, ny);
             }
 }
 
@@ -224,9 +247,21 @@ __device__ bool barycentric(const float *p, const float *v0, const float *v1, co
     if (fabsf(denom) < 1e-12f) return false;
     const float qc0 = q1 * c2 - q2 * c1, qc1 = q2 * c0 - q0 * c2, qc2 = q0 * c1 - q1 * c0;
     const float bq0 = b1 * q2 - b2 * q1, bq1 = b2 * q0 - b0 * q2, bq2 = b0 * q1 - b1 * q0;
-    w[1] = (q0 * bc0 + q1 * bc1 + q2 * bc2) / denom;
-    w[2] = (a0 * qc0 + a1 * qc1 + a2 * qc2) / denom;
-    w[3] = (a0 * bq0 + a1 * bq1 + a2 * bq2) / denom;
+    const float n1 = q0 * bc0 + q1 * bc1 + q2 * bc2;
+    const float n2 = a0 * qc0 + a1 * qc1 + a2 * qc2;
+    const float n3 = a0 * bq0 + a1 * bq1 + a2 * bq2;
+    // most candidates fail on a sign: a numerator of the other sign than denom with
+    // |n| >= 1e-30 |denom| gives a quotient below -1e-30 (never a rounded -0), which the
+    // test below rejects, so the divisions are skipped for it
+    const float lim = 1e-30f * fabsf(denom);
+    const bool neg = denom < 0.0f;
+    if (((n1 < 0.0f) != neg && n1 != 0.0f && fabsf(n1) >= lim) ||
+        ((n2 < 0.0f) != neg && n2 != 0.0f && fabsf(n2) >= lim) ||
+        ((n3 < 0.0f) != neg && n3 != 0.0f && fabsf(n3) >= lim))
+        return false;
+    w[1] = n1 / denom;
+    w[2] = n2 / denom;
+    w[3] = n3 / denom;
     w[0] = 1.0f - w[1] - w[2] - w[3];
     return w[0] >= 0.0f && w[1] >= 0.0f && w[2] >= 0.0f && w[3] >= 0.0f;
 }
@@ -242,15 +277,15 @@ __global__ __launch_bounds__(256) void interp_locate_kernel(
     if (i >= n) return;
     const float p[3] = {qp[3 * i], qp[3 * i + 1], qp[3 * i + 2]};
     int ix, iy, iz;
-    long long best = LLONG_MAX;
+    int best = INT_MAX;
     int4 bid{-1, -1, -1, -1};
     float bw[4] = {0, 0, 0, 0};
     if (ntet > 0 && quantize(b, p[0], p[1], p[2], ix, iy, iz)) {
         const int64_t bin = ix + kBins * (iy + kBins * (int64_t)iz);
         for (long long e = start[bin]; e < start[bin + 1]; e++) {
-            const long long t = list[e];
+            const int t = list[e];
             if (t >= best) continue;
-            const int4 id = tet_vertices(t, nx, ny);
+            const int4 id = key_vertices(t, nx, ny);
             float w[4];
             if (barycentric(p, v + 3 * (int64_t)id.x, v + 3 * (int64_t)id.y, v + 3 * (int64_t)id.z,
                             v + 3 * (int64_t)id.w, w)) {
@@ -263,7 +298,7 @@ __global__ __launch_bounds__(256) void interp_locate_kernel(
     }
     if (mode == FFDAS_INTERP_NEAREST) {
         int r = -1;
-        if (best != LLONG_MAX) {   // float4_max_by_key (spatial_hash_kernels.cuh:51-59)
+        if (best != INT_MAX) {   // float4_max_by_key (spatial_hash_kernels.cuh:51-59)
             const int ixy = bw[0] > bw[1] ? bid.x : bid.y;
             const float mxy = ixy == bid.x ? bw[0] : bw[1];
             const int izw = bw[2] > bw[3] ? bid.z : bid.w;
@@ -556,8 +591,8 @@ extern "C" ffdas_error_t ffdas_create_interpolation_plan(ffdas_handle_t handle,
     if (nx <= 0 || ny <= 0 || nz <= 0) return FFDAS_ERROR_INVALID_DIMS;
     if (mode != FFDAS_INTERP_NEAREST && mode != FFDAS_INTERP_LINEAR)
         return FFDAS_ERROR_INVALID_ARGUMENT;
-    // vertex ids are 32-bit in the lookup arrays, tetrahedron ids too
-    if (nx > INT_MAX / ny / nz || 6 * (nx - 1) * (ny - 1) * (nz - 1) > INT_MAX)
+    // vertex ids are 32-bit in the lookup arrays; the bin lists hold vertex * 8 + t
+    if (nx > INT_MAX / 8 / ny / nz || 6 * (nx - 1) * (ny - 1) * (nz - 1) > INT_MAX)
         return FFDAS_ERROR_DIMS_TOO_LARGE;
     FFDAS_TRY(handle->check_device());
     auto *p = new (std::nothrow) ffdas_interpolation_plan;
