@@ -397,7 +397,7 @@ def bench_interpolate(a):
         t0 = time.perf_counter()
         OI.interpolate(gs, xs, qs, mode="linear")
         secs = time.perf_counter() - t0
-        cpu = {"value": round(B * qs.shape[0] / secs / 1e9, 6), "unit": "Gsample/s",
+        cpu = {"value": float(f"{B * qs.shape[0] / secs / 1e9:.3g}"), "unit": "Gsample/s",
                "cores": 1, "kind": "port",
                "sample": "oracle interpolate (numpy, one thread) of 16 frames of a 64 x 24 x 24 "
                          "sector at 4000 points, plan and location included"}
