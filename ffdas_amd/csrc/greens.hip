@@ -1,9 +1,9 @@
 // Frequency-domain Green's function summation (ffdas_greens):
 //   out[b, t, f] = sum_s x[b, s, f] e^{i k_f r_ts} / r_ts,   r_ts = |dst_t - src_s|.
-// Fewer than 8 events (greens_kernel): one lane per target, its position and per-source
+// Fewer than 5 events (greens_kernel): one lane per target, its position and per-source
 // distance in registers; the source position and the x values of a (source, frequency
 // chunk, event chunk) are the same for the whole wave, so they come through the scalar unit;
-// each thread keeps FPT frequencies x BT events of complex sums on the VALU.  8 or more
+// each thread keeps FPT frequencies x BT events of complex sums on the VALU.  5 or more
 // events (greens_mfma_kernel): the sums on the MFMA, see there.  FP32 throughout (the input
 // may be stored as FP16).
 #include <hip/hip_runtime.h>
@@ -169,7 +169,10 @@ __global__ __launch_bounds__(256) void greens_mfma_kernel(const float *__restric
 template <typename XT>
 ffdas_error_t launch(hipStream_t st, const float *src, const float *wav, const void *x,
                      const float *dst, void *out, int nsrc, int nfreq, int ndst, int B) {
-    if (B >= 8) {
+    // each (target, source, frequency) sincos once where it pays (measured): the MFMA tile
+    // from 5 events (one 16-event tile instead of two VALU passes), the 4-event VALU kernel
+    // for 2-4, the 1-event one for 1
+    if (B >= 5) {
         const dim3 g((unsigned)((ndst + 63) / 64), (unsigned)((nfreq + kGreensFPW - 1) / kGreensFPW),
                      (unsigned)((B + 15) / 16));
         ProfScope ps(st, "greens");
@@ -181,7 +184,7 @@ ffdas_error_t launch(hipStream_t st, const float *src, const float *wav, const v
     const dim3 grid((unsigned)((ndst + 255) / 256), (unsigned)((nfreq + kGreensFPT - 1) / kGreensFPT),
                     1);
     ProfScope ps(st, "greens");
-    if (B >= 4) {
+    if (B >= 2) {
         const dim3 g(grid.x, grid.y, (unsigned)((B + 3) / 4));
         hipLaunchKernelGGL((greens_kernel<XT, 4>), g, dim3(256), 0, st, src, wav,
                            static_cast<const XT *>(x), dst, static_cast<float2 *>(out), nsrc,
