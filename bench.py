@@ -11,6 +11,7 @@ counted once per forward+adjoint step.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--storage fp16|bf16|fp32]
   python bench.py --profile-only N   # N steps + a 256 MiB copy, for rocprofv3 --pmc runs
+  python bench.py --config 2         # volumetric 3-D das, the largest single call (3.4e10 U)
   python bench.py --config 5         # batched sparse compounding, three probes, one graph
   python bench.py --op truncate_rank # clutter filter (rank truncation) of a frame ensemble
   python bench.py --op interpolate   # scan conversion of a sector volume to a Cartesian grid
@@ -51,8 +52,10 @@ def parse():
                     help="voxel rows (of 256) in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--config", type=int, default=4, choices=[4, 5],
-                    help="4: fwd+adjoint headline (default); 5: batched sparse compounding")
+    ap.add_argument("--config", type=int, default=4, choices=[2, 4, 5],
+                    help="4: fwd+adjoint headline (default); 2: volumetric 3-D das (FP32, "
+                         "1024 channels, 16 diverging waves, 128^3 voxels); 5: batched sparse "
+                         "compounding")
     ap.add_argument("--op", default="das",
                     choices=["das", "truncate_rank", "interpolate", "greens"],
                     help="das (default): the delay-and-sum benchmarks; truncate_rank: the "
@@ -207,31 +210,203 @@ def profile_only(p, x, n):
     log(f"profile-only: {n} steps and the calibration copy done")
 
 
-def cpu_baseline(w, rows):
+def lds_peak():
+    """Conflict-free 64-bit LDS atomic rate of this device [lane-ops/clk/CU], measured live
+    by the calibration probe (ffdas_amd/calib/lds_rate.hip: every lane of 8 waves per
+    workgroup adds at its own bank pair), or None when the probe library is missing."""
+    try:
+        from ffdas_amd.utils import calib
+        return calib.lds_atomic_rate("u64")
+    except (ImportError, RuntimeError, OSError) as e:
+        log(f"LDS calibration probe unavailable: {e}")
+        return None
+
+
+def roofline(a, p, kt, U, tuned):
+    """Roofline object of the dominant kernel.  frac = HBM bytes per launch from the counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated) / live kernel time / 8 TB/s.  The
+    algorithmic bytes (gathered samples or FP32 scatter-adds per unit) are reported apart:
+    L1/L2 and LDS serve them, so they do not bound against HBM.  onchip: the resource that
+    binds the kernel, against its measured peak."""
+    esz = p.x.element_size()
+    bytes_per_u = {"das_fwd_fi": 2 * esz, "das_fwd": 2 * esz, "das_bwd_param_fi": 2 * esz,
+                   "das_bwd_param": 2 * esz, "das_bwd_x": 2 * 4, "das_bwd_x_plain": 2 * 4}
+    dom = max((k for k in kt if k in bytes_per_u), key=kt.get)
+    t = kt[dom] * 1e-3
+    traffic = traffic_for(a.storage, dom)
+    hbm = None if traffic is None else traffic / t / 1e9
+    alg = U * bytes_per_u[dom]
+    cus, ghz = 256, 2.4
+    if dom == "das_bwd_x":
+        # per unit: one 64-bit LDS atomic (a frame pair at each of the two samples) plus one
+        # 32-bit count per tap and frame chunk of FS frames
+        fs = bwdx_frames_used(a, tuned)
+        ops = U * (1.0 + 1.0 / fs)
+        cal = lds_peak()
+        if cal is not None:
+            cus, ghz = cal["cus"], cal["clock_ghz"]
+        peak = round(cal["lane_ops_per_clk_cu"], 2) if cal else 10.05
+        onchip = {"resource": "LDS 64-bit atomics", "unit": "lane-ops/clk/CU",
+                  "achieved": round(ops / t / (ghz * 1e9 * cus), 2), "peak": peak,
+                  "peak_source": "measured now: ffdas_calib_lds_atomics, distinct bank pairs"
+                  if cal else "ffdas_calib_lds_atomics on gfx950 (earlier run)"}
+    else:
+        onchip = {"resource": "vector L1", "unit": "B/clk/CU",
+                  "achieved": round(alg / t / (ghz * 1e9 * cus), 1), "peak": 64.0,
+                  "peak_source": "gfx950 vector L1 line rate, 64 B/clk/CU"}
+    onchip["frac"] = round(onchip["achieved"] / onchip["peak"], 3)
+    return {"bound": "hbm", "kernel": dom,
+            "achieved": None if hbm is None else round(hbm, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": None if hbm is None else round(hbm / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "traffic_source": f"profiles/das_config4_{a.storage}_traffic.json",
+            "algorithmic": {"bytes_per_unit": bytes_per_u[dom], "units_per_launch": U,
+                            "bytes_per_launch": alg, "GBps": round(alg / t / 1e9, 1),
+                            "vs_hbm_peak": round(alg / t / 1e9 / HBM_PEAK_GBS, 3)},
+            "onchip": onchip,
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(w, rows, single_rows=16):
     """The FP32 build of the oracle on the host cores: forward + adjoint (grad x, offsets,
     scale) over a bounded sample (the first `rows` voxel rows of the grid, all frames,
-    channels and transmits)."""
+    channels and transmits) on all host threads, median of 5; and the same on one thread over
+    the first `single_rows` rows, median of 3."""
     import oracle
     nthreads = oracle.hardware_threads()
-    dst = np.ascontiguousarray(w.dstpos[:rows])
-    ofs = np.ascontiguousarray(w.offsets[:, :rows])
-    wt = np.ascontiguousarray(w.weights[:, :rows])
-    B, C, T, S = w.x.shape
-    U = B * C * T * dst.shape[0] * dst.shape[1]
-    g = np.random.default_rng(0).standard_normal((B,) + dst.shape[:-1]).astype(np.float32)
-    times = []
-    oracle.das(w.x, w.srcpos, dst, ofs, wt, bits=32, threads=nthreads)     # warm-up
-    for _ in range(5):
-        t0 = time.perf_counter()
-        oracle.das(w.x, w.srcpos, dst, ofs, wt, bits=32, threads=nthreads)
-        oracle.das_adjoint(w.x, w.srcpos, dst, ofs, wt, g, bits=32, threads=nthreads)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": U / t / 1e9, "unit": "Gvoxel·ch·tx/s", "cores": nthreads,
-            "kind": "port",
-            "sample": f"FP32 oracle fwd+adjoint on {dst.shape[0]}x{dst.shape[1]} of the "
-                      f"256x256 voxels, all 16 frames/128 ch/11 tx ({U:.3g} U), median of 5, "
-                      f"{nthreads} threads"}
+
+    def run(nrows, threads, reps):
+        dst = np.ascontiguousarray(w.dstpos[:nrows])
+        ofs = np.ascontiguousarray(w.offsets[:, :nrows])
+        wt = np.ascontiguousarray(w.weights[:, :nrows])
+        B, C, T, S = w.x.shape
+        U = B * C * T * dst.shape[0] * dst.shape[1]
+        g = np.random.default_rng(0).standard_normal((B,) + dst.shape[:-1]).astype(np.float32)
+        oracle.das(w.x, w.srcpos, dst, ofs, wt, bits=32, threads=threads)     # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            oracle.das(w.x, w.srcpos, dst, ofs, wt, bits=32, threads=threads)
+            oracle.das_adjoint(w.x, w.srcpos, dst, ofs, wt, g, bits=32, threads=threads)
+            times.append(time.perf_counter() - t0)
+        return U, U / float(np.median(times)) / 1e9
+
+    U, v = run(rows, nthreads, 5)
+    U1, v1 = run(single_rows, 1, 3)
+    return {"value": v, "unit": "Gvoxel·ch·tx/s", "cores": nthreads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "single_core": {"value": v1, "cores": 1,
+                            "sample": f"first {single_rows} of 256 voxel rows ({U1:.3g} U), "
+                                      f"median of 3"},
+            "sample": f"FP32 oracle fwd+adjoint on {rows}x256 of the 256x256 voxels, all 16 "
+                      f"frames/128 ch/11 tx ({U:.3g} U), median of 5, {nthreads} threads"}
+
+
+def bench_config2(a):
+    """Plain volumetric das (FP32): 32x32 matrix array (1024 channels), 16 diverging waves
+    from virtual sources 10 mm behind the array, 4096 samples, 128^3 voxels over +-15 mm x
+    5-50 mm, directivity cone 30 deg and Hann angular weights: the largest single call of the
+    benchmark set (3.436e10 units).  One step = one forward call; synthesised echoes of 50
+    scatterers."""
+    import torch
+    import ffdas_amd
+    from ffdas_amd import workloads as W
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    w = W.config2()
+    log(f"config 2 generated in {time.time() - t0:.1f}s: x {w.x.shape}, grid "
+        f"{w.dstpos.shape[:-1]}, U {w.units:.4g}")
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+    x, srcpos, dstpos = t(w.x), t(w.srcpos), t(w.dstpos)
+    offsets, weights, srcdir = t(w.offsets), t(w.weights), t(w.srcdir)
+    run = lambda: ffdas_amd.das(x, srcpos, dstpos, offsets, weights, srcdir=srcdir)
+    if a.profile_only:
+        for _ in range(a.profile_only):
+            run()
+        calibration_copy()
+        log(f"profile-only: {a.profile_only} calls and the calibration copy done")
+        return
+    steps = max(2, min(a.steps, 10))
+    ms, wall = time_steps(run, steps, max(1, a.warmup))
+    U = w.units
+    value = U / (ms * 1e-3) / 1e9
+    from ffdas_amd.utils import profiling
+    with profiling.profiled():
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+    kt = {k: v["avg_ms"] for k, v in profiling.kernel_stats().items()}
+    log(f"config 2: {ms:.2f} ms per call, {value:.1f} G U/s; kernels {kt}")
+    dom = max(kt, key=kt.get)
+    tk = kt[dom] * 1e-3
+    traffic = traffic_for("fp32", dom, summary="das_config2_fp32_traffic.json")
+    hbm = None if traffic is None else traffic / tk / 1e9
+    alg = U * 8
+    # the transmit tables (offsets, weights: 16 x 128^3 FP32 each) are read once per call
+    tables = 2 * w.offsets.size * 4
+    roof = {"bound": "hbm", "kernel": dom,
+            "achieved": None if hbm is None else round(hbm, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": None if hbm is None else round(hbm / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "traffic_source": "profiles/das_config2_fp32_traffic.json",
+            "algorithmic": {"bytes_per_unit": 8, "units_per_launch": U, "bytes_per_launch": alg,
+                            "GBps": round(alg / tk / 1e9, 1),
+                            "vs_hbm_peak": round(alg / tk / 1e9 / HBM_PEAK_GBS, 3)},
+            "transmit_table_bytes": tables,
+            "onchip": {"resource": "vector L1", "unit": "B/clk/CU",
+                       "achieved": round(alg / tk / (2.4e9 * 256), 1), "peak": 64.0,
+                       "frac": round(alg / tk / (2.4e9 * 256) / 64.0, 3),
+                       "peak_source": "gfx950 vector L1 line rate, 64 B/clk/CU"},
+            "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+    cpu = None
+    if not a.no_cpu:
+        import oracle
+        nz = 4            # a slab of 4 of the 128 depth planes, all lateral voxels
+        dst = np.ascontiguousarray(w.dstpos[:nz])
+        ofs = np.ascontiguousarray(w.offsets[:, :nz])
+        wts = np.ascontiguousarray(w.weights[:, :nz])
+        n = int(np.prod(dst.shape[:-1])) * w.x.shape[1] * w.offsets.shape[0]
+        oracle.das(w.x, w.srcpos, dst[:1, :8], ofs[:, :1, :8], wts[:, :1, :8],
+                   srcdir=w.srcdir, bits=32)
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            oracle.das(w.x, w.srcpos, dst, ofs, wts, srcdir=w.srcdir, bits=32)
+            ts.append(time.perf_counter() - t1)
+        d1, o1, w1 = dst[:1, :16], ofs[:, :1, :16], wts[:, :1, :16]
+        n1 = int(np.prod(d1.shape[:-1])) * w.x.shape[1] * w.offsets.shape[0]
+        t1 = time.perf_counter()
+        oracle.das(w.x, w.srcpos, d1, o1, w1, srcdir=w.srcdir, bits=32, threads=1)
+        v1 = n1 / (time.perf_counter() - t1) / 1e9
+        cpu = {"value": n / float(np.median(ts)) / 1e9, "unit": "Gvoxel·ch·tx/s",
+               "cores": oracle.hardware_threads(), "kind": "port", "cpu_model": cpu_model(),
+               "single_core": {"value": v1, "cores": 1,
+                               "sample": f"16 voxels x 1024 ch x 16 tx ({n1:.3g} U), one run"},
+               "sample": f"FP32 oracle forward on {nz} of the 128 depth planes (all 128x128 "
+                         f"lateral voxels, 1024 ch, 16 tx: {n:.3g} U), median of 3"}
+        log(f"cpu baseline: {cpu}")
+    line = {"metric": "Gvoxel·ch·tx/s volumetric das forward (FP32)", "value": round(value, 2),
+            "unit": "Gvoxel·ch·tx/s", "n_gpus": 1, "steps": steps, "warmup": max(1, a.warmup),
+            "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "3-D das: 32x32 matrix array (1024 ch), 16 diverging waves, "
+                                   "4096 samples, 128^3 voxels, directivity 30 deg, Hann "
+                                   "weights", "storage": "fp32", "units_per_step": U,
+                       "wall_s": round(wall, 3)},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
 
 
 def bench_config5(a):
@@ -600,6 +775,9 @@ def main():
     if a.config == 5:
         bench_config5(a)
         return
+    if a.config == 2:
+        bench_config2(a)
+        return
     import torch
     import ffdas_amd
     from ffdas_amd import workloads as W
@@ -643,35 +821,9 @@ def main():
 
     kt = kernel_times(p, p.x, max(3, a.steps // 2))
     log("kernel ms: " + ", ".join(f"{k}={v:.3f}" for k, v in kt.items()))
-    esz = p.x.element_size()
-    # algorithmic bytes per unit: the two gathered samples (forward, parameter adjoint) or
-    # the two FP32 scatter-adds (grad_x adjoint)
-    bytes_per_u = {"das_fwd_fi": 2 * esz, "das_fwd": 2 * esz, "das_bwd_param_fi": 2 * esz,
-                   "das_bwd_param": 2 * esz, "das_bwd_x": 2 * 4, "das_bwd_x_plain": 2 * 4}
-    dom = max((k for k in kt if k in bytes_per_u), key=kt.get)
-    achieved = U * bytes_per_u[dom] / (kt[dom] * 1e-3) / 1e9
-    # on-chip figure of the dominant kernel: grad_x is bound by 64-bit LDS atomics (per tap
-    # and frame chunk of FS frames: FS/2 frame pairs at each of the two samples, plus one
-    # 32-bit count, i.e. 1 + 1/FS per unit),
-    # measured on gfx950 at 8.9 lane-ops/clk/CU without bank conflicts; the gathers by the
-    # vector L1 (64 B/clk/CU), 2 x sizeof(storage) bytes per unit
-    clk, cus = 2.4e9, 256
-    if dom == "das_bwd_x":
-        fs = bwdx_frames_used(a, tuned)
-        ops = U * (1.0 + 1.0 / fs)
-        onchip = {"resource": "LDS 64-bit atomics", "unit": "lane-ops/clk/CU",
-                  "achieved": round(ops / (kt[dom] * 1e-3) / (clk * cus), 2), "peak": 8.9}
-    else:
-        onchip = {"resource": "vector L1", "unit": "B/clk/CU",
-                  "achieved": round(U * bytes_per_u[dom] / (kt[dom] * 1e-3) / (clk * cus), 1),
-                  "peak": 64.0}
-    onchip["frac"] = round(onchip["achieved"] / onchip["peak"], 3)
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "onchip": onchip,
-            "traffic": traffic_for(a.storage, dom), "bytes_per_unit": bytes_per_u[dom],
-            "units_per_launch": U,
-            "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+    kt32 = kernel_times(p, p.x32, max(3, a.steps // 2))
+    log("fp32 tuned kernel ms: " + ", ".join(f"{k}={v:.3f}" for k, v in kt32.items()))
+    roof = roofline(a, p, kt, U, tuned)
 
     # the CPU leg: the oracle timed on the host cores (FP32 build), and its FP64 build as the
     # checker of frame 0
@@ -703,6 +855,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    line["config"]["fp32_tuned_kernel_ms"] = {k: round(v, 4) for k, v in kt32.items()}
     print(json.dumps(line), flush=True)
 
 

@@ -23,34 +23,109 @@ from ._lib import DType, check, lib
 # --------------------------------------------------------------------------------------
 @dataclass(frozen=True)
 class ElementType:
-    """A library element type usable with :func:`view`."""
+    """A library element type usable with :func:`view` (reference _ffdas.cpp:57-70)."""
     name: str
     code: int
     base: torch.dtype          # dtype of the underlying torch tensor
     pair: bool                 # underlying tensor carries a trailing dimension of 2
+    dl_code: int = 2           # DLPack type code of one element (0 int, 2 float, 4 bfloat,
+    dl_bits: int = 32          # 5 complex), bits per lane and lanes
+    dl_lanes: int = 1
 
     def __repr__(self):
         return f"ffdas_amd.{self.name}"
 
 
-half = ElementType("half", DType.R_16F, torch.float16, False)
-bfloat16 = ElementType("bfloat16", DType.R_16BF, torch.bfloat16, False)
-float32 = ElementType("float32", DType.R_32F, torch.float32, False)
-float64 = ElementType("float64", DType.R_64F, torch.float64, False)
-half2 = ElementType("half2", DType.C_16F, torch.float16, True)
-bfloat162 = ElementType("bfloat162", DType.C_16BF, torch.bfloat16, True)
-float2 = ElementType("float2", DType.C_32F, torch.float32, True)
-double2 = ElementType("double2", DType.C_64F, torch.float64, True)
+_DL_INT, _DL_FLOAT, _DL_BFLOAT, _DL_COMPLEX = 0, 2, 4, 5
+
+half = ElementType("half", DType.R_16F, torch.float16, False, _DL_FLOAT, 16)
+bfloat16 = ElementType("bfloat16", DType.R_16BF, torch.bfloat16, False, _DL_BFLOAT, 16)
+float32 = ElementType("float32", DType.R_32F, torch.float32, False, _DL_FLOAT, 32)
+float64 = ElementType("float64", DType.R_64F, torch.float64, False, _DL_FLOAT, 64)
+int16 = ElementType("int16", DType.R_16I, torch.int16, False, _DL_INT, 16)
+int32 = ElementType("int32", DType.R_32I, torch.int32, False, _DL_INT, 32)
+half2 = ElementType("half2", DType.C_16F, torch.float16, True, _DL_FLOAT, 16, 2)
+bfloat162 = ElementType("bfloat162", DType.C_16BF, torch.bfloat16, True, _DL_BFLOAT, 16, 2)
+float2 = ElementType("float2", DType.C_32F, torch.float32, True, _DL_FLOAT, 32, 2)
+double2 = ElementType("double2", DType.C_64F, torch.float64, True, _DL_FLOAT, 64, 2)
+short2 = ElementType("short2", DType.C_16I, torch.int16, True, _DL_INT, 16, 2)
+int2 = ElementType("int2", DType.C_32I, torch.int32, True, _DL_INT, 32, 2)
+complex64 = ElementType("complex64", DType.C_32F, torch.complex64, False, _DL_COMPLEX, 64)
+complex128 = ElementType("complex128", DType.C_64F, torch.complex128, False, _DL_COMPLEX, 128)
 
 _TORCH_CODE = {
     torch.float16: DType.R_16F,
     torch.bfloat16: DType.R_16BF,
     torch.float32: DType.R_32F,
     torch.float64: DType.R_64F,
+    torch.int16: DType.R_16I,
+    torch.int32: DType.R_32I,
     torch.complex32: DType.C_16F,
     torch.complex64: DType.C_32F,
     torch.complex128: DType.C_64F,
 }
+
+
+# --------------------------------------------------------------------------------------
+# DLPack export (the legacy "dltensor" capsule, DLPack 0.x ABI)
+# --------------------------------------------------------------------------------------
+class _DLDevice(ctypes.Structure):
+    _fields_ = [("device_type", ctypes.c_int32), ("device_id", ctypes.c_int32)]
+
+
+class _DLDataType(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16)]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device", _DLDevice), ("ndim", ctypes.c_int32),
+                ("dtype", _DLDataType), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManagedTensor(ctypes.Structure):
+    pass
+
+
+_DL_DELETER = ctypes.CFUNCTYPE(None, ctypes.POINTER(_DLManagedTensor))
+_DLManagedTensor._fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p),
+                             ("deleter", _DL_DELETER)]
+
+KDL_CPU, KDL_ROCM = 1, 10
+# exported tensors alive until their consumer (or the unconsumed capsule) calls the deleter:
+# address of the DLManagedTensor -> (struct, shape array, strides array, base tensor)
+_dl_live: dict[int, tuple] = {}
+
+
+@_DL_DELETER
+def _dl_deleter(mt):
+    _dl_live.pop(ctypes.addressof(mt.contents), None)
+
+
+_CAPSULE_DTOR = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+# private prototypes: the attributes of ctypes.pythonapi.* are shared with other code
+_capsule_new = ctypes.PYFUNCTYPE(ctypes.py_object, ctypes.c_void_p, ctypes.c_char_p,
+                                 _CAPSULE_DTOR)(("PyCapsule_New", ctypes.pythonapi))
+_capsule_valid = ctypes.PYFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p)(
+    ("PyCapsule_IsValid", ctypes.pythonapi))
+_capsule_ptr = ctypes.PYFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p)(
+    ("PyCapsule_GetPointer", ctypes.pythonapi))
+
+
+@_CAPSULE_DTOR
+def _capsule_dtor(cap):
+    # an unconsumed capsule keeps its name "dltensor": release the export; a consumer renames
+    # it ("used_dltensor") and calls the deleter itself
+    if _capsule_valid(cap, b"dltensor"):
+        addr = _capsule_ptr(cap, b"dltensor")
+        if addr:
+            _dl_live.pop(addr, None)
+
+
+def dlpack_device_of(t: torch.Tensor) -> tuple[int, int]:
+    if t.device.type == "cuda":
+        return KDL_ROCM, t.device.index if t.device.index is not None else 0
+    return KDL_CPU, 0
 
 
 class TensorView:
@@ -90,7 +165,39 @@ class TensorView:
         return self.base.data_ptr()
 
     def __repr__(self):
-        return f"TensorView(shape={self.shape}, dtype={self.dtype})"
+        shp = "(" + ", ".join(map(str, self.shape)) + ("," if len(self.shape) == 1 else "") + ")"
+        return f"TensorView(shape={shp}, dtype={self.dtype!r})"
+
+    def __dlpack_device__(self) -> tuple[int, int]:
+        """(device type, device id): kDLROCM (10) for a ROCm tensor (the reference reports
+        kDLCUDA for its CUDA arrays, _ffdas.cpp:229-231)."""
+        return dlpack_device_of(self.base)
+
+    def __dlpack__(self, stream=None, **kwargs):
+        """Export the view as a DLPack capsule with the view's element type (reference
+        _ffdas.cpp:232-275): complex pair types as two-lane vectors (half2: float16 x 2),
+        complex64/complex128 as DLPack complex, the view's shape and its element strides.  The
+        base tensor stays alive until the consumer releases the capsule.  The data is not
+        copied or synchronised: `stream` is accepted and ignored, as work on the base tensor
+        is already ordered on torch's current stream."""
+        nd = len(self.shape)
+        shape = (ctypes.c_int64 * max(nd, 1))(*self.shape)
+        strides = (ctypes.c_int64 * max(nd, 1))(*self.strides)
+        mt = _DLManagedTensor()
+        mt.dl_tensor.data = self.base.data_ptr()
+        dt, di = self.__dlpack_device__()
+        mt.dl_tensor.device = _DLDevice(dt, di)
+        mt.dl_tensor.ndim = nd
+        mt.dl_tensor.dtype = _DLDataType(self.dtype.dl_code, self.dtype.dl_bits,
+                                         self.dtype.dl_lanes)
+        mt.dl_tensor.shape = ctypes.cast(shape, ctypes.POINTER(ctypes.c_int64))
+        mt.dl_tensor.strides = ctypes.cast(strides, ctypes.POINTER(ctypes.c_int64))
+        mt.dl_tensor.byte_offset = 0
+        mt.manager_ctx = None
+        mt.deleter = _dl_deleter
+        addr = ctypes.addressof(mt)
+        _dl_live[addr] = (mt, shape, strides, self.base)
+        return _capsule_new(addr, b"dltensor", _capsule_dtor)
 
 
 def view(a, dtype: ElementType) -> TensorView:

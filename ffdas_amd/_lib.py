@@ -59,6 +59,12 @@ class Algorithm(enum.IntEnum):
     ALG4 = 4
 
 
+class InterpMode(enum.IntEnum):
+    """ffdas_interp_mode_t; the reference's InterpMode (_ffdas.cpp:315-317)."""
+    NEAREST = 0
+    LINEAR = 1
+
+
 class DasProblem(ctypes.Structure):
     """ffdas_das_problem_t (include/ffdas.h)."""
     _fields_ = [("srcpos", ctypes.c_void_p), ("srcdir", ctypes.c_void_p),

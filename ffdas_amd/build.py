@@ -19,6 +19,9 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libffdas.so"
+# measurement probe (LDS atomic rate), a library of its own: not part of the C-ABI
+CALIB_SRC = PKG / "calib" / "lds_rate.hip"
+CALIB_LIB = PKG / "calib" / "libffdas_calib.so"
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -58,20 +61,35 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-           *map(str, objs), "-Wl,--no-undefined", "-lamdhip64"]
+           *map(str, objs), "-Wl,--no-undefined", "-lamdhip64", "-ldl"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_calib(verbose)
     return LIB
+
+
+def build_calib(verbose: bool = False) -> Path:
+    """Compile the LDS atomic-rate probe into its own shared library."""
+    tmp = CALIB_LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, *COMMON, *DEVICE, "-shared", "-x", "hip", str(CALIB_SRC), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"calibration probe build failed\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CALIB_LIB)
+    return CALIB_LIB
 
 
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
-    if LIB.exists():
-        LIB.unlink()
+    for lib in (LIB, CALIB_LIB):
+        if lib.exists():
+            lib.unlink()
 
 
 if __name__ == "__main__":

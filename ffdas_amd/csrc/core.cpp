@@ -3,13 +3,77 @@
 // src/core/tensor.cu:9-42, src/core/type_utils.cu:5-30, src/core/error_utils.cu:3-87)
 // re-implemented on the HIP runtime: no BLAS/solver handles, a stream-ordered HIP
 // allocation for workspace instead of a caching allocator.
+#include <dlfcn.h>
+
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "internal.h"
 
 namespace ffdas {
+
+// ---- named ranges and the debug check around C-ABI calls ------------------------------
+// The reference brackets every entry point with an NVTX range (src/core/context.cuh:55-77,
+// e.g. src/das/das.cu:37) and, when built with FFDAS_DEBUG, synchronises after its launches
+// (src/core/error_checking.h:115-126).  Here both are run-time switches: ranges go to roctx
+// (librocprofiler-sdk-roctx, loaded on first use; rocprofv3 --marker-trace records them)
+// unless FFDAS_ROCTX=0, and FFDAS_DEBUG=1 makes every entry point wait for its stream and
+// report an asynchronous failure as FFDAS_ERROR_CUDA_RUNTIME (not while the stream is being
+// captured into a graph, where waiting is not allowed).
+namespace {
+struct Roctx {
+    int (*push)(const char *) = nullptr;
+    int (*pop)() = nullptr;
+};
+
+const Roctx &roctx() {
+    static Roctx r;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *off = std::getenv("FFDAS_ROCTX");
+        if (off && std::strcmp(off, "0") == 0) return;
+        void *lib = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+        if (!lib) lib = dlopen("librocprofiler-sdk-roctx.so", RTLD_NOW | RTLD_LOCAL);
+        if (!lib) return;
+        auto push = reinterpret_cast<int (*)(const char *)>(dlsym(lib, "roctxRangePushA"));
+        auto pop = reinterpret_cast<int (*)()>(dlsym(lib, "roctxRangePop"));
+        if (push && pop) {
+            r.push = push;
+            r.pop = pop;
+        }
+    });
+    return r;
+}
+}  // namespace
+
+bool debug_sync_enabled() {
+    const char *e = std::getenv("FFDAS_DEBUG");
+    return e && *e && std::strcmp(e, "0") != 0;
+}
+
+CallScope::CallScope(const ffdas_context *h, const char *name) : h_(h) {
+    const Roctx &r = roctx();
+    if (r.push) {
+        r.push(name);
+        pushed_ = true;
+    }
+}
+
+CallScope::~CallScope() {
+    if (pushed_) roctx().pop();
+}
+
+ffdas_error_t CallScope::done(ffdas_error_t status) const {
+    if (status != FFDAS_SUCCESS || !h_ || !debug_sync_enabled()) return status;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h_->stream, &cs) != hipSuccess) return FFDAS_ERROR_CUDA_RUNTIME;
+    if (cs != hipStreamCaptureStatusNone) return status;
+    if (hipStreamSynchronize(h_->stream) != hipSuccess) return FFDAS_ERROR_CUDA_RUNTIME;
+    if (hipGetLastError() != hipSuccess) return FFDAS_ERROR_CUDA_RUNTIME;
+    return status;
+}
 
 ffdas_error_t hip_status(hipError_t e, ffdas_error_t kind) {
     if (e == hipSuccess) return FFDAS_SUCCESS;

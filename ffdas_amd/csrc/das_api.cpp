@@ -1,6 +1,7 @@
 // C-ABI entry points of the delay-and-sum path: ffdas_das, ffdas_das_sparse (reference
 // src/das/das.cu:9-83, src/das/das_sparse.cu:6-80), the scaled/sparse forward ffdas_das_ex,
-// the adjoint ffdas_das_backward and ffdas_contiguous_copy.
+// the adjoint ffdas_das_backward and the batched forward ffdas_das_batched
+// (ffdas_contiguous_copy is in copy.hip).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -142,7 +143,7 @@ using namespace ffdas;
 
 extern "C" {
 
-ffdas_error_t ffdas_das(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+static ffdas_error_t das_call(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
                         float wavenum, ffdas_tensor_desc_t x_desc, const void *x,
                         const float *dstpos, const float *offsets, const float *weights,
                         const void *beta, ffdas_tensor_desc_t out_desc, void *out,
@@ -151,7 +152,17 @@ ffdas_error_t ffdas_das(ffdas_handle_t handle, const float *srcpos, const float 
                               weights, false, 0, nullptr, beta, out_desc, out, compute_type, alg);
 }
 
-ffdas_error_t ffdas_das_sparse(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+ffdas_error_t ffdas_das(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                        float wavenum, ffdas_tensor_desc_t x_desc, const void *x,
+                        const float *dstpos, const float *offsets, const float *weights,
+                        const void *beta, ffdas_tensor_desc_t out_desc, void *out,
+                        ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    ffdas::CallScope cs(handle, "ffdas_das");
+    return cs.done(das_call(handle, srcpos, srcdir, wavenum, x_desc, x, dstpos, offsets, weights,
+                            beta, out_desc, out, compute_type, alg));
+}
+
+static ffdas_error_t das_sparse_call(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
                                float wavenum, ffdas_tensor_desc_t x_desc, const void *x,
                                const float *dstpos, const float *offsets, const float *weights,
                                int sparse_count, const int *sparse_indices, const void *beta,
@@ -162,7 +173,19 @@ ffdas_error_t ffdas_das_sparse(ffdas_handle_t handle, const float *srcpos, const
                               compute_type, alg);
 }
 
-ffdas_error_t ffdas_das_ex(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+ffdas_error_t ffdas_das_sparse(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                               float wavenum, ffdas_tensor_desc_t x_desc, const void *x,
+                               const float *dstpos, const float *offsets, const float *weights,
+                               int sparse_count, const int *sparse_indices, const void *beta,
+                               ffdas_tensor_desc_t out_desc, void *out,
+                               ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    ffdas::CallScope cs(handle, "ffdas_das_sparse");
+    return cs.done(das_sparse_call(handle, srcpos, srcdir, wavenum, x_desc, x, dstpos, offsets,
+                                   weights, sparse_count, sparse_indices, beta, out_desc, out,
+                                   compute_type, alg));
+}
+
+static ffdas_error_t das_ex_call(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
                            float wavenum, float scale, ffdas_tensor_desc_t x_desc, const void *x,
                            const float *dstpos, const float *offsets, const float *weights,
                            int sparse_count, const int *sparse_indices, const void *beta,
@@ -174,7 +197,19 @@ ffdas_error_t ffdas_das_ex(ffdas_handle_t handle, const float *srcpos, const flo
                               beta, out_desc, out, compute_type, alg);
 }
 
-ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+ffdas_error_t ffdas_das_ex(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                           float wavenum, float scale, ffdas_tensor_desc_t x_desc, const void *x,
+                           const float *dstpos, const float *offsets, const float *weights,
+                           int sparse_count, const int *sparse_indices, const void *beta,
+                           ffdas_tensor_desc_t out_desc, void *out,
+                           ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    ffdas::CallScope cs(handle, "ffdas_das_ex");
+    return cs.done(das_ex_call(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos, offsets,
+                               weights, sparse_count, sparse_indices, beta, out_desc, out,
+                               compute_type, alg));
+}
+
+static ffdas_error_t das_backward_call(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
                                  float wavenum, float scale, ffdas_tensor_desc_t x_desc,
                                  const void *x, const float *dstpos, const float *offsets,
                                  const float *weights, int sparse_count,
@@ -236,48 +271,20 @@ ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, con
                            grad_weights, grad_scale, gt, pws);
 }
 
-ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
-                                    const void *x, ffdas_tensor_desc_t out_desc, void *out) {
-    if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
-    FFDAS_TRY(handle->check_device());
-    if (!x_desc || !out_desc) return FFDAS_ERROR_NULL_POINTER;
-    if (!x || !out) return FFDAS_ERROR_NULL_POINTER;
-    const ffdas_tensor_desc &xd = *x_desc, &od = *out_desc;
-    if (xd.dims != od.dims) return FFDAS_ERROR_INVALID_DIMS;
-    // express the copy as ([B,]C,T,S) rows: fold to 4-D with a unit sample stride
-    if (xd.dims.empty() || xd.dims.size() > 4) return FFDAS_ERROR_INVALID_DIMS;
-    if (xd.strides.back() != 1) return FFDAS_ERROR_INVALID_ARGUMENT;
-    int64_t s = 1;
-    for (int i = (int)od.dims.size() - 1; i >= 0; i--) {
-        if (od.dims[i] > 1 && od.strides[i] != s) return FFDAS_ERROR_NON_CONTIGUOUS;
-        s *= od.dims[i];
-    }
-    DasProblem p;
-    bool cx = is_complex(xd.dtype), co = is_complex(od.dtype);
-    if (cx != co) return FFDAS_ERROR_UNSUPPORTED_TYPE;
-    auto st_of = [](ffdas_datatype_t t, Storage &o) {
-        switch (t) {
-            case FFDAS_R_16F: case FFDAS_C_16F: o = Storage::F16; return true;
-            case FFDAS_R_16BF: case FFDAS_C_16BF: o = Storage::BF16; return true;
-            case FFDAS_R_32F: case FFDAS_C_32F: o = Storage::F32; return true;
-            case FFDAS_R_64F: case FFDAS_C_64F: o = Storage::F64; return true;
-            default: return false;
-        }
-    };
-    if (!st_of(xd.dtype, p.x_storage) || !st_of(od.dtype, p.k_storage))
-        return FFDAS_ERROR_UNSUPPORTED_TYPE;
-    p.cplx = cx;
-    std::vector<int64_t> d(4, 1), st(4, 0);
-    const int nd = (int)xd.dims.size();
-    for (int i = 0; i < nd; i++) {
-        d[4 - nd + i] = xd.dims[i];
-        st[4 - nd + i] = xd.strides[i];
-    }
-    for (int i = 0; i < 4; i++)
-        if (d[i] > INT32_MAX || st[i] < 0) return FFDAS_ERROR_INVALID_DIMS;
-    p.B = (int)d[0]; p.C = (int)d[1]; p.T = (int)d[2]; p.S = (int)d[3];
-    p.xs_b = st[0]; p.xs_c = st[1]; p.xs_t = st[2];
-    return launch_cast(*handle, p, x, out);
+ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                                 float wavenum, float scale, ffdas_tensor_desc_t x_desc,
+                                 const void *x, const float *dstpos, const float *offsets,
+                                 const float *weights, int sparse_count,
+                                 const int *sparse_indices, ffdas_tensor_desc_t grad_out_desc,
+                                 const void *grad_out, ffdas_tensor_desc_t grad_x_desc,
+                                 void *grad_x, float *grad_offsets, float *grad_weights,
+                                 float *grad_scale, ffdas_compute_type_t compute_type,
+                                 ffdas_alg_t alg) {
+    ffdas::CallScope cs(handle, "ffdas_das_backward");
+    return cs.done(das_backward_call(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                     offsets, weights, sparse_count, sparse_indices,
+                                     grad_out_desc, grad_out, grad_x_desc, grad_x, grad_offsets,
+                                     grad_weights, grad_scale, compute_type, alg));
 }
 
 }  // extern "C"
@@ -285,7 +292,7 @@ ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x
 // ---------------------------------------------------------------------------------------
 // batched forward
 // ---------------------------------------------------------------------------------------
-extern "C" ffdas_error_t ffdas_das_batched(ffdas_handle_t handle, int count,
+static ffdas_error_t das_batched_call(ffdas_handle_t handle, int count,
                                           const ffdas_das_problem_t *problems,
                                           ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
     if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
@@ -348,6 +355,13 @@ extern "C" ffdas_error_t ffdas_das_batched(ffdas_handle_t handle, int count,
     return FFDAS_SUCCESS;
 }
 
+extern "C" ffdas_error_t ffdas_das_batched(ffdas_handle_t handle, int count,
+                                          const ffdas_das_problem_t *problems,
+                                          ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    ffdas::CallScope cs(handle, "ffdas_das_batched");
+    return cs.done(das_batched_call(handle, count, problems, compute_type, alg));
+}
+
 // ---------------------------------------------------------------------------------------
 // autotuner
 // ---------------------------------------------------------------------------------------
@@ -391,7 +405,7 @@ struct ForceGuard {
 }  // namespace
 }  // namespace ffdas
 
-extern "C" ffdas_error_t ffdas_das_autotune(
+static ffdas_error_t das_autotune_call(
     ffdas_handle_t handle, const float *srcpos, const float *srcdir, float wavenum, float scale,
     ffdas_tensor_desc_t x_desc, const void *x, const float *dstpos, const float *offsets,
     const float *weights, int sparse_count, const int *sparse_indices,
@@ -579,6 +593,18 @@ extern "C" ffdas_error_t ffdas_das_autotune(
         result->candidates = candidates;
     }
     return FFDAS_SUCCESS;
+}
+
+extern "C" ffdas_error_t ffdas_das_autotune(
+    ffdas_handle_t handle, const float *srcpos, const float *srcdir, float wavenum, float scale,
+    ffdas_tensor_desc_t x_desc, const void *x, const float *dstpos, const float *offsets,
+    const float *weights, int sparse_count, const int *sparse_indices,
+    ffdas_tensor_desc_t out_desc, void *out, ffdas_tensor_desc_t grad_x_desc, void *grad_x,
+    ffdas_compute_type_t compute_type, ffdas_alg_t alg, int reps, ffdas_das_tuning_t *result) {
+    ffdas::CallScope cs(handle, "ffdas_das_autotune");
+    return cs.done(das_autotune_call(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos,
+                                     offsets, weights, sparse_count, sparse_indices, out_desc,
+                                     out, grad_x_desc, grad_x, compute_type, alg, reps, result));
 }
 
 extern "C" ffdas_error_t ffdas_set_reuse_x(ffdas_handle_t handle, int enable) {

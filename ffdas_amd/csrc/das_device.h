@@ -76,7 +76,9 @@ struct Radius {
     float dx, dy, dz;   // voxel - channel
 };
 
-// Plain FP32: r = r2 * rsqrt(r2) as the reference does (das_alg1_kernels.cuh:64-66).
+// Plain FP32: r = r2 * rsqrt(r2) as the reference does (das_alg1_kernels.cuh:64-66).  Every
+// launcher instantiates the kernels with the compensated radius below (PRECISE = true); this
+// form is the one the CPU oracle's FP32 build restates.
 __device__ __forceinline__ Radius radius_fast(const SplitPt &y, const SplitPt &p) {
     Radius R;
     R.dx = y.x - p.x; R.dy = y.y - p.y; R.dz = y.z - p.z;

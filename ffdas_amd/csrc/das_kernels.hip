@@ -1685,7 +1685,7 @@ dev::Geo make_geo(const DasGeomArgs &a, const DasProblem &p) {
 template <Storage ST, bool ACC64, bool CPLX, int FT>
 hipError_t launch_fwd_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl, const XView &xv,
                          void *out, int64_t os_b, double br, double bi) {
-    constexpr bool PRECISE = (ST == Storage::F32 || ST == Storage::F64);
+    constexpr bool PRECISE = true;
     const int ntiles = tl.nta * tl.nth * tl.ntw;
     dim3 grid((ntiles + 3) / 4, (g.B + FT - 1) / FT);
     ProfScope ps(st, "das_fwd");
@@ -1728,7 +1728,7 @@ hipError_t finish_params(hipStream_t st, const ParamOut &po, int chunks, int64_t
 template <Storage ST, bool ACC64, bool CPLX, int FT>
 hipError_t launch_bwdp_ft(hipStream_t st, const dev::Geo &g, const dev::Tiling &tl,
                           const XView &xv, const void *gout, int64_t gs_b, const ParamOut &po) {
-    constexpr bool PRECISE = (ST == Storage::F32 || ST == Storage::F64);
+    constexpr bool PRECISE = true;
     const int ntiles = tl.nta * tl.nth * tl.ntw;
     dim3 grid((ntiles + 3) / 4, (g.B + FT - 1) / FT);
     const bool multi = grid.y > 1;
@@ -1882,7 +1882,7 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
     if constexpr (!fi_ok<ST, CPLX, F>()) {
         return hipErrorInvalidValue;
     } else {
-        constexpr bool PRECISE = (ST == Storage::F32);
+        constexpr bool PRECISE = true;
         const int ntiles = tl.nta * tl.nth * tl.ntw;
         dim3 grid(ntiles, (g.B + F - 1) / F);
         const int G = channel_groups(g, (int64_t)grid.x * grid.y, groups);
@@ -1915,7 +1915,7 @@ hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling
     if constexpr (!fi_ok<ST, CPLX, F>()) {
         return hipErrorInvalidValue;
     } else {
-        constexpr bool PRECISE = (ST == Storage::F32);
+        constexpr bool PRECISE = true;
         const int ntiles = tl.nta * tl.nth * tl.ntw;
         dim3 grid(ntiles, (g.B + F - 1) / F);
         const int G = channel_groups(g, (int64_t)grid.x * grid.y, groups);
@@ -1950,7 +1950,7 @@ hipError_t launch_fwd_batched_f(hipStream_t st, const std::vector<BatchMember> &
     if constexpr (!fi_ok<ST, CPLX, F>()) {
         return hipErrorInvalidValue;
     } else {
-        constexpr bool PRECISE = (ST == Storage::F32);
+        constexpr bool PRECISE = true;
         for (size_t off = 0; off < ms.size(); off += dev::kMaxBatch) {
             dev::BatchArgs args{};
             const int n = (int)std::min(ms.size() - off, (size_t)dev::kMaxBatch);
@@ -2298,7 +2298,6 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     const dev::Tiling tl = make_tiling(p.spatial, p.tune);
     hipStream_t st = h.stream;
     hipError_t e = hipSuccess;
-    const bool precise = p.k_storage == Storage::F32 || p.k_storage == Storage::F64;
 
     if (gofs || gwt || gscale) {
         const ParamOut po = param_out(p, gofs, gwt, gscale, pws);
@@ -2336,37 +2335,30 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
         if (p.acc64) {
             e = p.cplx ? launch_bwdx_plain_t<true, true, true>(st, g, tl, gout, gs_b, gx)
                        : launch_bwdx_plain_t<true, false, true>(st, g, tl, gout, gs_b, gx);
-        } else if (precise) {
+        } else {
             e = p.cplx ? launch_bwdx_plain_t<false, true, true>(st, g, tl, gout, gs_b, gx)
                        : launch_bwdx_plain_t<false, false, true>(st, g, tl, gout, gs_b, gx);
-        } else {
-            e = p.cplx ? launch_bwdx_plain_t<false, true, false>(st, g, tl, gout, gs_b, gx)
-                       : launch_bwdx_plain_t<false, false, false>(st, g, tl, gout, gs_b, gx);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     } else if (gx && use_sorted_bwdx(p)) {
         const int KPT = sorted_kpt(p);
         const int F = sorted_frames(KPT, p.cplx, p.B);
-        if (p.cplx)
-            e = precise ? launch_bwdx_sorted_t<true, true>(st, g, gout, gs_b, gt, gx, KPT, F)
-                        : launch_bwdx_sorted_t<true, false>(st, g, gout, gs_b, gt, gx, KPT, F);
-        else
-            e = precise ? launch_bwdx_sorted_t<false, true>(st, g, gout, gs_b, gt, gx, KPT, F)
-                        : launch_bwdx_sorted_t<false, false>(st, g, gout, gs_b, gt, gx, KPT, F);
+        e = p.cplx ? launch_bwdx_sorted_t<true, true>(st, g, gout, gs_b, gt, gx, KPT, F)
+                   : launch_bwdx_sorted_t<false, true>(st, g, gout, gs_b, gt, gx, KPT, F);
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     } else if (gx) {
         const int FS = bwdx_frames(p);
         if (FS <= 0) return FFDAS_ERROR_DIMS_TOO_LARGE;
         const dev::Axial ax = make_axial(p);
+        // grad_x never reads the channel data, so its delays are the compensated FP32 ones
+        // whatever the storage type: the scatter is bound by LDS atomics, and the extra
+        // delay arithmetic overlaps with them
         if (p.acc64) {
             e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
                        : launch_bwdx_t<true, false, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
-        } else if (precise) {
+        } else {
             e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
                        : launch_bwdx_t<false, false, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
-        } else {
-            e = p.cplx ? launch_bwdx_t<false, true, false>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, false>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }

@@ -212,7 +212,7 @@ bool packed(const ffdas_tensor_desc &d) {
 
 using namespace ffdas;
 
-extern "C" ffdas_error_t ffdas_greens(ffdas_handle_t handle, const float *srcpos,
+static ffdas_error_t greens_call(ffdas_handle_t handle, const float *srcpos,
                                       const float *wavenums, ffdas_tensor_desc_t x_desc,
                                       const void *x, const float *dstpos,
                                       ffdas_tensor_desc_t out_desc, void *out) {
@@ -239,4 +239,12 @@ extern "C" ffdas_error_t ffdas_greens(ffdas_handle_t handle, const float *srcpos
                                      (int)nfreq, (int)ndst, (int)B)
                    : launch<float2>(handle->stream, srcpos, wavenums, x, dstpos, out, (int)nsrc,
                                     (int)nfreq, (int)ndst, (int)B);
+}
+
+extern "C" ffdas_error_t ffdas_greens(ffdas_handle_t handle, const float *srcpos,
+                                      const float *wavenums, ffdas_tensor_desc_t x_desc,
+                                      const void *x, const float *dstpos,
+                                      ffdas_tensor_desc_t out_desc, void *out) {
+    ffdas::CallScope cs(handle, "ffdas_greens");
+    return cs.done(greens_call(handle, srcpos, wavenums, x_desc, x, dstpos, out_desc, out));
 }

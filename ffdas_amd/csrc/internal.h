@@ -111,6 +111,25 @@ ffdas_error_t hip_status(hipError_t e, ffdas_error_t kind = FFDAS_ERROR_CUDA_RUN
 
 }  // namespace ffdas
 
+namespace ffdas {
+// Scope of one C-ABI call: a named roctx range (if the roctx library is present and
+// FFDAS_ROCTX is not 0) and, with FFDAS_DEBUG set, done() waits for the handle's stream and
+// turns an asynchronous failure into FFDAS_ERROR_CUDA_RUNTIME.
+class CallScope {
+  public:
+    CallScope(const ffdas_context *h, const char *name);
+    ~CallScope();
+    CallScope(const CallScope &) = delete;
+    CallScope &operator=(const CallScope &) = delete;
+    ffdas_error_t done(ffdas_error_t status) const;
+
+  private:
+    const ffdas_context *h_;
+    bool pushed_ = false;
+};
+bool debug_sync_enabled();
+}  // namespace ffdas
+
 #define FFDAS_TRY(expr)                               \
     do {                                              \
         ffdas_error_t _e = (expr);                    \

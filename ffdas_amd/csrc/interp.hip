@@ -580,7 +580,7 @@ ffdas_error_t build_plan(ffdas_context &h, ffdas_interpolation_plan &p, const fl
 
 using namespace ffdas;
 
-extern "C" ffdas_error_t ffdas_create_interpolation_plan(ffdas_handle_t handle,
+static ffdas_error_t create_interpolation_plan_call(ffdas_handle_t handle,
                                                          ffdas_interpolation_plan_t *plan,
                                                          int64_t nx, int64_t ny, int64_t nz,
                                                          const float *gridpos,
@@ -613,7 +613,16 @@ extern "C" ffdas_error_t ffdas_create_interpolation_plan(ffdas_handle_t handle,
     return FFDAS_SUCCESS;
 }
 
-extern "C" ffdas_error_t ffdas_interpolation_preprocess(ffdas_handle_t handle,
+extern "C" ffdas_error_t ffdas_create_interpolation_plan(ffdas_handle_t handle,
+                                                         ffdas_interpolation_plan_t *plan,
+                                                         int64_t nx, int64_t ny, int64_t nz,
+                                                         const float *gridpos,
+                                                         ffdas_interp_mode_t mode) {
+    ffdas::CallScope cs(handle, "ffdas_create_interpolation_plan");
+    return cs.done(create_interpolation_plan_call(handle, plan, nx, ny, nz, gridpos, mode));
+}
+
+static ffdas_error_t interpolation_preprocess_call(ffdas_handle_t handle,
                                                         ffdas_interpolation_plan_t plan,
                                                         int64_t num_querypos,
                                                         const float *querypos) {
@@ -638,7 +647,15 @@ extern "C" ffdas_error_t ffdas_interpolation_preprocess(ffdas_handle_t handle,
     return FFDAS_SUCCESS;
 }
 
-extern "C" ffdas_error_t ffdas_interpolation(ffdas_handle_t handle, ffdas_interpolation_plan_t plan,
+extern "C" ffdas_error_t ffdas_interpolation_preprocess(ffdas_handle_t handle,
+                                                        ffdas_interpolation_plan_t plan,
+                                                        int64_t num_querypos,
+                                                        const float *querypos) {
+    ffdas::CallScope cs(handle, "ffdas_interpolation_preprocess");
+    return cs.done(interpolation_preprocess_call(handle, plan, num_querypos, querypos));
+}
+
+static ffdas_error_t interpolation_call(ffdas_handle_t handle, ffdas_interpolation_plan_t plan,
                                              int64_t num_querypos, const float *querypos,
                                              ffdas_tensor_desc_t x_desc, const void *x, void *out,
                                              const void *fill_value) {
@@ -696,6 +713,15 @@ extern "C" ffdas_error_t ffdas_interpolation(ffdas_handle_t handle, ffdas_interp
         if (err == FFDAS_SUCCESS) err = hip_status(e, FFDAS_ERROR_CUDA_SYNC_FAILED);
     }
     return err;
+}
+
+extern "C" ffdas_error_t ffdas_interpolation(ffdas_handle_t handle, ffdas_interpolation_plan_t plan,
+                                             int64_t num_querypos, const float *querypos,
+                                             ffdas_tensor_desc_t x_desc, const void *x, void *out,
+                                             const void *fill_value) {
+    ffdas::CallScope cs(handle, "ffdas_interpolation");
+    return cs.done(interpolation_call(handle, plan, num_querypos, querypos, x_desc, x, out,
+                                      fill_value));
 }
 
 extern "C" ffdas_error_t ffdas_destroy_interpolation_plan(ffdas_handle_t handle,

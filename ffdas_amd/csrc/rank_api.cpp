@@ -148,7 +148,7 @@ void release_rank_solver(ffdas_context &h) {
 
 using namespace ffdas;
 
-extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
+static ffdas_error_t truncate_rank_call(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
                                              const void *x, int64_t start, int64_t stop,
                                              ffdas_tensor_desc_t out_desc, void *out) {
     if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
@@ -270,4 +270,11 @@ extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor
     FFDAS_TRY(eigh(*handle, pl, H, D, E, info, &fail));
     FFDAS_TRY(launch_rank_project(st, pl, H, xs, out, Qh, Qn));
     return check_info(*handle, info, fail);
+}
+
+extern "C" ffdas_error_t ffdas_truncate_rank(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
+                                             const void *x, int64_t start, int64_t stop,
+                                             ffdas_tensor_desc_t out_desc, void *out) {
+    ffdas::CallScope cs(handle, "ffdas_truncate_rank");
+    return cs.done(truncate_rank_call(handle, x_desc, x, start, stop, out_desc, out));
 }

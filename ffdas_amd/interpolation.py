@@ -8,7 +8,7 @@ import ctypes
 import torch
 
 from ._core import as_tensor, check_device_tensor, element_code, handle_for
-from ._lib import TensorDesc, check, lib
+from ._lib import InterpMode, TensorDesc, check, lib
 
 __all__ = ["Interpolator", "interpolate"]
 
@@ -21,14 +21,16 @@ class Interpolator:
 
     Args:
         gridpos: grid vertex positions, shape (nz, ny, nx, 3) (converted to float32).
-        mode: "nearest" or "linear".
+        mode: "nearest" or "linear" (or :class:`InterpMode`).
     """
 
-    def __init__(self, gridpos, mode: str = "linear"):
+    def __init__(self, gridpos, mode="linear"):
         self._plan = None
         gridpos = as_tensor(gridpos)
         if gridpos.ndim != 4 or gridpos.shape[3] != 3:
             raise ValueError("gridpos must have shape (nz, ny, nx, 3)")
+        if isinstance(mode, InterpMode):
+            mode = mode.name.lower()
         if mode not in _MODES:
             raise ValueError(f"invalid interpolation mode '{mode}' "
                              f"(must be one of {tuple(_MODES.keys())})")

@@ -8,8 +8,9 @@
  * reference tree).  Pointers named "device" must point to memory on the handle's GPU;
  * everything runs asynchronously on the handle's stream and no call synchronises.
  *
- * The reference's other operations (contraction, interpolation, greens, gather, scatter,
- * truncate_rank) are not part of this library and are not declared here.
+ * From the reference's other operations, truncate_rank, interpolation and greens are declared
+ * here with their reference signatures; contraction, gather and scatter are not part of this
+ * library.
  *
  * Additive extensions (new symbols or new enum values; no existing value changes meaning)
  * are marked "EXTENSION".
@@ -144,8 +145,13 @@ FFDAS_API ffdas_error_t ffdas_destroy(ffdas_handle_t handle);
 FFDAS_API ffdas_error_t ffdas_set_stream(ffdas_handle_t handle, uintptr_t stream);
 FFDAS_API ffdas_error_t ffdas_get_stream(ffdas_handle_t handle, uintptr_t *stream);
 
-/* Strided copy with element-type conversion (out must be contiguous; same dims; real to
- * real or complex to complex among 16F/16BF/32F/64F).  Replaces include/ffdas.h:149-155. */
+/* Strided copy with element-type conversion: x (any rank, any element strides, negative
+ * included) into out (same dims, contiguous), else FFDAS_ERROR_INVALID_DIMS.  The (x, out)
+ * type pairs of the reference: R16I->{R16I,R32F}, R16F->{R16F,R32F}, R32F->{R16F,R32F,R64F},
+ * R64F->{R64F,R32F} and the complex counterparts C16I, C16F, C32F, C64F likewise; EXTENSION:
+ * R16BF->{R16BF,R32F}, R32F->R16BF and the complex counterparts.  Other pairs return
+ * FFDAS_ERROR_UNSUPPORTED_TYPE.  Narrowing rounds to nearest even.
+ * Replaces include/ffdas.h:149-155. */
 FFDAS_API ffdas_error_t ffdas_contiguous_copy(ffdas_handle_t handle, ffdas_tensor_desc_t x_desc,
                                               const void *x, ffdas_tensor_desc_t out_desc,
                                               void *out);
