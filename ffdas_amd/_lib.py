@@ -124,6 +124,7 @@ def lib():
         "ffdas_get_stream": (i32, [vp, P(u)]),
         "ffdas_contiguous_copy": (i32, [vp, vp, vp, vp, vp]),
         "ffdas_das": (i32, [vp, vp, vp, f32, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32]),
+        "ffdas_das_noalloc": (i32, [vp, vp, vp, f32, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32]),
         "ffdas_das_sparse": (i32, [vp, vp, vp, f32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32,
                                    i32]),
         "ffdas_das_ex": (i32, [vp, vp, vp, f32, f32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32,

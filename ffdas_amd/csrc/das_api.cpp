@@ -105,13 +105,16 @@ ffdas_error_t prepare_x(ffdas_context &h, const DasProblem &p, const void *x, vo
     return FFDAS_SUCCESS;
 }
 
-ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
-                                 float wavenum, float scale, ffdas_tensor_desc_t x_desc,
-                                 const void *x, const float *dstpos, const float *offsets,
-                                 const float *weights, bool sparse_mode, int sparse_count,
-                                 const int *sparse_indices, const void *beta,
-                                 ffdas_tensor_desc_t out_desc, void *out,
-                                 ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+}  // namespace
+
+ffdas_error_t das_forward_with(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                               float wavenum, float scale, ffdas_tensor_desc_t x_desc,
+                               const void *x, const float *dstpos, const float *offsets,
+                               const float *weights, bool sparse_mode, int sparse_count,
+                               const int *sparse_indices, const void *beta,
+                               ffdas_tensor_desc_t out_desc, void *out,
+                               ffdas_compute_type_t compute_type, ffdas_alg_t alg,
+                               const WorkspaceFn &get_ws, bool record_copy) {
     if (!handle) return FFDAS_ERROR_INVALID_HANDLE;
     FFDAS_TRY(handle->check_device());
     if (!x_desc || !out_desc) return FFDAS_ERROR_NULL_POINTER;
@@ -126,14 +129,31 @@ ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, con
     double br, bi;
     read_beta(beta, p, br, bi);
     void *ws = nullptr;
-    FFDAS_TRY(handle->workspace(cast_bytes(p), &ws));
+    FFDAS_TRY(get_ws(cast_bytes(p), &ws));
     XView xv;
     FFDAS_TRY(prepare_x(*handle, p, x, ws, xv));
-    record_xcopy(*handle, xd, p, x, ws);
+    if (record_copy) record_xcopy(*handle, xd, p, x, ws);
     DasGeomArgs a;
     a.srcpos = srcpos; a.srcdir = srcdir; a.dstpos = dstpos; a.offsets = offsets;
     a.weights = weights; a.sparse_indices = sparse_indices; a.wavenum = wavenum; a.scale = scale;
     return launch_forward(*handle, p, a, xv, out, br, bi);
+}
+
+namespace {
+
+ffdas_error_t das_forward_common(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                                 float wavenum, float scale, ffdas_tensor_desc_t x_desc,
+                                 const void *x, const float *dstpos, const float *offsets,
+                                 const float *weights, bool sparse_mode, int sparse_count,
+                                 const int *sparse_indices, const void *beta,
+                                 ffdas_tensor_desc_t out_desc, void *out,
+                                 ffdas_compute_type_t compute_type, ffdas_alg_t alg) {
+    const WorkspaceFn ws = [handle](size_t bytes, void **ptr) {
+        return handle->workspace(bytes, ptr);
+    };
+    return das_forward_with(handle, srcpos, srcdir, wavenum, scale, x_desc, x, dstpos, offsets,
+                            weights, sparse_mode, sparse_count, sparse_indices, beta, out_desc,
+                            out, compute_type, alg, ws, true);
 }
 
 }  // namespace

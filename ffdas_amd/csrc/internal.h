@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,19 @@ struct BatchMember {
 };
 // All members share storage type, complexness and frames per chunk (p.fi_frames > 0).
 ffdas_error_t launch_forward_batched(ffdas_context &h, const std::vector<BatchMember> &ms);
+
+// The forward of ffdas_das / ffdas_das_sparse / ffdas_das_ex (das_api.cpp) with its workspace
+// (the converted copy of x) taken from get_ws(bytes, &ptr); record_copy marks that copy for
+// reuse by a later ffdas_das_backward on the same handle.
+using WorkspaceFn = std::function<ffdas_error_t(size_t, void **)>;
+ffdas_error_t das_forward_with(ffdas_handle_t handle, const float *srcpos, const float *srcdir,
+                               float wavenum, float scale, ffdas_tensor_desc_t x_desc,
+                               const void *x, const float *dstpos, const float *offsets,
+                               const float *weights, bool sparse_mode, int sparse_count,
+                               const int *sparse_indices, const void *beta,
+                               ffdas_tensor_desc_t out_desc, void *out,
+                               ffdas_compute_type_t compute_type, ffdas_alg_t alg,
+                               const WorkspaceFn &get_ws, bool record_copy);
 
 // tuning cache (tune.cpp)
 std::string tune_key(const ffdas_context &h, const DasProblem &p, bool has_srcdir);

@@ -232,6 +232,21 @@ FFDAS_API ffdas_error_t ffdas_das_backward(ffdas_handle_t handle, const float *s
                                            float *grad_scale, ffdas_compute_type_t compute_type,
                                            ffdas_alg_t alg);
 
+/* ffdas_das without device allocation: the same arguments, checks and result, with the
+ * converted copy of x (FP16/BF16 cast, frame-inner relayout) kept in a device array of
+ * 256 MiB that the library holds on each GPU it runs on until the process ends, instead of
+ * in the handle's workspace.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  A problem whose copy exceeds 256 MiB returns FFDAS_ERROR_DIMS_TOO_LARGE.
+ * Calls that overlap on one GPU share the array and race. */
+FFDAS_API ffdas_error_t ffdas_das_noalloc(ffdas_handle_t handle, const float *srcpos,
+                                          const float *srcdir, float wavenum,
+                                          ffdas_tensor_desc_t x_desc, const void *x,
+                                          const float *dstpos, const float *offsets,
+                                          const float *weights, const void *beta,
+                                          ffdas_tensor_desc_t out_desc, void *out,
+                                          ffdas_compute_type_t compute_type, ffdas_alg_t alg);
+
 /* Host-only argument check: returns the error code ffdas_das_ex would return for these
  * descriptors and scalars, without touching a device (pointers are only NULL-checked). */
 FFDAS_API ffdas_error_t ffdas_das_validate(ffdas_tensor_desc_t x_desc,
