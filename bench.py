@@ -474,6 +474,10 @@ def bench_config5(a):
             "config": {"workload": "batched sparse compounding: " + ",".join(names) +
                                    f", {a.c5_frames} frames, 64 focused transmits, k=8, "
                                    "2048 samples", "storage": "fp16", "units_per_step": U,
+                       "parity": "timed on device-drawn noise; the same batched path is "
+                                 "checked against the FP64 oracle with synthesised echoes at "
+                                 "<= 5 frames (tests/test_config5_gpu.py) and on 2 frames of "
+                                 "the full probe geometries (tests/test_fullsize_gpu.py)",
                        "eager_ms": round(ms_eager, 3), "wall_s": round(wall, 3)},
             "roofline": {"bound": "hbm", "kernel": "das_fwd_fi_batched",
                          "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -206,7 +206,11 @@ ffdas_error_t ffdas_create_tensor_desc(ffdas_tensor_desc_t *desc, int64_t ndim,
             int64_t s = 1;
             for (int64_t i = ndim - 1; i >= 0; i--) {
                 d->strides[i] = s;
-                s *= dims[i];
+                // a stride that does not fit in 64 bits describes no addressable tensor
+                if (i > 0 && __builtin_mul_overflow(s, dims[i], &s)) {
+                    delete d;
+                    return FFDAS_ERROR_DIMS_TOO_LARGE;
+                }
             }
         }
     } catch (...) {

@@ -93,7 +93,7 @@ ffdas_error_t plan_das(const ffdas_tensor_desc &x, const ffdas_tensor_desc &out,
     p.N = 1;
     p.spatial.clear();
     for (size_t i = o; i < out.dims.size(); i++) {
-        p.N *= out.dims[i];
+        if (__builtin_mul_overflow(p.N, out.dims[i], &p.N)) return FFDAS_ERROR_INVALID_DIMS;
         p.spatial.push_back(out.dims[i]);
     }
     if (p.spatial.empty()) p.spatial.push_back(1);

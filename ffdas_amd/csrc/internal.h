@@ -16,9 +16,11 @@ struct ffdas_tensor_desc {
     std::vector<int64_t> strides;
     ffdas_datatype_t dtype = FFDAS_R_32F;
 
+    // product of the dims, saturated at INT64_MAX (such a tensor is rejected elsewhere)
     int64_t numel() const {
         int64_t n = 1;
-        for (int64_t d : dims) n *= d;
+        for (int64_t d : dims)
+            if (__builtin_mul_overflow(n, d, &n)) return INT64_MAX;
         return n;
     }
 };

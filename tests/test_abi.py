@@ -192,3 +192,14 @@ def test_tuning_cache_roundtrip(tmp_path):
         ffdas_amd.load_tuning(tmp_path / "missing.txt")
     ffdas_amd.clear_tuning()
     assert ffdas_amd.tuning_entries() == 0
+
+
+def test_descriptor_stride_overflow_is_refused():
+    """Contiguous strides that do not fit in 64 bits are refused with
+    FFDAS_ERROR_DIMS_TOO_LARGE instead of overflowing."""
+    d = ctypes.c_void_p()
+    dims = (ctypes.c_int64 * 4)(1 << 40, 1 << 40, 1 << 40, 9)     # stride of dim 0: 9 * 2^80
+    assert L().ffdas_create_tensor_desc(ctypes.byref(d), 4, dims, None, int(DType.R_32F)) == 7
+    dims = (ctypes.c_int64 * 3)(1 << 20, 1 << 20, 9)
+    assert L().ffdas_create_tensor_desc(ctypes.byref(d), 3, dims, None, int(DType.R_32F)) == 0
+    L().ffdas_destroy_tensor_desc(d)

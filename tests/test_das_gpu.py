@@ -6,7 +6,9 @@ Tolerances (rel-L2 against the FP64 oracle of the documented definition):
                             are float; FP64 data and accumulation)
   FP16 storage      1e-3   against the oracle on the ORIGINAL float data; 2e-5 against the
                             oracle on the fp16-rounded data (kernel arithmetic alone)
-  BF16 storage      1e-3 / 2e-5 likewise where the bf16 quantisation allows (stated per test)
+  BF16 storage      1e-5 against the oracle on the bf16-rounded data; against the original
+                   data the bf16 rounding of the data itself sets a floor of about 1e-3
+                   (bound 1.5e-3, stated in the test)
 """
 import numpy as np
 import pytest
@@ -160,10 +162,15 @@ def test_config3_fp16(dev, mode):
     eq = rel_l2(got, _ref(w, x=w.x.astype(np.float16).astype(np.float64)))
     print(f"config3 fp16 ({mode}) rel-L2 vs FP64 = {e:.3e}, vs FP64 on fp16 data = {eq:.3e}")
     assert e < 1e-3
-    assert eq < 1e-4
+    assert eq < 1e-5
 
 
 def test_config3_bf16(dev):
+    """The kernel's own arithmetic is pinned against the oracle run on the bf16-rounded data
+    (1e-5, the FP32 bound).  Against the unrounded data the error is the rounding of the data
+    itself: bfloat16 keeps 8 significant bits, 2^-9 / sqrt(3) = 1.1e-3 rms relative per
+    sample, which the sum over 384 channel-transmit terms carries to about 1e-3 (9.5e-4 on
+    this seed), so the bound there is the quantisation floor, 1.5e-3."""
     import torch
     w = W.config3()
     ref = _ref(w)
@@ -172,8 +179,8 @@ def test_config3_bf16(dev):
     e = rel_l2(got, ref)
     eq = rel_l2(got, _ref(w, x=xq))
     print(f"config3 bf16 rel-L2 vs FP64 = {e:.3e}, vs FP64 on bf16 data = {eq:.3e}")
-    assert e < 1e-3
-    assert eq < 1e-4
+    assert eq < 1e-5
+    assert e < 1.5e-3
 
 
 def test_config3_iq_complex(dev):
