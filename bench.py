@@ -113,7 +113,7 @@ def tune(p, x):
     """Autotune the forward and adjoint launch parameters for this problem (kept in the
     library's tuning cache for the timed steps)."""
     import ffdas_amd
-    return ffdas_amd.autotune(x, p.srcpos, p.dstpos, p.offsets, p.weights, reps=3)
+    return ffdas_amd.autotune(x, p.srcpos, p.dstpos, p.offsets, p.weights, reps=5)
 
 
 def bwdx_frames_used(a, tuned):
