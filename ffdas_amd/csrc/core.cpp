@@ -134,6 +134,27 @@ ffdas_error_t ffdas_context::workspace(size_t bytes, void **ptr) {
     return FFDAS_SUCCESS;
 }
 
+ffdas_error_t ffdas_context::side_stream(hipStream_t *s) {
+    if (!side) {
+        hipStream_t st = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&a, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (st) (void)hipStreamDestroy(st);
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            return hip_status(e);
+        }
+        side = st;
+        fork_ev = a;
+        join_ev = b;
+    }
+    *s = side;
+    return FFDAS_SUCCESS;
+}
+
 extern "C" {
 
 const char *ffdas_version(void) { return "ffdas-mi355x 0.1.0 (gfx950)"; }
@@ -259,6 +280,12 @@ ffdas_error_t ffdas_destroy(ffdas_handle_t h) {
     if (!h) return FFDAS_ERROR_INVALID_HANDLE;
     ffdas_error_t err = FFDAS_SUCCESS;
     ffdas::release_rank_solver(*h);
+    if (h->side) {
+        (void)hipStreamSynchronize(h->side);
+        (void)hipStreamDestroy(h->side);
+        (void)hipEventDestroy(h->fork_ev);
+        (void)hipEventDestroy(h->join_ev);
+    }
     if (h->ws) h->retired.push_back(h->ws);
     for (void *p : h->retired) {
         // release in stream order; if the runtime is already torn down, just leak

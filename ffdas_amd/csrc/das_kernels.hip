@@ -2310,7 +2310,22 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     hipStream_t st = h.stream;
     hipError_t e = hipSuccess;
 
-    if (gofs || gwt || gscale) {
+    // With both the parameter gradients and grad_x asked for, the grad_x scatter (bound by
+    // LDS atomics) runs on the handle's second stream beside the parameter adjoint (bound by
+    // its gathers), forked from and joined back into the handle's stream.
+    const bool params = gofs || gwt || gscale;
+    bool concurrent = params && gx && !p.plain && !use_sorted_bwdx(p);
+    if (const char *env = std::getenv("FFDAS_BWD_CONCURRENT"))
+        if (std::atoi(env) == 0) concurrent = false;
+    hipStream_t sx = st;
+    if (concurrent) {
+        FFDAS_TRY(h.side_stream(&sx));
+        if ((e = hipEventRecord(h.fork_ev, st)) != hipSuccess ||
+            (e = hipStreamWaitEvent(sx, h.fork_ev, 0)) != hipSuccess)
+            return hip_status(e);
+    }
+
+    if (params) {
         const ParamOut po = param_out(p, gofs, gwt, gscale, pws);
 #define BWDP(ST, A64, CX) e = launch_bwdp_t<ST, A64, CX>(st, g, tl, xv, gout, gs_b, po)
 #define BWDPFI(ST, CX) e = launch_bwdp_fi_t<ST, CX>(st, g, tl, p.fi_frames, p.tune.param_groups, xv.ptr, gout, gs_b, po)
@@ -2365,13 +2380,18 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
         // whatever the storage type: the scatter is bound by LDS atomics, and the extra
         // delay arithmetic overlaps with them
         if (p.acc64) {
-            e = p.cplx ? launch_bwdx_t<true, true, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<true, false, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<true, true, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<true, false, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
         } else {
-            e = p.cplx ? launch_bwdx_t<false, true, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, true>(st, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
+                       : launch_bwdx_t<false, false, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+    }
+    if (sx != st) {
+        if ((e = hipEventRecord(h.join_ev, sx)) != hipSuccess ||
+            (e = hipStreamWaitEvent(st, h.join_ev, 0)) != hipSuccess)
+            return hip_status(e);
     }
     return FFDAS_SUCCESS;
 }

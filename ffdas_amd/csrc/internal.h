@@ -70,6 +70,13 @@ struct ffdas_context {
     // rocBLAS handle used by the rank-truncation eigensolver (created on first use)
     void *blas = nullptr;
 
+    // A second stream (default priority, non-blocking) on which ffdas_das_backward runs the
+    // grad_x scatter beside the parameter adjoint, and the events that fork it from and join
+    // it back into `stream`; created on first use.
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    ffdas_error_t side_stream(hipStream_t *s);
+
     ffdas_error_t check_device() const;
     ffdas_error_t workspace(size_t bytes, void **ptr);
 };
