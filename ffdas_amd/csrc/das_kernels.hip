@@ -2310,13 +2310,15 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     hipStream_t st = h.stream;
     hipError_t e = hipSuccess;
 
-    // With both the parameter gradients and grad_x asked for, the grad_x scatter (bound by
-    // LDS atomics) runs on the handle's second stream beside the parameter adjoint (bound by
-    // its gathers), forked from and joined back into the handle's stream.
+    // With FFDAS_BWD_CONCURRENT=1 and both the parameter gradients and grad_x asked for, the
+    // grad_x scatter (bound by LDS atomics) runs on the handle's second stream beside the
+    // parameter adjoint (bound by its gathers), forked from and joined back into the
+    // handle's stream.  Off by default: the gain is small (grad_x holds most of a CU's LDS,
+    // so the two share few CUs), and per-kernel durations then overlap.
     const bool params = gofs || gwt || gscale;
-    bool concurrent = params && gx && !p.plain && !use_sorted_bwdx(p);
+    bool concurrent = false;
     if (const char *env = std::getenv("FFDAS_BWD_CONCURRENT"))
-        if (std::atoi(env) == 0) concurrent = false;
+        concurrent = std::atoi(env) != 0 && params && gx && !p.plain && !use_sorted_bwdx(p);
     hipStream_t sx = st;
     if (concurrent) {
         FFDAS_TRY(h.side_stream(&sx));
