@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 
 #include "das_device.h"
 
@@ -79,6 +80,11 @@ struct NaturalRows {
     }
 };
 
+// Two rows of zeros for the widest frame-inner row (16 complex FP32 frames): the gather
+// address of a tap that reads nothing, so that the loads of a group of taps need no branch.
+// Never written.
+__device__ __align__(16) uint4 kZeroRows[16];
+
 // Frame-inner layout [chunk][C][T][S][FT] (written by relayout_fi_kernel): the FT frames of
 // rows k and k+1 are 2*FT*CW contiguous elements, fetched with 16-byte vector loads.
 template <Storage ST, int FT, int CW>
@@ -86,6 +92,7 @@ struct FrameInnerRows {
     using raw = typename Elem<ST>::raw;
     static constexpr int NB = 2 * FT * CW * (int)sizeof(raw);
     static_assert((FT * CW * sizeof(raw)) % 16 == 0, "frame-inner rows must be 16-byte multiples");
+    static_assert(NB <= (int)sizeof(kZeroRows), "zero rows shorter than two frame-inner rows");
     const raw *base;
     int T, S;
     __device__ __forceinline__ FrameInnerRows(const void *xv, int chunk, int C, int T_, int S_)
@@ -95,20 +102,34 @@ struct FrameInnerRows {
     __device__ __forceinline__ uint32_t offset(int m, int s, int k) const {
         return (uint32_t)((m * T + s) * S + k);
     }
-    template <class A>
-    __device__ __forceinline__ void load(uint32_t off, A (&v)[2][FT][CW]) const {
-        const uint4 *q = reinterpret_cast<const uint4 *>(base + (size_t)off * FT * CW);
-        uint4 tmp[NB / 16];
+    static constexpr int NQ = NB / 16;     // 16-byte loads per tap
+    __device__ __forceinline__ void fetch(uint32_t off, uint4 (&q)[NQ]) const {
+        fetch_from(reinterpret_cast<const uint4 *>(base + (size_t)off * FT * CW), q);
+    }
+    // rows k, k+1 of (m, s), or two rows of zeros when the tap reads nothing
+    __device__ __forceinline__ void fetch_or_zero(bool in, uint32_t off, uint4 (&q)[NQ]) const {
+        fetch_from(in ? reinterpret_cast<const uint4 *>(base + (size_t)off * FT * CW) : kZeroRows, q);
+    }
+    __device__ __forceinline__ static void fetch_from(const uint4 *p, uint4 (&q)[NQ]) {
 #pragma unroll
-        for (int i = 0; i < NB / 16; i++) tmp[i] = q[i];
+        for (int i = 0; i < NQ; i++) q[i] = p[i];
+    }
+    template <class A>
+    __device__ __forceinline__ void unpack(const uint4 (&q)[NQ], A (&v)[2][FT][CW]) const {
         raw r[2 * FT * CW];
-        __builtin_memcpy(r, tmp, NB);
+        __builtin_memcpy(r, q, NB);
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
             for (int f = 0; f < FT; f++)
 #pragma unroll
                 for (int c = 0; c < CW; c++) v[h][f][c] = (A)Elem<ST>::real(r[(h * FT + f) * CW + c]);
+    }
+    template <class A>
+    __device__ __forceinline__ void load(uint32_t off, A (&v)[2][FT][CW]) const {
+        uint4 q[NQ];
+        fetch(off, q);
+        unpack(q, v);
     }
 };
 
@@ -186,6 +207,100 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
     }
 }
 
+// Frame-inner forward with the loads of TB taps in flight together: each group of TB taps
+// issues all its 16-byte loads before the first of them is used, so a wave keeps TB times
+// more gathers in flight than tap by tap.  A tap that reads nothing loads two rows of zeros
+// (one cache line for every such lane) with its weights zeroed, so it adds exactly zero and
+// no branch separates the loads from their use.
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
+__device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRows<ST, FT, CPLX ? 2 : 1> &xr,
+                                                int n, int m0, int mstep,
+                                                AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1]) {
+    static_assert(KC % TB == 0, "taps per group must divide taps per round");
+    using A = AccT<ACC64>;
+    using Rows = FrameInnerRows<ST, FT, CPLX ? 2 : 1>;
+    constexpr int CW = CPLX ? 2 : 1;
+    const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
+#pragma unroll
+    for (int f = 0; f < FT; f++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) acc[f][c] = A(0);
+    if (g.S < 2) return;                       // no tap has two rows to read
+    const bool unit = g.unit_scale != 0;
+
+    for (int kc = 0; kc < g.K; kc += KC) {
+        float ofs[KC], wt[KC];
+        int sq[KC];
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            const int jj = kc + j;
+            if (jj < g.K) {
+                const int64_t o = (int64_t)jj * g.N + n;
+                ofs[j] = g.offsets[o];
+                wt[j] = g.weights[o];
+                sq[j] = g.sparse ? g.sparse[o] : jj;
+            } else {
+                ofs[j] = 0.0f;
+                wt[j] = 0.0f;
+                sq[j] = -1;
+            }
+        }
+        for (int m = m0; m < g.C; m += mstep) {
+            const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
+            const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
+            bool dok = true;
+            if (g.srcdir) {
+                const float *sd = g.srcdir + 4 * m;
+                dok = (R.dx * sd[0] + R.dy * sd[1] + R.dz * sd[2]) * R.rinv > sd[3];
+            }
+#pragma unroll
+            for (int j0 = 0; j0 < KC; j0 += TB) {
+                if (kc + j0 >= g.K) break;
+                uint4 q[TB][Rows::NQ];
+                Tap tp[TB];
+                bool in[TB];
+#pragma unroll
+                for (int u = 0; u < TB; u++) {
+                    const int j = j0 + u;
+                    tp[u] = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                    in[u] = dok && (uint32_t)tp[u].k < (uint32_t)(g.S - 1) &&
+                            (uint32_t)sq[j] < (uint32_t)g.T;
+                    xr.fetch_or_zero(in[u], xr.offset(m, sq[j], tp[u].k), q[u]);
+                }
+                // keep the group's loads ahead of their first use
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < TB; u++) {
+                    const int j = j0 + u;
+                    A v[2][FT][CW];
+                    xr.unpack(q[u], v);
+                    if constexpr (!CPLX) {
+                        const A c0 = (A)wt[j] * (A)tp[u].t;
+                        const A c = in[u] ? c0 : A(0);
+                        const A a = in[u] ? (A)wt[j] - c0 : A(0);
+#pragma unroll
+                        for (int f = 0; f < FT; f++)
+                            acc[f][0] = fmaA(c, v[1][f][0], fmaA(a, v[0][f][0], acc[f][0]));
+                    } else {
+                        float sn, cs;
+                        phase<PRECISE>(g.rev_h, g.rev_l, tp[u], sn, cs);
+                        const A wr = in[u] ? (A)(wt[j] * cs) : A(0);
+                        const A wi = in[u] ? (A)(wt[j] * sn) : A(0);
+                        const A t = in[u] ? (A)tp[u].t : A(0);
+#pragma unroll
+                        for (int f = 0; f < FT; f++) {
+                            const A lr = v[0][f][0] + t * (v[1][f][0] - v[0][f][0]);
+                            const A li = v[0][f][1] + t * (v[1][f][1] - v[0][f][1]);
+                            acc[f][0] += wr * lr - wi * li;
+                            acc[f][1] += wr * li + wi * lr;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <bool ACC64, bool CPLX, int FT>
 __device__ __forceinline__ void store_out(const AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1], void *outv,
                                           int64_t os_b, int n, int b0, int B, double beta_re,
@@ -237,7 +352,7 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
 // w, w+G, ... and the G partial sums are combined through LDS, so a launch has G times more
 // waves in flight than tiles (latency hiding for the gathers) without repeating any delay
 // arithmetic.
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
 __device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, const void *xv,
                                             void *outv, int64_t os_b, double beta_re,
                                             double beta_im, int tile, int chunk,
@@ -245,14 +360,17 @@ __device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, cons
     using A = AccT<ACC64>;
     constexpr int CW = CPLX ? 2 : 1;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int G = blockDim.x >> 6;
     bool valid;
     int n = voxel_of(tl, tile, lane, valid);
     if (!valid) n = 0;          // computed, never stored: keeps every lane at the barrier
     const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
     A acc[FT][CW];
-    das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, wave, G, acc);
+    if constexpr (TB == 1)
+        das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, wave, G, acc);
+    else
+        das_fwd_body_fi<ST, ACC64, CPLX, PRECISE, FT, KC, TB>(g, xr, n, wave, G, acc);
     if (G == 1) {
         if (valid) store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, chunk * FT, g.B, beta_re, beta_im);
         return;
@@ -292,12 +410,12 @@ __device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, cons
 }
 
 // grid.x = voxel tile, grid.y = frame chunk
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
 __global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
                                                          void *__restrict__ outv, int64_t os_b,
                                                          double beta_re, double beta_im) {
     extern __shared__ unsigned char fwd_lds[];
-    fwd_fi_tile<ST, ACC64, CPLX, PRECISE, FT, KC>(g, tl, xv, outv, os_b, beta_re, beta_im,
+    fwd_fi_tile<ST, ACC64, CPLX, PRECISE, FT, KC, TB>(g, tl, xv, outv, os_b, beta_re, beta_im,
                                                    blockIdx.x, blockIdx.y, fwd_lds);
 }
 
@@ -320,7 +438,7 @@ struct BatchArgs {
     BatchItem item[kMaxBatch];
 };
 
-template <Storage ST, bool CPLX, bool PRECISE, int FT, int KC>
+template <Storage ST, bool CPLX, bool PRECISE, int FT, int KC, int TB>
 __global__ __launch_bounds__(512) void das_fwd_fi_batched_kernel(BatchArgs args) {
     extern __shared__ unsigned char fwd_lds[];
     const int b = blockIdx.x;
@@ -329,7 +447,7 @@ __global__ __launch_bounds__(512) void das_fwd_fi_batched_kernel(BatchArgs args)
     const BatchItem &it = args.item[i];
     const int local = b - it.start;
     const int chunk = local / it.tiles;
-    fwd_fi_tile<ST, false, CPLX, PRECISE, FT, KC>(it.g, it.tl, it.x, it.out, it.os_b, it.beta_re,
+    fwd_fi_tile<ST, false, CPLX, PRECISE, FT, KC, TB>(it.g, it.tl, it.x, it.out, it.os_b, it.beta_re,
                                                    it.beta_im, local - chunk * it.tiles, chunk,
                                                    fwd_lds);
 }
@@ -554,7 +672,7 @@ __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     constexpr int CW = CPLX ? 2 : 1;
     extern __shared__ unsigned char param_lds[];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int G = blockDim.x >> 6;
     bool valid;
     int n = voxel_of(tl, blockIdx.x, lane, valid);
@@ -1882,6 +2000,21 @@ hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl
     }
 }
 
+// Taps whose gathers a frame-inner forward wave issues together (1, 2 or 4).  Measured on
+// the 2-D fwd+adjoint benchmark (16 frames per chunk): FP32 storage 0.494 ms (1 tap), 0.436
+// (2), 0.429 (4); FP16 storage 0.269 / 0.281 / 0.277 ms, where one tap keeps 6 waves per SIMD
+// against 5 and 4.  Complex FP32 at 4 taps spills registers.  FFDAS_TAPS overrides.
+template <Storage ST, bool CPLX>
+int tap_group() {
+    static const int env = [] {
+        const char *e = std::getenv("FFDAS_TAPS");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (env > 0) return env >= 4 && !(CPLX && ST == Storage::F32) ? 4 : (env >= 2 ? 2 : 1);
+    if (ST != Storage::F32) return 1;
+    return CPLX ? 2 : 4;
+}
+
 template <Storage ST, bool CPLX, int F>
 constexpr bool fi_ok() {
     return (F * (CPLX ? 2 : 1) * (ST == Storage::F32 ? 4 : 2)) % 16 == 0;
@@ -1898,13 +2031,19 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
         dim3 grid(ntiles, (g.B + F - 1) / F);
         const int G = channel_groups(g, (int64_t)grid.x * grid.y, groups);
         const size_t lds = G > 1 ? (size_t)G * F * (CPLX ? 2 : 1) * 64 * sizeof(float) : 0;
-        if (lds > 0)
-            (void)hipFuncSetAttribute((const void *)dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        ProfScope ps(st, "das_fwd_fi");
-        dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 64 * G, lds, st>>>(
-            g, tl, x, out, os_b, br, bi);
-        return hipGetLastError();
+        auto go = [&](auto kern) {
+            if (lds > 0)
+                (void)hipFuncSetAttribute((const void *)kern,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            ProfScope ps(st, "das_fwd_fi");
+            kern<<<grid, 64 * G, lds, st>>>(g, tl, x, out, os_b, br, bi);
+            return hipGetLastError();
+        };
+        switch (tap_group<ST, CPLX>()) {
+            case 4: return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 4>);
+            case 2: return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 2>);
+            default: return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 1>);
+        }
     }
 }
 
@@ -1993,14 +2132,20 @@ hipError_t launch_fwd_batched_f(hipStream_t st, const std::vector<BatchMember> &
             gc.C = minC;
             const int G = channel_groups(gc, start, want);
             const size_t lds = G > 1 ? (size_t)G * F * (CPLX ? 2 : 1) * 64 * sizeof(float) : 0;
-            if (lds > 0)
-                (void)hipFuncSetAttribute(
-                    (const void *)dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC>,
-                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            ProfScope ps(st, "das_fwd_fi_batched");
-            dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC>
-                <<<(unsigned)start, 64 * G, lds, st>>>(args);
-            const hipError_t e = hipGetLastError();
+            auto go = [&](auto kern) {
+                if (lds > 0)
+                    (void)hipFuncSetAttribute((const void *)kern,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                ProfScope ps(st, "das_fwd_fi_batched");
+                kern<<<(unsigned)start, 64 * G, lds, st>>>(args);
+                return hipGetLastError();
+            };
+            hipError_t e;
+            switch (tap_group<ST, CPLX>()) {
+                case 4: e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 4>); break;
+                case 2: e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 2>); break;
+                default: e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 1>); break;
+            }
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
