@@ -80,53 +80,63 @@ struct NaturalRows {
     }
 };
 
-// Two rows of zeros for the widest frame-inner row (16 complex FP32 frames): the gather
-// address of a tap that reads nothing, so that the loads of a group of taps need no branch.
-// Never written.
-__device__ __align__(16) uint4 kZeroRows[16];
+// Sixteen zero bytes: the gather address of a tap that reads nothing, so that the loads of a
+// group of taps need no branch.  Never written.
+__device__ __align__(16) uint4 kZeroRows[1];
 
-// Frame-inner layout [chunk][C][T][S][FT] (written by relayout_fi_kernel): the FT frames of
-// rows k and k+1 are 2*FT*CW contiguous elements, fetched with 16-byte vector loads.
+// Frame-inner layout, frames in groups of GS (written by the relayout kernels): per chunk
+// [C][T][NG][S][GS][CW], NG = FT / GS, with GS * CW elements = 8 bytes.  One 16-byte load at
+// sample k of plane g holds rows k and k+1 of the group's frames, so the NG loads of a tap
+// step through planes S * 8 bytes apart, and the lanes of a wave (neighbouring voxels, close
+// sample indices) fall on few cache lines: a 128-byte line holds 16 samples of a plane.
 template <Storage ST, int FT, int CW>
 struct FrameInnerRows {
     using raw = typename Elem<ST>::raw;
-    static constexpr int NB = 2 * FT * CW * (int)sizeof(raw);
-    static_assert((FT * CW * sizeof(raw)) % 16 == 0, "frame-inner rows must be 16-byte multiples");
-    static_assert(NB <= (int)sizeof(kZeroRows), "zero rows shorter than two frame-inner rows");
+    static constexpr int GS = 8 / (CW * (int)sizeof(raw));    // frames per group
+    static constexpr int NG = FT / GS;                          // planes per (channel, transmit)
+    static constexpr int NQ = NG;                               // 16-byte loads per tap
+    static_assert(GS >= 1 && FT % GS == 0, "frame groups must tile the frames");
     const raw *base;
     int T, S;
     __device__ __forceinline__ FrameInnerRows(const void *xv, int chunk, int C, int T_, int S_)
         : T(T_), S(S_) {
         base = static_cast<const raw *>(xv) + (int64_t)chunk * C * T_ * S_ * FT * CW;
     }
-    __device__ __forceinline__ uint32_t offset(int m, int s, int k) const {
-        return (uint32_t)((m * T + s) * S + k);
+    // element offset of sample k of plane 0 of row (m, s)
+    __device__ __forceinline__ size_t offset(int m, int s, int k) const {
+        return ((size_t)(m * T + s) * NG * S + (size_t)k) * GS * CW;
     }
-    static constexpr int NQ = NB / 16;     // 16-byte loads per tap
-    __device__ __forceinline__ void fetch(uint32_t off, uint4 (&q)[NQ]) const {
-        fetch_from(reinterpret_cast<const uint4 *>(base + (size_t)off * FT * CW), q);
-    }
-    // rows k, k+1 of (m, s), or two rows of zeros when the tap reads nothing
-    __device__ __forceinline__ void fetch_or_zero(bool in, uint32_t off, uint4 (&q)[NQ]) const {
-        fetch_from(in ? reinterpret_cast<const uint4 *>(base + (size_t)off * FT * CW) : kZeroRows, q);
-    }
-    __device__ __forceinline__ static void fetch_from(const uint4 *p, uint4 (&q)[NQ]) {
+    __device__ __forceinline__ void fetch(size_t off, uint4 (&q)[NQ]) const {
+        const raw *p = base + off;
 #pragma unroll
-        for (int i = 0; i < NQ; i++) q[i] = p[i];
+        for (int g = 0; g < NG; g++)
+            __builtin_memcpy(&q[g], p + (size_t)g * S * GS * CW, 16);
     }
+    // rows k, k+1 of (m, s), or zeros when the tap reads nothing
+    __device__ __forceinline__ void fetch_or_zero(bool in, size_t off, uint4 (&q)[NQ]) const {
+        const raw *p = in ? base + off : reinterpret_cast<const raw *>(kZeroRows);
+        const size_t step = in ? (size_t)S * GS * CW : 0;
+#pragma unroll
+        for (int g = 0; g < NG; g++) __builtin_memcpy(&q[g], p + g * step, 16);
+    }
+    // v[h][f][c]: row k + h, frame f = g * GS + i, part c
     template <class A>
     __device__ __forceinline__ void unpack(const uint4 (&q)[NQ], A (&v)[2][FT][CW]) const {
-        raw r[2 * FT * CW];
-        __builtin_memcpy(r, q, NB);
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int g = 0; g < NG; g++) {
+            raw r[2 * GS * CW];
+            __builtin_memcpy(r, &q[g], 16);
 #pragma unroll
-            for (int f = 0; f < FT; f++)
+            for (int h = 0; h < 2; h++)
 #pragma unroll
-                for (int c = 0; c < CW; c++) v[h][f][c] = (A)Elem<ST>::real(r[(h * FT + f) * CW + c]);
+                for (int i = 0; i < GS; i++)
+#pragma unroll
+                    for (int c = 0; c < CW; c++)
+                        v[h][g * GS + i][c] = (A)Elem<ST>::real(r[(h * GS + i) * CW + c]);
+        }
     }
     template <class A>
-    __device__ __forceinline__ void load(uint32_t off, A (&v)[2][FT][CW]) const {
+    __device__ __forceinline__ void load(size_t off, A (&v)[2][FT][CW]) const {
         uint4 q[NQ];
         fetch(off, q);
         unpack(q, v);
@@ -688,9 +698,10 @@ __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     if (scale_part) block_scale_store<ACC64>(part, scale_part);
 }
 
-// Relayout + cast into the frame-inner layout: dst[chunk][c][t][k][f] = src[chunk*FT+f][c][t][k]
-// (zero for frames past B).  One block handles a 64-sample segment of one (chunk, c, t) row
-// for all FT frames, transposing through LDS so reads and writes are both contiguous.
+// Relayout + cast into the frame-inner layout: dst[chunk][c][t][g][k][i] =
+// src[chunk*FT + g*GS + i][c][t][k] (zero for frames past B), GS frames per group as read by
+// FrameInnerRows.  One block handles a 64-sample segment of one (chunk, c, t) row for all FT
+// frames, transposing through LDS so reads and writes are both contiguous.
 template <Storage SS, class DST, int FT, int CW>
 __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS>::raw *__restrict__ src, int64_t sb,
                                                           int64_t sc, int64_t st, int B, int C,
@@ -715,19 +726,23 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
         tile[f][e] = v;
     }
     __syncthreads();
+    constexpr int GS = 8 / (CW * (int)sizeof(DST));
     DST *d = dst + ((((int64_t)chunk * C + c) * T + t) * S) * FT * CW;
     for (int i = threadIdx.x; i < 64 * FT * CW; i += blockDim.x) {
-        const int kk = i / (FT * CW), r = i - kk * FT * CW;
-        const int f = r / CW, cc = r - f * CW;
+        // i = ((g * 64 + kk) * GS + fi) * CW + cc: consecutive threads write consecutive elements
+        const int cc = i % CW, r = i / CW;
+        const int fi = r % GS, r2 = r / GS;
+        const int kk = r2 % 64, g = r2 / 64;
         const int k = k0 + kk;
-        if (k < S) d[((int64_t)k * FT + f) * CW + cc] = convert_to<DST>(tile[f][kk * CW + cc]);
+        if (k < S)
+            d[(((int64_t)g * S + k) * GS + fi) * CW + cc] = convert_to<DST>(tile[g * GS + fi][kk * CW + cc]);
     }
 }
 
 // The same relayout with 16-byte vector loads and stores, for rows whose start and length are
 // 16-byte multiples: a block moves a segment of kRelL samples of one (chunk, c, t) row for
-// all FT frames (each frame's segment read as 16-byte vectors, the frame-inner segment
-// written as 16-byte vectors), transposing through LDS.
+// all FT frames (each frame's segment read as 16-byte vectors, each plane's segment -- two
+// samples of GS frames per vector -- written as 16-byte vectors), transposing through LDS.
 constexpr int kRelL = 128;
 
 template <Storage SS, class DST, int FT, int CW>
@@ -765,20 +780,24 @@ __global__ __launch_bounds__(256) void relayout_fi_vec_kernel(
         for (int j = 0; j < SV; j++) tile[f][e + j] = v[j];
     }
     __syncthreads();
-    DST *d = dst + (((((int64_t)chunk * C + c) * T + t) * S) + (int64_t)seg * kRelL) * FT * CW;
-    const int kmax = min(kRelL, S - seg * kRelL);   // samples of this segment inside the row
-    for (int i = threadIdx.x; i < kmax * FT * CW / DV; i += blockDim.x) {
+    constexpr int GS = 8 / (CW * (int)sizeof(DST));
+    static_assert(DV == 2 * GS * CW, "a vector holds two samples of one plane");
+    DST *d = dst + (((int64_t)chunk * C + c) * T + t) * S * FT * CW;
+    const int k0 = seg * kRelL;
+    const int kmax = min(kRelL, S - k0);            // samples of this segment inside the row
+    const int per_plane = kmax / 2;                 // vectors per plane (S is even)
+    for (int i = threadIdx.x; i < FT / GS * per_plane; i += blockDim.x) {
+        const int g = i / per_plane, kk = 2 * (i - g * per_plane);
         DST o[DV];
 #pragma unroll
         for (int j = 0; j < DV; j++) {
-            const int idx = i * DV + j;
-            const int kk = idx / (FT * CW), r = idx - kk * FT * CW;
-            const int f = r / CW, cc = r - f * CW;
-            o[j] = convert_to<DST>(tile[f][kk * CW + cc]);
+            const int h = j / (GS * CW), r = j - h * GS * CW;
+            const int fi = r / CW, cc = r - fi * CW;
+            o[j] = convert_to<DST>(tile[g * GS + fi][(kk + h) * CW + cc]);
         }
         uint4 q;
         __builtin_memcpy(&q, o, 16);
-        reinterpret_cast<uint4 *>(d)[i] = q;
+        *reinterpret_cast<uint4 *>(d + (((int64_t)g * S + k0 + kk) * GS) * CW) = q;
     }
 }
 
@@ -2001,9 +2020,9 @@ hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl
 }
 
 // Taps whose gathers a frame-inner forward wave issues together (1, 2 or 4).  Measured on
-// the 2-D fwd+adjoint benchmark (16 frames per chunk): FP32 storage 0.494 ms (1 tap), 0.436
-// (2), 0.429 (4); FP16 storage 0.269 / 0.281 / 0.277 ms, where one tap keeps 6 waves per SIMD
-// against 5 and 4.  Complex FP32 at 4 taps spills registers.  FFDAS_TAPS overrides.
+// the 2-D fwd+adjoint benchmark (16 frames per chunk, frame-group planes): FP16 storage
+// 0.280 ms (1 tap), 0.230 (2), 0.219 (4); FP32 storage 0.494 / 0.436 / 0.429 ms with the
+// earlier row-pair layout.  Complex FP32 at 4 taps spills registers.  FFDAS_TAPS overrides.
 template <Storage ST, bool CPLX>
 int tap_group() {
     static const int env = [] {
@@ -2011,7 +2030,6 @@ int tap_group() {
         return e ? std::atoi(e) : 0;
     }();
     if (env > 0) return env >= 4 && !(CPLX && ST == Storage::F32) ? 4 : (env >= 2 ? 2 : 1);
-    if (ST != Storage::F32) return 1;
     return CPLX ? 2 : 4;
 }
 
@@ -2168,7 +2186,8 @@ hipError_t launch_relayout_f(hipStream_t st, const DasProblem &p, const void *x,
     constexpr size_t ES = sizeof(typename dev::Elem<SS>::raw) * CW;   // bytes per sample
     const bool vec = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (p.xs_b * ES) % 16 == 0 &&
                      (p.xs_c * ES) % 16 == 0 && (p.xs_t * ES) % 16 == 0 &&
-                     ((size_t)p.S * ES) % 16 == 0 && (F * CW * sizeof(DST)) % 16 == 0;
+                     ((size_t)p.S * ES) % 16 == 0 && p.S % 2 == 0 &&
+                     (F * CW * sizeof(DST)) % 16 == 0;
     if (vec) {
         const int64_t blocks = (int64_t)((p.S + dev::kRelL - 1) / dev::kRelL) * p.C * p.T;
         if (blocks > 0x7fffffff) return hipErrorInvalidValue;

@@ -165,6 +165,50 @@ def counter_table(base: str, cus: int = 256, simds: int = 4, xcds: int = 8) -> s
     return "\n".join(lines) + "\n"
 
 
+# vector-memory passes of `vmem` (at most 4 TCP, 2 TA, 2 TD counters per pass)
+VMEM_PASSES = {
+    "l1a": "TCP_PENDING_STALL_CYCLES_sum TCP_READ_TAGCONFLICT_STALL_CYCLES_sum "
+           "TCP_TCR_TCP_STALL_CYCLES_sum TCP_UTCL1_TRANSLATION_MISS_sum "
+           "TA_ADDR_STALLED_BY_TC_CYCLES_sum TA_DATA_STALLED_BY_TC_CYCLES_sum GRBM_GUI_ACTIVE",
+    "l1b": "TCP_TCC_READ_REQ_LATENCY_sum TCP_TCC_READ_REQ_sum TCP_TOTAL_CACHE_ACCESSES_sum "
+           "TCP_TCP_TA_DATA_STALL_CYCLES_sum TA_TA_BUSY_sum TA_FLAT_READ_WAVEFRONTS_sum "
+           "TD_TD_BUSY_sum TD_TC_STALL_sum GRBM_GUI_ACTIVE",
+}
+
+
+def vmem_table(base: str, cus: int = 256, xcds: int = 8) -> str:
+    """Per-kernel vector-memory pipeline from the passes of VMEM_PASSES under base/<pass>/
+    (the last dispatch of each kernel), per CU and cycle: vector L1 (TCP) cache accesses,
+    load instructions (TA flat read wavefronts), accesses per instruction, texture-address
+    (TA) and texture-data (TD) busy shares, their stalls on the L1, the L1's tag-conflict and
+    pending-miss stalls, and the mean L1-to-L2 read latency in cycles."""
+    vals = collections.defaultdict(dict)
+    for name, counters in VMEM_PASSES.items():
+        for c in counters.split():
+            for kern, v in read_counter(os.path.join(base, name), c).items():
+                vals[kern][c] = v[-1]
+    lines = ["| kernel | us | L1 accesses/clk | loads/clk | accesses/load | TA busy | "
+             "TA addr stalled by L1 | TD busy | TD stalled by L1 | L1 tag-conflict stall | "
+             "L1 pending stall | L2 read latency (clk) |",
+             "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for kern in sorted(vals):
+        v = vals[kern]
+        if "GRBM_GUI_ACTIVE" not in v or not kern.endswith("kernel"):
+            continue
+        cyc = v["GRBM_GUI_ACTIVE"] / xcds
+        per = lambda c: v.get(c, 0.0) / cus / cyc
+        acc, loads = per("TCP_TOTAL_CACHE_ACCESSES_sum"), per("TA_FLAT_READ_WAVEFRONTS_sum")
+        req = v.get("TCP_TCC_READ_REQ_sum", 0.0)
+        lat = v.get("TCP_TCC_READ_REQ_LATENCY_sum", 0.0) / req if req else float("nan")
+        lines.append(
+            f"| {kern} | {cyc / 2.4e3:.0f} | {acc:.2f} | {loads:.3f} | "
+            f"{acc / loads if loads else float('nan'):.1f} | {per('TA_TA_BUSY_sum'):.2f} | "
+            f"{per('TA_ADDR_STALLED_BY_TC_CYCLES_sum'):.2f} | {per('TD_TD_BUSY_sum'):.2f} | "
+            f"{per('TD_TC_STALL_sum'):.2f} | {per('TCP_READ_TAGCONFLICT_STALL_CYCLES_sum'):.2f} | "
+            f"{per('TCP_PENDING_STALL_CYCLES_sum'):.2f} | {lat:.0f} |")
+    return "\n".join(lines) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -182,7 +226,23 @@ def main():
     c.add_argument("base", help="directory holding one run per pass: sq1/ sq2/ cache/")
     c.add_argument("--out", required=True, help="output path without extension")
     c.add_argument("--command", default="", help="the profiled command, for the summary")
+    m = sub.add_parser("vmem")
+    m.add_argument("base", help="directory holding one run per pass: l1a/ l1b/")
+    m.add_argument("--out", required=True, help="output path without extension")
+    m.add_argument("--command", default="", help="the profiled command, for the summary")
     a = ap.parse_args()
+    if a.cmd == "vmem":
+        text = (f"# Vector-memory pipeline per kernel\n\nCommand per pass: `{a.command}` with "
+                f"`--pmc` set to each of:\n\n" +
+                "".join(f"- {k}: `{v}`\n" for k, v in VMEM_PASSES.items()) +
+                "\nPer CU and cycle (GRBM_GUI_ACTIVE over the 8 XCDs, 256 CUs): vector-L1 cache "
+                "accesses (tag lookups), vector load instructions, and the busy and stall "
+                "shares of the texture-address (TA) and texture-data (TD) units and the L1.\n\n"
+                + vmem_table(a.base))
+        with open(a.out + ".md", "w") as fh:
+            fh.write(text)
+        print(text)
+        return
     if a.cmd == "counters":
         text = (f"# Counter summary per kernel\n\nCommand per pass: `{a.command}` with "
                 f"`--pmc` set to each of:\n\n" +
