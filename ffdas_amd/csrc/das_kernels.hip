@@ -2021,8 +2021,10 @@ hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl
 
 // Taps whose gathers a frame-inner forward wave issues together (1, 2 or 4).  Measured on
 // the 2-D fwd+adjoint benchmark (16 frames per chunk, frame-group planes): FP16 storage
-// 0.280 ms (1 tap), 0.230 (2), 0.219 (4); FP32 storage 0.494 / 0.436 / 0.429 ms with the
-// earlier row-pair layout.  Complex FP32 at 4 taps spills registers.  FFDAS_TAPS overrides.
+// 0.280 ms (1 tap), 0.230 (2), 0.219 (4); FP32 storage (row-pair layout) 0.494 / 0.436 /
+// 0.429 ms.  Complex FP32 at 4 taps spills registers.  The parameter adjoint keeps one tap
+// per group: 2 and 4 measured 0.302 and 0.292 ms against 0.238 (its registers drop it to 3
+// and 2 waves per SIMD).  FFDAS_TAPS overrides.
 template <Storage ST, bool CPLX>
 int tap_group() {
     static const int env = [] {
@@ -2057,11 +2059,11 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
             kern<<<grid, 64 * G, lds, st>>>(g, tl, x, out, os_b, br, bi);
             return hipGetLastError();
         };
-        switch (tap_group<ST, CPLX>()) {
-            case 4: return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 4>);
-            case 2: return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 2>);
-            default: return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 1>);
-        }
+        const int tb = tap_group<ST, CPLX>();
+        if constexpr (!(CPLX && ST == Storage::F32))
+            if (tb == 4) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 4>);
+        if (tb >= 2) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 2>);
+        return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 1>);
     }
 }
 
@@ -2158,12 +2160,17 @@ hipError_t launch_fwd_batched_f(hipStream_t st, const std::vector<BatchMember> &
                 kern<<<(unsigned)start, 64 * G, lds, st>>>(args);
                 return hipGetLastError();
             };
-            hipError_t e;
-            switch (tap_group<ST, CPLX>()) {
-                case 4: e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 4>); break;
-                case 2: e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 2>); break;
-                default: e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 1>); break;
-            }
+            const int tb = tap_group<ST, CPLX>();
+            hipError_t e = hipSuccess;
+            bool done = false;
+            if constexpr (!(CPLX && ST == Storage::F32))
+                if (tb == 4) {
+                    e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 4>);
+                    done = true;
+                }
+            if (!done)
+                e = tb >= 2 ? go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 2>)
+                            : go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 1>);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
