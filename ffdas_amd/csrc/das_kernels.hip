@@ -80,15 +80,18 @@ struct NaturalRows {
     }
 };
 
-// Sixteen zero bytes: the gather address of a tap that reads nothing, so that the loads of a
-// group of taps need no branch.  Never written.
-__device__ __align__(16) uint4 kZeroRows[1];
-
 // Frame-inner layout, frames in groups of GS (written by the relayout kernels): per chunk
 // [C][T][NG][S][GS][CW], NG = FT / GS, with GS * CW elements = 8 bytes.  One 16-byte load at
 // sample k of plane g holds rows k and k+1 of the group's frames, so the NG loads of a tap
 // step through planes S * 8 bytes apart, and the lanes of a wave (neighbouring voxels, close
 // sample indices) fall on few cache lines: a 128-byte line holds 16 samples of a plane.
+// The rows of one channel are read through a buffer resource spanning that channel's block
+// (below 2^31 bytes, which the host checks): the plane step is a scalar offset, and a tap
+// that reads nothing takes an offset past the block, for which the hardware returns zeros,
+// so the loads of a group of taps need no branch and no select of addresses.
+constexpr int kBufWord3 = 0x00020000;          // raw buffer, 32-bit data format
+constexpr uint32_t kNoRows = 0x80000000u;      // beyond every channel block
+
 template <Storage ST, int FT, int CW>
 struct FrameInnerRows {
     using raw = typename Elem<ST>::raw;
@@ -96,28 +99,39 @@ struct FrameInnerRows {
     static constexpr int NG = FT / GS;                          // planes per (channel, transmit)
     static constexpr int NQ = NG;                               // 16-byte loads per tap
     static_assert(GS >= 1 && FT % GS == 0, "frame groups must tile the frames");
+    struct Ref {
+        __amdgpu_buffer_rsrc_t rows;   // the channel's block
+        uint32_t off;                  // byte offset of rows k, k+1 of plane 0
+    };
     const raw *base;
     int T, S;
     __device__ __forceinline__ FrameInnerRows(const void *xv, int chunk, int C, int T_, int S_)
         : T(T_), S(S_) {
         base = static_cast<const raw *>(xv) + (int64_t)chunk * C * T_ * S_ * FT * CW;
     }
-    // element offset of sample k of plane 0 of row (m, s)
-    __device__ __forceinline__ size_t offset(int m, int s, int k) const {
-        return ((size_t)(m * T + s) * NG * S + (size_t)k) * GS * CW;
+    __device__ __forceinline__ Ref offset(int m, int s, int k) const {
+        const int mu = __builtin_amdgcn_readfirstlane(m);       // one channel per wave
+        const raw *blk = base + (size_t)mu * T * S * FT * CW;
+        Ref r;
+        r.rows = __builtin_amdgcn_make_buffer_rsrc(const_cast<raw *>(blk), (short)0,
+                                                   T * S * FT * CW * (int)sizeof(raw), kBufWord3);
+        r.off = ((uint32_t)s * NG * (uint32_t)S + (uint32_t)k) * 8u;
+        return r;
     }
-    __device__ __forceinline__ void fetch(size_t off, uint4 (&q)[NQ]) const {
-        const raw *p = base + off;
-#pragma unroll
-        for (int g = 0; g < NG; g++)
-            __builtin_memcpy(&q[g], p + (size_t)g * S * GS * CW, 16);
+    __device__ __forceinline__ void fetch(const Ref &r, uint4 (&q)[NQ]) const {
+        fetch_at(r.rows, r.off, q);
     }
     // rows k, k+1 of (m, s), or zeros when the tap reads nothing
-    __device__ __forceinline__ void fetch_or_zero(bool in, size_t off, uint4 (&q)[NQ]) const {
-        const raw *p = in ? base + off : reinterpret_cast<const raw *>(kZeroRows);
-        const size_t step = in ? (size_t)S * GS * CW : 0;
+    __device__ __forceinline__ void fetch_or_zero(bool in, const Ref &r, uint4 (&q)[NQ]) const {
+        fetch_at(r.rows, in ? r.off : kNoRows, q);
+    }
+    __device__ __forceinline__ void fetch_at(__amdgpu_buffer_rsrc_t rows, uint32_t off,
+                                             uint4 (&q)[NQ]) const {
 #pragma unroll
-        for (int g = 0; g < NG; g++) __builtin_memcpy(&q[g], p + g * step, 16);
+        for (int g = 0; g < NG; g++) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rows, (int)off, g * S * 8, 0);
+            q[g] = make_uint4(v[0], v[1], v[2], v[3]);
+        }
     }
     // v[h][f][c]: row k + h, frame f = g * GS + i, part c
     template <class A>
@@ -136,9 +150,9 @@ struct FrameInnerRows {
         }
     }
     template <class A>
-    __device__ __forceinline__ void load(size_t off, A (&v)[2][FT][CW]) const {
+    __device__ __forceinline__ void load(const Ref &r, A (&v)[2][FT][CW]) const {
         uint4 q[NQ];
-        fetch(off, q);
+        fetch(r, q);
         unpack(q, v);
     }
 };
@@ -219,9 +233,9 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
 
 // Frame-inner forward with the loads of TB taps in flight together: each group of TB taps
 // issues all its 16-byte loads before the first of them is used, so a wave keeps TB times
-// more gathers in flight than tap by tap.  A tap that reads nothing loads two rows of zeros
-// (one cache line for every such lane) with its weights zeroed, so it adds exactly zero and
-// no branch separates the loads from their use.
+// more gathers in flight than tap by tap.  A tap that reads nothing loads zeros (an offset
+// past the channel's block) with its weights zeroed, so it adds exactly zero and no branch
+// separates the loads from their use.
 template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
 __device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRows<ST, FT, CPLX ? 2 : 1> &xr,
                                                 int n, int m0, int mstep,
@@ -2022,7 +2036,7 @@ hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl
 // Taps whose gathers a frame-inner forward wave issues together (1, 2 or 4).  Measured on
 // the 2-D fwd+adjoint benchmark (16 frames per chunk, frame-group planes): FP16 storage
 // 0.280 ms (1 tap), 0.230 (2), 0.219 (4); FP32 storage (row-pair layout) 0.494 / 0.436 /
-// 0.429 ms.  Complex FP32 at 4 taps spills registers.  The parameter adjoint keeps one tap
+// 0.429 ms.  Complex data at 4 taps spills registers.  The parameter adjoint keeps one tap
 // per group: 2 and 4 measured 0.302 and 0.292 ms against 0.238 (its registers drop it to 3
 // and 2 waves per SIMD).  FFDAS_TAPS overrides.
 template <Storage ST, bool CPLX>
@@ -2031,7 +2045,7 @@ int tap_group() {
         const char *e = std::getenv("FFDAS_TAPS");
         return e ? std::atoi(e) : 0;
     }();
-    if (env > 0) return env >= 4 && !(CPLX && ST == Storage::F32) ? 4 : (env >= 2 ? 2 : 1);
+    if (env > 0) return env >= 4 && !CPLX ? 4 : (env >= 2 ? 2 : 1);
     return CPLX ? 2 : 4;
 }
 
@@ -2060,7 +2074,7 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
             return hipGetLastError();
         };
         const int tb = tap_group<ST, CPLX>();
-        if constexpr (!(CPLX && ST == Storage::F32))
+        if constexpr (!CPLX)
             if (tb == 4) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 4>);
         if (tb >= 2) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 2>);
         return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 1>);
@@ -2163,7 +2177,7 @@ hipError_t launch_fwd_batched_f(hipStream_t st, const std::vector<BatchMember> &
             const int tb = tap_group<ST, CPLX>();
             hipError_t e = hipSuccess;
             bool done = false;
-            if constexpr (!(CPLX && ST == Storage::F32))
+            if constexpr (!CPLX)
                 if (tb == 4) {
                     e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 4>);
                     done = true;
@@ -2618,8 +2632,10 @@ int fi_frames_for(const DasProblem &p, ffdas_alg_t alg) {
     if (const char *env = std::getenv("FFDAS_LAYOUT"))
         if (std::strcmp(env, "natural") == 0) return 0;
     const int esz = (int)storage_bytes(p.k_storage) * (p.cplx ? 2 : 1);
+    // the gathers address one channel's rows ((T, S, F) samples) with 32-bit offsets
+    auto fits = [&](int F) { return (double)p.T * p.S * F * esz < 2147483648.0 - 64.0; };
     const int want = p.tune.fi_frames;
-    if ((want == 4 || want == 8 || want == 16) && (want * esz) % 16 == 0 &&
+    if ((want == 4 || want == 8 || want == 16) && (want * esz) % 16 == 0 && fits(want) &&
         (double)((p.B + want - 1) / want) * want <= 2.0 * p.B)
         return want;
     // cost ~ padded frames * (1 + G / F): per-voxel delay arithmetic is worth about G
@@ -2628,7 +2644,7 @@ int fi_frames_for(const DasProblem &p, ffdas_alg_t alg) {
     int best = 0;
     double best_cost = 0.0;
     for (int F : {16, 8, 4}) {
-        if ((F * esz) % 16 != 0) continue;
+        if ((F * esz) % 16 != 0 || !fits(F)) continue;
         const double padded = (double)((p.B + F - 1) / F) * F;
         if (padded > 2.0 * p.B) continue;
         const double cost = padded * (1.0 + G / F);
