@@ -148,13 +148,12 @@ struct Tap {
 
 // tau = scale * (ofs + r)
 template <bool PRECISE>
-__device__ __forceinline__ Tap make_tap(float ofs, const Radius &R, float scale, bool unit_scale) {
+__device__ __forceinline__ Tap make_tap(float ofs, const Radius &R, float scale) {
 #pragma clang fp contract(off)
     Tap tp;
     if constexpr (!PRECISE) {
         float tau = ofs + R.rh;
         tau *= scale;                      // exact for scale == 1
-        (void)unit_scale;
         tp.kf = __builtin_floorf(tau);
         tp.t = tau - tp.kf;
     } else {
@@ -164,12 +163,12 @@ __device__ __forceinline__ Tap make_tap(float ofs, const Radius &R, float scale,
         float ap = th - bp;
         float tl = (ofs - ap) + (R.rh - bp);
         tl += R.rl;
-        if (!unit_scale) {
-            float sh = scale * th;
-            float sl = __builtin_fmaf(scale, th, -sh);
-            tl = __builtin_fmaf(scale, tl, sl);
-            th = sh;
-        }
+        // times scale, the product error kept in tl; for scale == 1 this leaves th and tl
+        // unchanged (sl = 0), so it needs no branch
+        const float sh = scale * th;
+        const float sl = __builtin_fmaf(scale, th, -sh);
+        tl = __builtin_fmaf(scale, tl, sl);
+        th = sh;
         float kf = __builtin_floorf(th);
         float t = (th - kf) + tl;
         float c = __builtin_floorf(t);
@@ -269,7 +268,6 @@ struct Geo {
     const int *sparse;     // (K,N) or null
     int C, T, S, K, B, N;
     float scale;
-    int unit_scale;
     float rev_h, rev_l;    // wavenum / 2pi
     float wavenum;
 };

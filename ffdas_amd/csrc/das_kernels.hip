@@ -170,7 +170,6 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
     for (int f = 0; f < FT; f++)
 #pragma unroll
         for (int c = 0; c < CW; c++) acc[f][c] = A(0);
-    const bool unit = g.unit_scale != 0;
 
     for (int kc = 0; kc < g.K; kc += KC) {
         float ofs[KC], wt[KC];
@@ -200,7 +199,7 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
 #pragma unroll
             for (int j = 0; j < KC; j++) {
                 if (kc + j >= g.K) break;
-                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale);
                 const bool in = dok && (uint32_t)tp.k < (uint32_t)(g.S - 1) &&
                                 (uint32_t)sq[j] < (uint32_t)g.T;
                 if (in) {
@@ -250,7 +249,6 @@ __device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRo
 #pragma unroll
         for (int c = 0; c < CW; c++) acc[f][c] = A(0);
     if (g.S < 2) return;                       // no tap has two rows to read
-    const bool unit = g.unit_scale != 0;
 
     for (int kc = 0; kc < g.K; kc += KC) {
         float ofs[KC], wt[KC];
@@ -286,7 +284,7 @@ __device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRo
 #pragma unroll
                 for (int u = 0; u < TB; u++) {
                     const int j = j0 + u;
-                    tp[u] = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                    tp[u] = make_tap<PRECISE>(ofs[j], R, g.scale);
                     in[u] = dok && (uint32_t)tp[u].k < (uint32_t)(g.S - 1) &&
                             (uint32_t)sq[j] < (uint32_t)g.T;
                     xr.fetch_or_zero(in[u], xr.offset(m, sq[j], tp[u].k), q[u]);
@@ -505,7 +503,6 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
         for (int c = 0; c < CW; c++)
             gv[f][c] = (b < g.B) ? gout[((int64_t)b * gs_b + n) * CW + c] : A(0);
     }
-    const bool unit = g.unit_scale != 0;
 
     for (int kc = 0; kc < g.K; kc += KC) {
         float ofs[KC], wt[KC];
@@ -538,7 +535,7 @@ __device__ __forceinline__ AccT<ACC64> das_bwd_param_body(const Geo &g, const Ro
 #pragma unroll
             for (int j = 0; j < KC; j++) {
                 if (kc + j >= g.K) break;
-                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale);
                 const bool in = dok && (uint32_t)tp.k < (uint32_t)(g.S - 1) &&
                                 (uint32_t)sq[j] < (uint32_t)g.T;
                 if (in) {
@@ -1230,7 +1227,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
         __syncthreads();
 
         auto scatter = [&](const Vox &v, const Radius &R, float ofs, float w) {
-            const Tap tp = make_tap<PRECISE>(ofs, R, g.scale, g.unit_scale != 0);
+            const Tap tp = make_tap<PRECISE>(ofs, R, g.scale);
             if ((uint32_t)tp.k >= (uint32_t)(g.S - 1)) return;
             const int s0 = lds_slot(tp.k), s1 = lds_slot(tp.k + 1);
             U64 *l0 = lds + s0 * PS;
@@ -1405,7 +1402,6 @@ __global__ __launch_bounds__(256) void das_bwd_x_plain_kernel(
         for (int c = 0; c < CW; c++)
             gv[f][c] = (b0 + f < g.B) ? gout[((int64_t)(b0 + f) * gs_b + n) * CW + c] : A(0);
     const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
-    const bool unit = g.unit_scale != 0;
     for (int kc = 0; kc < g.K; kc += KC) {
         float ofs[KC], wt[KC];
         int sq[KC];
@@ -1433,7 +1429,7 @@ __global__ __launch_bounds__(256) void das_bwd_x_plain_kernel(
 #pragma unroll
             for (int j = 0; j < KC; j++) {
                 if (kc + j >= g.K) break;
-                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale, unit);
+                const Tap tp = make_tap<PRECISE>(ofs[j], R, g.scale);
                 if (!((uint32_t)tp.k < (uint32_t)(g.S - 1) && (uint32_t)sq[j] < (uint32_t)g.T)) continue;
                 A *row = gx + ((int64_t)b0 * gx_b + (int64_t)m * gx_c + (int64_t)sq[j] * gx_t +
                                tp.k) * CW;
@@ -1592,7 +1588,6 @@ __global__ __launch_bounds__(1024) void das_bwd_x_sorted_kernel(
         sdx = g.srcdir[4 * m]; sdy = g.srcdir[4 * m + 1]; sdz = g.srcdir[4 * m + 2];
         sdw = g.srcdir[4 * m + 3];
     }
-    const bool unit = g.unit_scale != 0;
     const int kslots = g.sparse ? g.K : 1;
     const float *gchunk = gtf + (int64_t)chunk * g.N * F * CW;
 
@@ -1616,7 +1611,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_sorted_kernel(
             const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
             const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);
             if (g.srcdir && !((R.dx * sdx + R.dy * sdy + R.dz * sdz) * R.rinv > sdw)) continue;
-            const Tap tp = make_tap<PRECISE>(g.offsets[o], R, g.scale, unit);
+            const Tap tp = make_tap<PRECISE>(g.offsets[o], R, g.scale);
             if ((uint32_t)tp.k >= (uint32_t)(g.S - 1)) continue;
             const float w = g.weights[o];
             rk[q] = tp.k;
@@ -1836,7 +1831,6 @@ dev::Geo make_geo(const DasGeomArgs &a, const DasProblem &p) {
     g.offsets = a.offsets; g.weights = a.weights; g.sparse = p.sparse ? a.sparse_indices : nullptr;
     g.C = p.C; g.T = p.T; g.S = p.S; g.K = p.K; g.B = p.B; g.N = (int)p.N;
     g.scale = a.scale;
-    g.unit_scale = a.scale == 1.0f;
     g.wavenum = a.wavenum;
     const double rev = (double)a.wavenum / (2.0 * 3.14159265358979323846);
     g.rev_h = (float)rev;
