@@ -169,7 +169,7 @@ def counter_table(base: str, cus: int = 256, simds: int = 4, xcds: int = 8) -> s
 VMEM_PASSES = {
     "l1a": "TCP_PENDING_STALL_CYCLES_sum TCP_READ_TAGCONFLICT_STALL_CYCLES_sum "
            "TCP_TCR_TCP_STALL_CYCLES_sum TCP_UTCL1_TRANSLATION_MISS_sum "
-           "TA_ADDR_STALLED_BY_TC_CYCLES_sum TA_DATA_STALLED_BY_TC_CYCLES_sum GRBM_GUI_ACTIVE",
+           "TA_ADDR_STALLED_BY_TC_CYCLES_sum TA_BUFFER_READ_WAVEFRONTS_sum GRBM_GUI_ACTIVE",
     "l1b": "TCP_TCC_READ_REQ_LATENCY_sum TCP_TCC_READ_REQ_sum TCP_TOTAL_CACHE_ACCESSES_sum "
            "TCP_TCP_TA_DATA_STALL_CYCLES_sum TA_TA_BUSY_sum TA_FLAT_READ_WAVEFRONTS_sum "
            "TD_TD_BUSY_sum TD_TC_STALL_sum GRBM_GUI_ACTIVE",
@@ -179,7 +179,7 @@ VMEM_PASSES = {
 def vmem_table(base: str, cus: int = 256, xcds: int = 8) -> str:
     """Per-kernel vector-memory pipeline from the passes of VMEM_PASSES under base/<pass>/
     (the last dispatch of each kernel), per CU and cycle: vector L1 (TCP) cache accesses,
-    load instructions (TA flat read wavefronts), accesses per instruction, texture-address
+    load instructions (TA flat and buffer read wavefronts), accesses per instruction, texture-address
     (TA) and texture-data (TD) busy shares, their stalls on the L1, the L1's tag-conflict and
     pending-miss stalls, and the mean L1-to-L2 read latency in cycles."""
     vals = collections.defaultdict(dict)
@@ -197,7 +197,8 @@ def vmem_table(base: str, cus: int = 256, xcds: int = 8) -> str:
             continue
         cyc = v["GRBM_GUI_ACTIVE"] / xcds
         per = lambda c: v.get(c, 0.0) / cus / cyc
-        acc, loads = per("TCP_TOTAL_CACHE_ACCESSES_sum"), per("TA_FLAT_READ_WAVEFRONTS_sum")
+        acc = per("TCP_TOTAL_CACHE_ACCESSES_sum")
+        loads = per("TA_FLAT_READ_WAVEFRONTS_sum") + per("TA_BUFFER_READ_WAVEFRONTS_sum")
         req = v.get("TCP_TCC_READ_REQ_sum", 0.0)
         lat = v.get("TCP_TCC_READ_REQ_LATENCY_sum", 0.0) / req if req else float("nan")
         lines.append(
