@@ -230,7 +230,18 @@ struct Tiling {
     int A, H, W;        // spatial view (outer dims folded into A)
     int ta, th, tw;     // wave tile, ta*th*tw == 64
     int nta, nth, ntw;  // tile counts
+    int xcd;            // 1: blocks regrouped so that each XCD takes a contiguous range
 };
+
+// Workgroups are dispatched to the eight XCDs in turn (linear block id mod 8).  With xcd
+// set, linear id b is read as block (b mod 8) * (nb / 8) + b / 8 of the grid's order, so the
+// blocks of one XCD form a contiguous range of tiles and share that XCD's L2.  A bijection
+// on the complete rounds of 8; the last nb mod 8 blocks keep their ids.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int full = nb & ~7;
+    if (b >= full) return b;
+    return (b & 7) * (full >> 3) + (b >> 3);
+}
 
 __device__ __forceinline__ int voxel_of(const Tiling &tl, int tile, int lane, bool &valid) {
     const int ntiles = tl.nta * tl.nth * tl.ntw;
