@@ -192,6 +192,27 @@ def traffic_for(storage, kernel, summary=None):
         return None
 
 
+def counters_for(kernel, summary):
+    """Per-kernel counter figures (VALU busy, ...) from a committed counter summary
+    (profiles/<summary>, written by `python -m ffdas_amd.utils.pmc counters`), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", summary)) as fh:
+            return json.load(fh)["kernels"].get(kernel + "_kernel")
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def valu_onchip(kernel, summary):
+    """onchip object for a VALU-bound kernel: its VALU busy share from the counter summary
+    (SQ_INSTS_VALU x 4 cycles over CUs x SIMDs x the kernel's cycles), against 1."""
+    c = counters_for(kernel, summary)
+    busy = None if c is None else c.get("valu_busy")
+    return {"resource": "VALU issue", "unit": "busy share", "achieved": busy and round(busy, 3),
+            "peak": 1.0, "frac": busy and round(busy, 3),
+            "peak_source": f"profiles/{summary}: SQ_INSTS_VALU x 4 cycles / (256 CUs x 4 SIMDs "
+                           f"x kernel cycles), one counter run of the same workload"}
+
+
 def calibration_copy():
     """A 256 MiB device copy, twice: the FETCH/WRITE_SIZE counters' calibration."""
     import torch
@@ -365,10 +386,7 @@ def bench_config2(a):
                             "GBps": round(alg / tk / 1e9, 1),
                             "vs_hbm_peak": round(alg / tk / 1e9 / HBM_PEAK_GBS, 3)},
             "transmit_table_bytes": tables,
-            "onchip": {"resource": "vector L1", "unit": "B/clk/CU",
-                       "achieved": round(alg / tk / (2.4e9 * 256), 1), "peak": 64.0,
-                       "frac": round(alg / tk / (2.4e9 * 256) / 64.0, 3),
-                       "peak_source": "gfx950 vector L1 line rate, 64 B/clk/CU"},
+            "onchip": valu_onchip(dom, "das_config2_fp32_counters.json"),
             "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
     cpu = None
     if not a.no_cpu:
@@ -436,6 +454,12 @@ def bench_config5(a):
             f"{grid.shape[:-1]}, k={idx.shape[0]}")
     log(f"geometry and data in {time.time() - geo_s:.1f}s, U/ensemble {U:.4g}")
     run = lambda: ffdas_amd.das_batched(probs)
+    if a.profile_only:
+        for _ in range(a.profile_only):
+            run()
+        calibration_copy()
+        log(f"profile-only: {a.profile_only} calls and the calibration copy done")
+        return
     ms_eager, _ = time_steps(run, a.steps, a.warmup)
     cap = ffdas_amd.CapturedCall(run)
     ms_graph, wall = time_steps(cap.replay, a.steps, a.warmup)
@@ -448,7 +472,11 @@ def bench_config5(a):
     value = U / (ms_graph * 1e-3) / 1e9
     log(f"eager {ms_eager:.2f} ms, graph {ms_graph:.2f} ms per ensemble: {value:.1f} G U/s; "
         f"kernels {kt}")
-    achieved = U * 4 / (kt["das_fwd_fi_batched"] * 1e-3) / 1e9
+    dom = "das_fwd_fi_batched"
+    tk = kt[dom] * 1e-3
+    traffic = traffic_for("fp16", dom, summary="das_config5_fp16_traffic.json")
+    hbm = None if traffic is None else traffic / tk / 1e9
+    alg = U * 4
     cpu = None
     if not a.no_cpu:
         import oracle
@@ -479,10 +507,17 @@ def bench_config5(a):
                                  "<= 5 frames (tests/test_config5_gpu.py) and on 2 frames of "
                                  "the full probe geometries (tests/test_fullsize_gpu.py)",
                        "eager_ms": round(ms_eager, 3), "wall_s": round(wall, 3)},
-            "roofline": {"bound": "hbm", "kernel": "das_fwd_fi_batched",
-                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                         "bytes_per_unit": 4, "units_per_launch": U,
+            "roofline": {"bound": "hbm", "kernel": dom,
+                         "achieved": None if hbm is None else round(hbm, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": None if hbm is None else round(hbm / HBM_PEAK_GBS, 4),
+                         "traffic": traffic,
+                         "traffic_source": "profiles/das_config5_fp16_traffic.json",
+                         "algorithmic": {"bytes_per_unit": 4, "units_per_launch": U,
+                                         "bytes_per_launch": alg,
+                                         "GBps": round(alg / tk / 1e9, 1),
+                                         "vs_hbm_peak": round(alg / tk / 1e9 / HBM_PEAK_GBS, 3)},
+                         "onchip": valu_onchip(dom, "das_config5_fp16_counters.json"),
                          "kernel_ms": {k: round(v, 3) for k, v in kt.items()}},
             "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)

@@ -135,33 +135,48 @@ COUNTER_PASSES = {
 }
 
 
-def counter_table(base: str, cus: int = 256, simds: int = 4, xcds: int = 8) -> str:
+def counter_values(base: str, cus: int = 256, simds: int = 4, xcds: int = 8) -> dict:
     """Per-kernel utilisation from the passes of COUNTER_PASSES under base/<pass>/ (the last
-    dispatch of each kernel): duration from GRBM_GUI_ACTIVE (summed over the XCDs), VALU
-    busy = SQ_INSTS_VALU * 4 cycles / (CUs * SIMDs) / cycles, the share of cycles with an LDS
-    bank conflict per CU, L2 (TCC) and L1 (TCP) hit rates, TA busy."""
+    dispatch of each kernel; passes that were not run are skipped): duration from
+    GRBM_GUI_ACTIVE (summed over the XCDs), VALU busy = SQ_INSTS_VALU * 4 cycles / (CUs *
+    SIMDs) / cycles, the share of cycles with an LDS bank conflict per CU, L2 (TCC) and L1
+    (TCP) hit rates, TA busy."""
     vals = collections.defaultdict(dict)
     for name, counters in COUNTER_PASSES.items():
+        if not os.path.isdir(os.path.join(base, name)):
+            continue
         for c in counters.split():
             for kern, v in read_counter(os.path.join(base, name), c).items():
                 vals[kern][c] = v[-1]
-    lines = ["| kernel | us | VALU busy | LDS instr/CU | LDS bank-conflict cycles | "
-             "L2 hit | L1 hit | TA busy |", "|---|---|---|---|---|---|---|---|"]
+    out = {}
+    nan = float("nan")
     for kern in sorted(vals):
         v = vals[kern]
         if "GRBM_GUI_ACTIVE" not in v or not kern.endswith("kernel"):
             continue
         cyc = v["GRBM_GUI_ACTIVE"] / xcds
-        valu = v.get("SQ_INSTS_VALU", 0.0) * 4 / (cus * simds) / cyc
-        lds = v.get("SQ_INSTS_LDS", 0.0) / cus
-        bank = v.get("SQ_LDS_BANK_CONFLICT", 0.0) / cus / cyc
-        hit, miss = v.get("TCC_HIT_sum", 0.0), v.get("TCC_MISS_sum", 0.0)
-        l2 = hit / (hit + miss) if hit + miss else float("nan")
-        acc = v.get("TCP_TOTAL_CACHE_ACCESSES_sum", 0.0)
-        l1 = 1.0 - v.get("TCP_TCC_READ_REQ_sum", 0.0) / acc if acc else float("nan")
-        ta = v.get("TA_BUSY_avr", 0.0) / cyc
-        lines.append(f"| {kern} | {cyc / 2.4e3:.0f} | {valu:.2f} | {lds:.3g} | {bank:.2f} | "
-                     f"{l2:.2f} | {l1:.2f} | {ta:.2f} |")
+        hit, miss = v.get("TCC_HIT_sum"), v.get("TCC_MISS_sum")
+        acc = v.get("TCP_TOTAL_CACHE_ACCESSES_sum")
+        out[kern] = {
+            "us": cyc / 2.4e3,
+            "valu_busy": v["SQ_INSTS_VALU"] * 4 / (cus * simds) / cyc if "SQ_INSTS_VALU" in v else nan,
+            "lds_instr_per_cu": v["SQ_INSTS_LDS"] / cus if "SQ_INSTS_LDS" in v else nan,
+            "lds_bank_conflict": v["SQ_LDS_BANK_CONFLICT"] / cus / cyc if "SQ_LDS_BANK_CONFLICT" in v else nan,
+            "l2_hit": hit / (hit + miss) if hit is not None and miss is not None and hit + miss else nan,
+            "l1_hit": 1.0 - v.get("TCP_TCC_READ_REQ_sum", 0.0) / acc if acc else nan,
+            "ta_busy": v["TA_BUSY_avr"] / cyc if "TA_BUSY_avr" in v else nan,
+        }
+    return out
+
+
+def counter_table(base: str, cus: int = 256, simds: int = 4, xcds: int = 8) -> str:
+    """Markdown table of counter_values."""
+    lines = ["| kernel | us | VALU busy | LDS instr/CU | LDS bank-conflict cycles | "
+             "L2 hit | L1 hit | TA busy |", "|---|---|---|---|---|---|---|---|"]
+    for kern, c in counter_values(base, cus, simds, xcds).items():
+        lines.append(f"| {kern} | {c['us']:.0f} | {c['valu_busy']:.2f} | "
+                     f"{c['lds_instr_per_cu']:.3g} | {c['lds_bank_conflict']:.2f} | "
+                     f"{c['l2_hit']:.2f} | {c['l1_hit']:.2f} | {c['ta_busy']:.2f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -253,6 +268,10 @@ def main():
                 "kernel's cycles per CU.\n\n" + counter_table(a.base))
         with open(a.out + ".md", "w") as fh:
             fh.write(text)
+        clean = {k: {n: (None if v != v else v) for n, v in c.items()}       # NaN -> null
+                 for k, c in counter_values(a.base).items()}
+        with open(a.out + ".json", "w") as fh:
+            json.dump({"command": a.command, "kernels": clean}, fh, indent=1)
         print(text)
         return
     if a.cmd == "traffic":
