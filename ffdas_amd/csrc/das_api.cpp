@@ -54,7 +54,8 @@ bool xcopy_matches(const ffdas_context &h, const ffdas_tensor_desc &xd, const Da
     const auto &c = h.xcopy;
     return c.valid && c.x == x && c.ws == ws && c.dims == xd.dims && c.strides == xd.strides &&
            c.dtype == (int)xd.dtype && c.storage == (int)p.k_storage &&
-           c.fi_frames == p.fi_frames && c.cast == (p.x_needs_copy ? 1 : 0);
+           c.fi_frames == p.fi_frames && c.cast == (p.x_needs_copy ? 1 : 0) &&
+           c.sparse == (p.sparse ? 1 : 0);
 }
 
 void record_xcopy(ffdas_context &h, const ffdas_tensor_desc &xd, const DasProblem &p,
@@ -69,6 +70,7 @@ void record_xcopy(ffdas_context &h, const ffdas_tensor_desc &xd, const DasProble
     c.storage = (int)p.k_storage;
     c.fi_frames = p.fi_frames;
     c.cast = p.x_needs_copy ? 1 : 0;
+    c.sparse = p.sparse ? 1 : 0;     // the frame-inner layout depends on it
 }
 
 // The view of an already converted copy at `ws` (no launch).
@@ -364,8 +366,11 @@ static ffdas_error_t das_batched_call(ffdas_handle_t handle, int count,
         }
         std::vector<BatchMember> group;
         for (int j = i; j < count; j++) {
+            // one launch per storage type, complexness, frame chunk and layout (sparse
+            // problems read whole frame rows, dense ones frame-group planes)
             if (done[j] || ms[j].p.fi_frames != ms[i].p.fi_frames ||
-                ms[j].p.k_storage != ms[i].p.k_storage || ms[j].p.cplx != ms[i].p.cplx)
+                ms[j].p.k_storage != ms[i].p.k_storage || ms[j].p.cplx != ms[i].p.cplx ||
+                ms[j].p.sparse != ms[i].p.sparse)
                 continue;
             group.push_back(ms[j]);
             done[j] = true;

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 
 #include "das_device.h"
@@ -81,27 +82,39 @@ struct NaturalRows {
 };
 
 // Frame-inner layout, frames in groups of GS (written by the relayout kernels): per chunk
-// [C][T][NG][S][GS][CW], NG = FT / GS, with GS * CW elements = 8 bytes.  One 16-byte load at
-// sample k of plane g holds rows k and k+1 of the group's frames, so the NG loads of a tap
-// step through planes S * 8 bytes apart, and the lanes of a wave (neighbouring voxels, close
-// sample indices) fall on few cache lines: a 128-byte line holds 16 samples of a plane.
+// [C][T][NG][S][GS][CW], NG = FT / GS, RB = GS * CW * esz bytes per row of a group.  The
+// rows k and k+1 of a group are 2 * RB contiguous bytes, so a tap is NG * (2 * RB / 16)
+// 16-byte loads, the groups S * RB bytes apart.  Two layouts:
+// - planes (PL, dense problems): RB = 8 (4 FP16 or 2 FP32 frames per group).  The lanes of
+//   a wave (neighbouring voxels, close sample indices, the same (channel, transmit) row)
+//   then share cache lines, 16 samples of a plane per 128-byte line, which keeps the vector
+//   L1's accesses per load low;
+// - rows (!PL, sparse compounding): GS = FT, all frames of a row contiguous.  Neighbouring
+//   voxels compound different transmits at a slot, so the lanes share few lines either way,
+//   and one line per lane and tap instead of NG keeps the HBM traffic down.
 // The rows of one channel are read through a buffer resource spanning that channel's block
-// (below 2^31 bytes, which the host checks): the plane step is a scalar offset, and a tap
+// (below 2^31 bytes, which the host checks): the group step is a scalar offset, and a tap
 // that reads nothing takes an offset past the block, for which the hardware returns zeros,
 // so the loads of a group of taps need no branch and no select of addresses.
 constexpr int kBufWord3 = 0x00020000;          // raw buffer, 32-bit data format
 constexpr uint32_t kNoRows = 0x80000000u;      // beyond every channel block
 
-template <Storage ST, int FT, int CW>
+template <int FT, int CW, int ESZ, bool PL>
+constexpr int fi_group() { return PL ? 8 / (CW * ESZ) : FT; }
+
+template <Storage ST, int FT, int CW, bool PL>
 struct FrameInnerRows {
     using raw = typename Elem<ST>::raw;
-    static constexpr int GS = 8 / (CW * (int)sizeof(raw));    // frames per group
-    static constexpr int NG = FT / GS;                          // planes per (channel, transmit)
-    static constexpr int NQ = NG;                               // 16-byte loads per tap
-    static_assert(GS >= 1 && FT % GS == 0, "frame groups must tile the frames");
+    static constexpr int GS = fi_group<FT, CW, (int)sizeof(raw), PL>();   // frames per group
+    static constexpr int NG = FT / GS;                          // groups per (channel, transmit)
+    static constexpr int RB = GS * CW * (int)sizeof(raw);       // bytes of one row of a group
+    static constexpr int LP = 2 * RB / 16;                      // 16-byte loads per group
+    static constexpr int NQ = NG * LP;                          // 16-byte loads per tap
+    static constexpr int E = 16 / (int)sizeof(raw);             // elements per load
+    static_assert(GS >= 1 && FT % GS == 0 && (2 * RB) % 16 == 0, "frame groups must tile");
     struct Ref {
         __amdgpu_buffer_rsrc_t rows;   // the channel's block
-        uint32_t off;                  // byte offset of rows k, k+1 of plane 0
+        uint32_t off;                  // byte offset of rows k, k+1 of group 0
     };
     const raw *base;
     int T, S;
@@ -115,7 +128,7 @@ struct FrameInnerRows {
         Ref r;
         r.rows = __builtin_amdgcn_make_buffer_rsrc(const_cast<raw *>(blk), (short)0,
                                                    T * S * FT * CW * (int)sizeof(raw), kBufWord3);
-        r.off = ((uint32_t)s * NG * (uint32_t)S + (uint32_t)k) * 8u;
+        r.off = ((uint32_t)s * NG * (uint32_t)S + (uint32_t)k) * (uint32_t)RB;
         return r;
     }
     __device__ __forceinline__ void fetch(const Ref &r, uint4 (&q)[NQ]) const {
@@ -128,10 +141,13 @@ struct FrameInnerRows {
     __device__ __forceinline__ void fetch_at(__amdgpu_buffer_rsrc_t rows, uint32_t off,
                                              uint4 (&q)[NQ]) const {
 #pragma unroll
-        for (int g = 0; g < NG; g++) {
-            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rows, (int)off, g * S * 8, 0);
-            q[g] = make_uint4(v[0], v[1], v[2], v[3]);
-        }
+        for (int g = 0; g < NG; g++)
+#pragma unroll
+            for (int l = 0; l < LP; l++) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(rows, (int)(off + 16 * l),
+                                                                      g * S * RB, 0);
+                q[g * LP + l] = make_uint4(v[0], v[1], v[2], v[3]);
+            }
     }
     // v[h][f][c]: row k + h, frame f = g * GS + i, part c
     template <class A>
@@ -139,7 +155,7 @@ struct FrameInnerRows {
 #pragma unroll
         for (int g = 0; g < NG; g++) {
             raw r[2 * GS * CW];
-            __builtin_memcpy(r, &q[g], 16);
+            __builtin_memcpy(r, &q[g * LP], 2 * RB);
 #pragma unroll
             for (int h = 0; h < 2; h++)
 #pragma unroll
@@ -235,13 +251,13 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
 // more gathers in flight than tap by tap.  A tap that reads nothing loads zeros (an offset
 // past the channel's block) with its weights zeroed, so it adds exactly zero and no branch
 // separates the loads from their use.
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
-__device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRows<ST, FT, CPLX ? 2 : 1> &xr,
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB, bool PL>
+__device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRows<ST, FT, CPLX ? 2 : 1, PL> &xr,
                                                 int n, int m0, int mstep,
                                                 AccT<ACC64> (&acc)[FT][CPLX ? 2 : 1]) {
     static_assert(KC % TB == 0, "taps per group must divide taps per round");
     using A = AccT<ACC64>;
-    using Rows = FrameInnerRows<ST, FT, CPLX ? 2 : 1>;
+    using Rows = FrameInnerRows<ST, FT, CPLX ? 2 : 1, PL>;
     constexpr int CW = CPLX ? 2 : 1;
     const SplitPt yp = split_point(g.dstpos[3 * n], g.dstpos[3 * n + 1], g.dstpos[3 * n + 2]);
 #pragma unroll
@@ -374,7 +390,7 @@ __global__ __launch_bounds__(256) void das_fwd_kernel(Geo g, Tiling tl, const vo
 // w, w+G, ... and the G partial sums are combined through LDS, so a launch has G times more
 // waves in flight than tiles (latency hiding for the gathers) without repeating any delay
 // arithmetic.
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB, bool PL>
 __device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, const void *xv,
                                             void *outv, int64_t os_b, double beta_re,
                                             double beta_im, int tile, int chunk,
@@ -387,12 +403,12 @@ __device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, cons
     bool valid;
     int n = voxel_of(tl, tile, lane, valid);
     if (!valid) n = 0;          // computed, never stored: keeps every lane at the barrier
-    const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
+    const FrameInnerRows<ST, FT, CW, PL> xr(xv, chunk, g.C, g.T, g.S);
     A acc[FT][CW];
     if constexpr (TB == 1)
-        das_fwd_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, wave, G, acc);
+        das_fwd_body<FrameInnerRows<ST, FT, CW, PL>, ST, ACC64, CPLX, PRECISE, FT, KC>(g, xr, n, wave, G, acc);
     else
-        das_fwd_body_fi<ST, ACC64, CPLX, PRECISE, FT, KC, TB>(g, xr, n, wave, G, acc);
+        das_fwd_body_fi<ST, ACC64, CPLX, PRECISE, FT, KC, TB, PL>(g, xr, n, wave, G, acc);
     if (G == 1) {
         if (valid) store_out<ACC64, CPLX, FT>(acc, outv, os_b, n, chunk * FT, g.B, beta_re, beta_im);
         return;
@@ -432,7 +448,7 @@ __device__ __forceinline__ void fwd_fi_tile(const Geo &g, const Tiling &tl, cons
 }
 
 // grid.x = voxel tile, grid.y = frame chunk
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB>
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, int TB, bool PL>
 __global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const void *__restrict__ xv,
                                                          void *__restrict__ outv, int64_t os_b,
                                                          double beta_re, double beta_im) {
@@ -443,8 +459,8 @@ __global__ __launch_bounds__(512) void das_fwd_fi_kernel(Geo g, Tiling tl, const
         chunk = b / gridDim.x;
         tile = b - chunk * gridDim.x;
     }
-    fwd_fi_tile<ST, ACC64, CPLX, PRECISE, FT, KC, TB>(g, tl, xv, outv, os_b, beta_re, beta_im,
-                                                   tile, chunk, fwd_lds);
+    fwd_fi_tile<ST, ACC64, CPLX, PRECISE, FT, KC, TB, PL>(g, tl, xv, outv, os_b, beta_re, beta_im,
+                                                       tile, chunk, fwd_lds);
 }
 
 // Several problems (mixed probes, grids, frame counts, sparse tables) in one launch.  The
@@ -467,7 +483,7 @@ struct BatchArgs {
     BatchItem item[kMaxBatch];
 };
 
-template <Storage ST, bool CPLX, bool PRECISE, int FT, int KC, int TB>
+template <Storage ST, bool CPLX, bool PRECISE, int FT, int KC, int TB, bool PL>
 __global__ __launch_bounds__(512) void das_fwd_fi_batched_kernel(BatchArgs args) {
     extern __shared__ unsigned char fwd_lds[];
     const int b = args.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -476,7 +492,7 @@ __global__ __launch_bounds__(512) void das_fwd_fi_batched_kernel(BatchArgs args)
     const BatchItem &it = args.item[i];
     const int local = b - it.start;
     const int chunk = local / it.tiles;
-    fwd_fi_tile<ST, false, CPLX, PRECISE, FT, KC, TB>(it.g, it.tl, it.x, it.out, it.os_b, it.beta_re,
+    fwd_fi_tile<ST, false, CPLX, PRECISE, FT, KC, TB, PL>(it.g, it.tl, it.x, it.out, it.os_b, it.beta_re,
                                                    it.beta_im, local - chunk * it.tiles, chunk,
                                                    fwd_lds);
 }
@@ -692,7 +708,7 @@ __global__ __launch_bounds__(256) void das_bwd_param_kernel(
 }
 
 // Frame-inner layout, G waves per voxel tile splitting the channels (as the forward).
-template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC>
+template <Storage ST, bool ACC64, bool CPLX, bool PRECISE, int FT, int KC, bool PL>
 __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     Geo g, Tiling tl, const void *__restrict__ xv, const void *__restrict__ goutv, int64_t gs_b,
     float *__restrict__ gofs, float *__restrict__ gwt, int64_t pstride,
@@ -711,10 +727,10 @@ __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
     bool valid;
     int n = voxel_of(tl, tile, lane, valid);
     if (!valid) n = 0;          // computed, never stored: keeps every lane at the barriers
-    const FrameInnerRows<ST, FT, CW> xr(xv, chunk, g.C, g.T, g.S);
+    const FrameInnerRows<ST, FT, CW, PL> xr(xv, chunk, g.C, g.T, g.S);
     float *go = gofs ? gofs + chunk * pstride : nullptr;
     float *gw = gwt ? gwt + chunk * pstride : nullptr;
-    AccT<ACC64> part = das_bwd_param_body<FrameInnerRows<ST, FT, CW>, ST, ACC64, CPLX, PRECISE, FT, KC>(
+    AccT<ACC64> part = das_bwd_param_body<FrameInnerRows<ST, FT, CW, PL>, ST, ACC64, CPLX, PRECISE, FT, KC>(
         g, xr, n, chunk * FT, goutv, gs_b, go, gw, wave, G, valid,
         reinterpret_cast<AccT<ACC64> *>(param_lds));
     if (!valid) part = 0;
@@ -725,7 +741,7 @@ __global__ __launch_bounds__(512) void das_bwd_param_fi_kernel(
 // src[chunk*FT + g*GS + i][c][t][k] (zero for frames past B), GS frames per group as read by
 // FrameInnerRows.  One block handles a 64-sample segment of one (chunk, c, t) row for all FT
 // frames, transposing through LDS so reads and writes are both contiguous.
-template <Storage SS, class DST, int FT, int CW>
+template <Storage SS, class DST, int FT, int CW, int GS>
 __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS>::raw *__restrict__ src, int64_t sb,
                                                           int64_t sc, int64_t st, int B, int C,
                                                           int T, int S, DST *__restrict__ dst) {
@@ -749,7 +765,6 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
         tile[f][e] = v;
     }
     __syncthreads();
-    constexpr int GS = 8 / (CW * (int)sizeof(DST));
     DST *d = dst + ((((int64_t)chunk * C + c) * T + t) * S) * FT * CW;
     for (int i = threadIdx.x; i < 64 * FT * CW; i += blockDim.x) {
         // i = ((g * 64 + kk) * GS + fi) * CW + cc: consecutive threads write consecutive elements
@@ -768,7 +783,7 @@ __global__ __launch_bounds__(256) void relayout_fi_kernel(const typename Elem<SS
 // samples of GS frames per vector -- written as 16-byte vectors), transposing through LDS.
 constexpr int kRelL = 128;
 
-template <Storage SS, class DST, int FT, int CW>
+template <Storage SS, class DST, int FT, int CW, int GS>
 __global__ __launch_bounds__(256) void relayout_fi_vec_kernel(
     const typename Elem<SS>::raw *__restrict__ src, int64_t sb, int64_t sc, int64_t st, int B,
     int C, int T, int S, DST *__restrict__ dst) {
@@ -803,24 +818,25 @@ __global__ __launch_bounds__(256) void relayout_fi_vec_kernel(
         for (int j = 0; j < SV; j++) tile[f][e + j] = v[j];
     }
     __syncthreads();
-    constexpr int GS = 8 / (CW * (int)sizeof(DST));
-    static_assert(DV == 2 * GS * CW, "a vector holds two samples of one plane");
+    // each group's segment, kmax samples of GS * CW elements, is contiguous: written as
+    // 16-byte vectors (the caller ensures kmax * GS * CW is a multiple of DV)
     DST *d = dst + (((int64_t)chunk * C + c) * T + t) * S * FT * CW;
     const int k0 = seg * kRelL;
     const int kmax = min(kRelL, S - k0);            // samples of this segment inside the row
-    const int per_plane = kmax / 2;                 // vectors per plane (S is even)
-    for (int i = threadIdx.x; i < FT / GS * per_plane; i += blockDim.x) {
-        const int g = i / per_plane, kk = 2 * (i - g * per_plane);
+    const int per_group = kmax * GS * CW / DV;      // vectors per group
+    for (int i = threadIdx.x; i < FT / GS * per_group; i += blockDim.x) {
+        const int g = i / per_group, e0 = (i - g * per_group) * DV;
         DST o[DV];
 #pragma unroll
         for (int j = 0; j < DV; j++) {
-            const int h = j / (GS * CW), r = j - h * GS * CW;
+            const int e = e0 + j;
+            const int kk = e / (GS * CW), r = e - kk * GS * CW;
             const int fi = r / CW, cc = r - fi * CW;
-            o[j] = convert_to<DST>(tile[g * GS + fi][(kk + h) * CW + cc]);
+            o[j] = convert_to<DST>(tile[g * GS + fi][kk * CW + cc]);
         }
         uint4 q;
         __builtin_memcpy(&q, o, 16);
-        *reinterpret_cast<uint4 *>(d + (((int64_t)g * S + k0 + kk) * GS) * CW) = q;
+        *reinterpret_cast<uint4 *>(d + ((int64_t)g * S + k0) * GS * CW + e0) = q;
     }
 }
 
@@ -2053,6 +2069,13 @@ hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl
     }
 }
 
+// Frame-inner layout of a problem: frame-group planes for dense problems, whole rows for
+// sparse compounding (see FrameInnerRows).  On the 2-D benchmark (dense) the planes took the
+// FP16 forward from 0.268 to 0.219 ms; on the batched sparse benchmark the rows fetch 2.56 TB
+// per ensemble beyond L2 against 3.44 TB for the planes, 498 against 574 ms.
+bool fi_planes(const DasProblem &p) { return !p.sparse; }
+bool fi_planes(const dev::Geo &g) { return g.sparse == nullptr; }
+
 // Taps whose gathers a frame-inner forward wave issues together (1, 2 or 4).  Measured on
 // the 2-D fwd+adjoint benchmark (16 frames per chunk, frame-group planes): FP16 storage
 // 0.280 ms (1 tap), 0.230 (2), 0.219 (4); FP32 storage (row-pair layout) 0.494 / 0.436 /
@@ -2093,11 +2116,15 @@ hipError_t launch_fwd_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling 
             kern<<<grid, 64 * G, lds, st>>>(g, tl, x, out, os_b, br, bi);
             return hipGetLastError();
         };
-        const int tb = tap_group<ST, CPLX>();
-        if constexpr (!CPLX)
-            if (tb == 4) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 4>);
-        if (tb >= 2) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 2>);
-        return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 1>);
+        auto by_taps = [&](auto pl) {
+            constexpr bool PL = decltype(pl)::value;
+            const int tb = tap_group<ST, CPLX>();
+            if constexpr (!CPLX)
+                if (tb == 4) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 4, PL>);
+            if (tb >= 2) return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 2, PL>);
+            return go(dev::das_fwd_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, 1, PL>);
+        };
+        return fi_planes(g) ? by_taps(std::true_type{}) : by_taps(std::false_type{});
     }
 }
 
@@ -2124,14 +2151,19 @@ hipError_t launch_bwdp_fi_f(hipStream_t st, const dev::Geo &g, const dev::Tiling
         dim3 grid(ntiles, (g.B + F - 1) / F);
         const int G = channel_groups(g, (int64_t)grid.x * grid.y, groups);
         const size_t lds = G > 1 ? (size_t)G * 2 * kKC * 64 * sizeof(float) : 0;
-        if (lds > 0)
-            (void)hipFuncSetAttribute((const void *)dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        ProfScope ps(st, "das_bwd_param_fi");
-        dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC><<<grid, 64 * G, lds, st>>>(
-            g, tl, x, gout, gs_b, grid.y > 1 ? po.pofs : po.gofs, grid.y > 1 ? po.pwt : po.gwt,
-            grid.y > 1 ? (int64_t)g.K * g.N : 0, po.gscale ? po.spart : nullptr);
-        const hipError_t e = hipGetLastError();
+        auto go = [&](auto kern) {
+            if (lds > 0)
+                (void)hipFuncSetAttribute((const void *)kern,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            ProfScope ps(st, "das_bwd_param_fi");
+            kern<<<grid, 64 * G, lds, st>>>(
+                g, tl, x, gout, gs_b, grid.y > 1 ? po.pofs : po.gofs, grid.y > 1 ? po.pwt : po.gwt,
+                grid.y > 1 ? (int64_t)g.K * g.N : 0, po.gscale ? po.spart : nullptr);
+            return hipGetLastError();
+        };
+        const hipError_t e = fi_planes(g)
+                                 ? go(dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, true>)
+                                 : go(dev::das_bwd_param_fi_kernel<ST, false, CPLX, PRECISE, F, kKC, false>);
         if (e != hipSuccess) return e;
         return finish_params(st, po, (int)grid.y, (int64_t)g.K * g.N, (int)(grid.x * grid.y));
     }
@@ -2195,17 +2227,17 @@ hipError_t launch_fwd_batched_f(hipStream_t st, const std::vector<BatchMember> &
                 kern<<<(unsigned)start, 64 * G, lds, st>>>(args);
                 return hipGetLastError();
             };
-            const int tb = tap_group<ST, CPLX>();
-            hipError_t e = hipSuccess;
-            bool done = false;
-            if constexpr (!CPLX)
-                if (tb == 4) {
-                    e = go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 4>);
-                    done = true;
-                }
-            if (!done)
-                e = tb >= 2 ? go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 2>)
-                            : go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 1>);
+            // the members of one launch share the layout (the caller groups them so)
+            auto by_taps = [&](auto pl) {
+                constexpr bool PL = decltype(pl)::value;
+                const int tb = tap_group<ST, CPLX>();
+                if constexpr (!CPLX)
+                    if (tb == 4) return go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 4, PL>);
+                if (tb >= 2) return go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 2, PL>);
+                return go(dev::das_fwd_fi_batched_kernel<ST, CPLX, PRECISE, F, kKC, 1, PL>);
+            };
+            const hipError_t e = fi_planes(ms[off].p) ? by_taps(std::true_type{})
+                                                      : by_taps(std::false_type{});
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -2230,24 +2262,33 @@ hipError_t launch_relayout_f(hipStream_t st, const DasProblem &p, const void *x,
                      (p.xs_c * ES) % 16 == 0 && (p.xs_t * ES) % 16 == 0 &&
                      ((size_t)p.S * ES) % 16 == 0 && p.S % 2 == 0 &&
                      (F * CW * sizeof(DST)) % 16 == 0;
-    if (vec) {
-        const int64_t blocks = (int64_t)((p.S + dev::kRelL - 1) / dev::kRelL) * p.C * p.T;
+    // frame groups of the layout the gathers read: planes for dense problems, whole rows
+    // for sparse compounding (FrameInnerRows)
+    auto launch = [&](auto gs) {
+        constexpr int GS = decltype(gs)::value;
+        if (vec) {
+            const int64_t blocks = (int64_t)((p.S + dev::kRelL - 1) / dev::kRelL) * p.C * p.T;
+            if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+            dim3 grid((unsigned)blocks, (p.B + F - 1) / F);
+            ProfScope ps(st, "relayout_fi");
+            dev::relayout_fi_vec_kernel<SS, DST, F, CW, GS><<<grid, 256, 0, st>>>(
+                static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B,
+                p.C, p.T, p.S, static_cast<DST *>(dst));
+            return hipGetLastError();
+        }
+        const int64_t blocks = (int64_t)((p.S + 63) / 64) * p.C * p.T;
         if (blocks > 0x7fffffff) return hipErrorInvalidValue;
         dim3 grid((unsigned)blocks, (p.B + F - 1) / F);
         ProfScope ps(st, "relayout_fi");
-        dev::relayout_fi_vec_kernel<SS, DST, F, CW><<<grid, 256, 0, st>>>(
-            static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B,
-            p.C, p.T, p.S, static_cast<DST *>(dst));
+        dev::relayout_fi_kernel<SS, DST, F, CW, GS><<<grid, 256, 0, st>>>(
+            static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B, p.C,
+            p.T, p.S, static_cast<DST *>(dst));
         return hipGetLastError();
-    }
-    const int64_t blocks = (int64_t)((p.S + 63) / 64) * p.C * p.T;
-    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    dim3 grid((unsigned)blocks, (p.B + F - 1) / F);
-    ProfScope ps(st, "relayout_fi");
-    dev::relayout_fi_kernel<SS, DST, F, CW><<<grid, 256, 0, st>>>(
-        static_cast<const typename dev::Elem<SS>::raw *>(x), p.xs_b, p.xs_c, p.xs_t, p.B, p.C,
-        p.T, p.S, static_cast<DST *>(dst));
-    return hipGetLastError();
+    };
+    constexpr int ESZ = (int)sizeof(DST);
+    if (fi_planes(p))
+        return launch(std::integral_constant<int, dev::fi_group<F, CW, ESZ, true>()>{});
+    return launch(std::integral_constant<int, dev::fi_group<F, CW, ESZ, false>()>{});
 }
 
 template <Storage SS, class DST, bool CPLX>

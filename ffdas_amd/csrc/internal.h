@@ -63,7 +63,7 @@ struct ffdas_context {
         const void *x = nullptr;
         void *ws = nullptr;
         std::vector<int64_t> dims, strides;
-        int dtype = 0, storage = 0, fi_frames = 0, cast = 0;
+        int dtype = 0, storage = 0, fi_frames = 0, cast = 0, sparse = 0;
     } xcopy;
     bool reuse_x = false;
 
