@@ -4,7 +4,8 @@
 // its on-chip ceiling is the LDS atomic rate.  This probe measures that rate on the device
 // it runs on, for a given per-lane address pattern: every lane of every wave adds into the
 // LDS word pattern[lane] (plus a fixed multiple of 64 words per unrolled step, which keeps
-// the bank of each lane), 16 atomics per loop iteration, no returned values.  The caller
+// the bank of each lane), 16 atomics per loop iteration, no returned values; a lane whose
+// pattern entry is -1 takes part in no atomic (partly active instructions).  The caller
 // converts the time into lane-ops per clock per CU.
 //
 //   op 0: ds_add_u64 on 64-bit words      op 1: ds_add_u32 on 32-bit words
@@ -29,6 +30,7 @@ __global__ __launch_bounds__(512) void lds_atomic_probe(const int *__restrict__ 
     for (int i = threadIdx.x; i < kLdsBytes / 8; i += blockDim.x) lds[i] = 0ull;
     __syncthreads();
     const int off = pattern[threadIdx.x & 63];
+    if (off >= 0)
     for (int r = 0; r < reps; r++) {
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
@@ -47,8 +49,8 @@ __global__ __launch_bounds__(512) void lds_atomic_probe(const int *__restrict__ 
 }  // namespace
 
 // Times one launch of `blocks` workgroups of `threads` threads, each lane doing reps * 16
-// atomics at pattern[lane] (64 host ints, word offsets; the largest must stay below
-// (32 KiB / word size) - 15 * 64).  Writes the kernel time [ms] and returns a hipError_t
+// atomics at pattern[lane] (64 host ints, word offsets, -1 for an idle lane; the largest
+// must stay below (32 KiB / word size) - 15 * 64).  Writes the kernel time [ms] and returns a hipError_t
 // value (0 on success).  Synchronous; allocates and frees its own buffers.
 extern "C" __attribute__((visibility("default"))) int
 ffdas_calib_lds_atomics(int op, const int *pattern, int reps, int blocks, int threads,
@@ -58,7 +60,7 @@ ffdas_calib_lds_atomics(int op, const int *pattern, int reps, int blocks, int th
         return (int)hipErrorInvalidValue;
     const int words = op == 0 ? kLdsBytes / 8 : kLdsBytes / 4;
     for (int i = 0; i < 64; i++)
-        if (pattern[i] < 0 || pattern[i] + (kUnroll - 1) * kStrideWords >= words)
+        if (pattern[i] < -1 || pattern[i] + (kUnroll - 1) * kStrideWords >= words)
             return (int)hipErrorInvalidValue;
     int *dpat = nullptr;
     unsigned long long *sink = nullptr;

@@ -45,7 +45,8 @@ def device_info() -> tuple[int, float]:
 def lds_atomic_rate(op: str = "u64", pattern=None, reps: int = 256, blocks: int | None = None,
                     threads: int = 512) -> dict:
     """Lane-ops per clock per CU of LDS atomics `op` ("u64", "u32" or "f32") when lane l of
-    every wave adds at word pattern[l] (default: lane l -> word l, distinct banks)."""
+    every wave adds at word pattern[l] (default: lane l -> word l, distinct banks; -1: the
+    lane stays idle and is not counted)."""
     cus, ghz = device_info()
     pat = list(range(64)) if pattern is None else [int(v) for v in pattern]
     if len(pat) != 64:
@@ -56,7 +57,8 @@ def lds_atomic_rate(op: str = "u64", pattern=None, reps: int = 256, blocks: int 
     rc = _load().ffdas_calib_lds_atomics(OPS[op], arr, reps, blocks, threads, ctypes.byref(ms))
     if rc != 0:
         raise RuntimeError(f"ffdas_calib_lds_atomics: hip error {rc}")
-    lane_ops = blocks * threads * reps * 16
+    active = sum(1 for v in pat if v >= 0)
+    lane_ops = blocks * (threads // 64) * active * reps * 16
     rate = lane_ops / (ms.value * 1e-3) / (ghz * 1e9 * cus)
     return {"op": op, "ms": ms.value, "lane_ops_per_clk_cu": rate, "cus": cus,
             "clock_ghz": ghz, "blocks": blocks, "threads": threads}
