@@ -246,6 +246,36 @@ __device__ __forceinline__ void das_fwd_body(const Geo &g, const Rows &xr, int n
     }
 }
 
+// Sparse compounding: the slots of a voxel in ascending transmit order (an odd-even merge
+// sorting network on (index, offset, weight), 19 compare-exchanges for 8 slots).  The sum
+// over slots does not depend on their order, but the lanes of a wave do: neighbouring
+// voxels that compound the same transmits, ranked by distance in different orders, then
+// read the same (channel, transmit) row at the same slot and share its cache lines.
+template <int KC>
+__device__ __forceinline__ void sort_slots(int (&sq)[KC], float (&ofs)[KC], float (&wt)[KC]) {
+    auto cx = [&](int a, int b) {
+        const bool sw = sq[a] > sq[b];
+        const int s0 = sq[a], s1 = sq[b];
+        const float o0 = ofs[a], o1 = ofs[b], w0 = wt[a], w1 = wt[b];
+        sq[a] = sw ? s1 : s0; sq[b] = sw ? s0 : s1;
+        ofs[a] = sw ? o1 : o0; ofs[b] = sw ? o0 : o1;
+        wt[a] = sw ? w1 : w0; wt[b] = sw ? w0 : w1;
+    };
+    if constexpr (KC == 8) {
+        cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+        cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+        cx(1, 2); cx(5, 6);
+        cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+        cx(2, 4); cx(3, 5);
+        cx(1, 2); cx(3, 4); cx(5, 6);
+    } else {
+#pragma unroll
+        for (int i = 0; i < KC; i++)           // odd-even transposition sort
+#pragma unroll
+            for (int j = i & 1; j + 1 < KC; j += 2) cx(j, j + 1);
+    }
+}
+
 // Frame-inner forward with the loads of TB taps in flight together: each group of TB taps
 // issues all its 16-byte loads before the first of them is used, so a wave keeps TB times
 // more gathers in flight than tap by tap.  A tap that reads nothing loads zeros (an offset
@@ -280,9 +310,10 @@ __device__ __forceinline__ void das_fwd_body_fi(const Geo &g, const FrameInnerRo
             } else {
                 ofs[j] = 0.0f;
                 wt[j] = 0.0f;
-                sq[j] = -1;
+                sq[j] = 0x7fffffff;       // selects nothing, sorts after every real slot
             }
         }
+        if (g.sparse) sort_slots<KC>(sq, ofs, wt);
         for (int m = m0; m < g.C; m += mstep) {
             const SplitPt pp = split_point(g.srcpos[3 * m], g.srcpos[3 * m + 1], g.srcpos[3 * m + 2]);
             const Radius R = PRECISE ? radius_precise(yp, pp) : radius_fast(yp, pp);

@@ -242,3 +242,38 @@ def test_sparse_adjoint_inverted_index(dev, K, T, N):
     assert rel_l2(go.cpu().numpy(), ref["grad_offsets"]) < 1e-4
     assert rel_l2(gw.cpu().numpy(), ref["grad_weights"]) < 1e-5
     assert abs(float(gs) - ref["grad_scale"]) <= 1e-4 * max(1.0, abs(ref["grad_scale"]))
+
+
+@pytest.mark.parametrize("B", [8, 16])
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+def test_sparse_frame_rows_16bit(dev, B, storage):
+    """Sparse compounding gathers its frame-inner copy in whole frame rows (dense problems
+    in frame-group planes): 16-bit storage, chunks of 8 and 16 frames, with indices outside
+    [0, T) that select nothing, forward and every gradient against the oracle on the same
+    rounded channel data."""
+    import torch
+    import ffdas_amd
+    w = W.small_random(seed=300 + B, B=B, C=5, T=6, K=3, S=130, N=(10, 9), sparse=True,
+                       srcdir=True)
+    bad = np.random.default_rng(B).random(w.sparse_indices.shape) < 0.1
+    w.sparse_indices[bad] = -1
+    xt = getattr(torch, storage)
+    xq = torch.from_numpy(w.x).to(xt).to(torch.float64).numpy()
+    d = to_dev(w, dev, xt)
+    out = ffdas_amd.das_sparse(d["x"], d["srcpos"], d["dstpos"], d["offsets"], d["weights"],
+                               d["sparse_indices"], srcdir=d["srcdir"]).cpu().numpy()
+    ref_out = oracle.das(xq, w.srcpos, w.dstpos, w.offsets, w.weights,
+                         sparse_indices=w.sparse_indices, srcdir=w.srcdir)
+    assert rel_l2(out, ref_out) < 2e-5
+    rng = np.random.default_rng(12)
+    gout = rng.standard_normal((B,) + w.dstpos.shape[:-1]).astype(np.float32)
+    ref = oracle.das_adjoint(xq, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             sparse_indices=w.sparse_indices, srcdir=w.srcdir)
+    gx, go, gw, gs = ffdas_amd.das_backward(d["x"], d["srcpos"], d["dstpos"], d["offsets"],
+                                            d["weights"], torch.from_numpy(gout).to(dev),
+                                            sparse_indices=d["sparse_indices"],
+                                            srcdir=d["srcdir"])
+    assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
+    assert rel_l2(go.cpu().numpy(), ref["grad_offsets"]) < 1e-4
+    assert rel_l2(gw.cpu().numpy(), ref["grad_weights"]) < 2e-5
+    assert abs(float(gs) - ref["grad_scale"]) <= 1e-4 * max(1.0, abs(ref["grad_scale"]))
