@@ -258,6 +258,19 @@ ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x
 bool rank_jacobi_fits(const RankPlan &pl, int lds_bytes);
 ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, int *info,
                                  int max_sweeps);
+// Real FP32, 2 <= p <= 224: tridiagonalisation (D, E, packed reflectors U and tau; clears
+// stat[0..7]),
+// the tridiagonal's eigenvectors for the plan's r columns (Y, p x r), and their
+// back-transformation into V's columns.
+bool rank_tridiag_fits(const RankPlan &pl, int lds_bytes);
+ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_bytes,
+                                  const void *H, void *D, void *E, void *U, void *tau,
+                                  int *stat);
+ffdas_error_t launch_rank_tridiag_eigvec(hipStream_t st, const RankPlan &pl, const void *D,
+                                         const void *E, void *Y, int *stat);
+ffdas_error_t launch_rank_backtransform(hipStream_t st, const RankPlan &pl, int lds_bytes,
+                                        const void *U, const void *tau, const void *Y,
+                                        const int *stat, void *V, int *info);
 ffdas_error_t launch_rank_project(hipStream_t st, const RankPlan &pl, const void *V,
                                   const void *x, void *out, void *Qh, void *Qn);
 ffdas_error_t launch_rank_copy(hipStream_t st, bool dbl, bool cplx, const void *x, int64_t xs0,

@@ -2,8 +2,10 @@
 // src/truncate_rank/truncate_rank_impl.cuh:382-531; Python bindings/python/ffdas/
 // truncate_rank.py:28-82).  Gram matrix and projection run on the kernels of
 // rank_kernels.hip; the p x p Hermitian eigendecomposition runs on the one-workgroup Jacobi
-// kernel for p <= 64, otherwise on rocSOLVER (syevd/heevd), loaded on first use so that the
-// library does not pull rocSOLVER in at load time.
+// kernel for p <= 64; above, real FP32 Gram matrices up to p = 224 take the tridiagonal path
+// (one-workgroup Householder reduction, the chosen eigenpairs by bisection and inverse
+// iteration, their back-transformation), and the rest go to rocSOLVER's syevd/heevd, loaded on
+// first use so that the library does not pull rocSOLVER in at load time.
 #include <dlfcn.h>
 
 #include <algorithm>
@@ -74,22 +76,81 @@ ffdas_error_t check_info(ffdas_context &h, const int *info, ffdas_error_t fail) 
     return hinfo == 0 ? FFDAS_SUCCESS : fail;
 }
 
-// Eigendecomposition of the column-major p x p Hermitian H in place (eigenvectors in the
-// columns, eigenvalues ascending), queued on the stream; *fail is the error to report if the
-// solver's status word (read by the caller once the call's work is done) is not zero.
-ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E, int *info,
-                   ffdas_error_t *fail) {
-    // The one-workgroup Jacobi kernel (A and V in LDS) up to p = 64, rocSOLVER above:
-    // measured on gfx950, at p = 64 Jacobi 0.74 ms vs rocSOLVER 1.19 ms for real FP32 and
-    // 1.04 vs 1.30 ms for complex; its p^3 cost per sweep loses above (p = 96 complex
-    // 3.3 ms, p = 128 real 5.4 ms, against rocSOLVER's 1.5-1.8 ms).
-    // FFDAS_RANK_EIG=jacobi (any p whose A and V fit in LDS) or =rocsolver overrides.
+enum class EigPath { jacobi, tridiag, rocsolver };
+
+// The one-workgroup Jacobi kernel (A and V in LDS) up to p = 64: measured on gfx950, at
+// p = 64 Jacobi 0.74 ms vs rocSOLVER syevd 1.19 ms for real FP32 and 1.04 vs 1.30 ms for
+// complex; its p^3 cost per sweep loses above (p = 96 complex 3.3 ms, p = 128 real 5.4 ms,
+// against syevd's 1.5-1.8 ms).  Above, real FP32 up to p = 224 takes the tridiagonal path
+// (only the r eigenvectors the projection reads are formed); FP64 and complex take syevd/heevd.
+// FFDAS_RANK_EIG=jacobi (any p whose A and V fit in LDS), =tridiag (real FP32, p <= 224) or
+// =rocsolver overrides; a forced path that does not fit falls back to rocSOLVER's syevd/heevd.
+EigPath eig_path(const RankPlan &pl, int lds_bytes) {
     const char *force = std::getenv("FFDAS_RANK_EIG");
     const std::string want = force ? force : "";
-    const bool jacobi = want == "jacobi" ? rank_jacobi_fits(pl, h.lds_bytes)
-                                         : want != "rocsolver" && pl.p <= kJacobiMaxP &&
-                                               rank_jacobi_fits(pl, h.lds_bytes);
-    if (jacobi) {
+    const bool tri = rank_tridiag_fits(pl, lds_bytes);
+    if (want == "jacobi")
+        return rank_jacobi_fits(pl, lds_bytes) ? EigPath::jacobi : EigPath::rocsolver;
+    if (want == "tridiag") return tri ? EigPath::tridiag : EigPath::rocsolver;
+    if (want == "rocsolver") return EigPath::rocsolver;
+    if (pl.p <= kJacobiMaxP && rank_jacobi_fits(pl, lds_bytes)) return EigPath::jacobi;
+    return tri ? EigPath::tridiag : EigPath::rocsolver;
+}
+
+// Device scratch of the tridiagonal path: reflectors U (p x p, packed into the first
+// p (p - 1) / 2), the tridiagonal's eigenvectors Y (p x r), tau (p) and 8 status words.
+size_t tridiag_bytes(const RankPlan &pl) {
+    const size_t p = (size_t)pl.p;
+    return align_up(p * p * 4) + align_up(p * (size_t)pl.r * 4) + align_up(p * 4) +
+           align_up(8 * sizeof(int));
+}
+
+ffdas_error_t blas_handle(ffdas_context &h, const Solver &s, rocblas_handle *out) {
+    if (!h.blas) {
+        rocblas_handle bh = nullptr;
+        if (s.create(&bh) != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
+        h.blas = bh;
+    }
+    *out = (rocblas_handle)h.blas;
+    if (s.set_stream(*out, h.stream) != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
+    return FFDAS_SUCCESS;
+}
+
+// The tridiagonal path: T = Q^T H Q (rank_tridiag_kernel), the eigenpairs of T for the columns
+// the projection reads (rank_tridiag_eigvec_kernel), then z = Q y into those columns of H
+// (rank_backtransform_kernel, which also sets *info).  The other columns of H keep A.
+ffdas_error_t eigh_tridiag(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E,
+                           int *info, void *scratch) {
+    const size_t p = (size_t)pl.p;
+    char *w = (char *)scratch;
+    float *U = (float *)w;
+    float *Y = (float *)(w + align_up(p * p * 4));
+    float *tau = (float *)(w + align_up(p * p * 4) + align_up(p * (size_t)pl.r * 4));
+    int *stat = (int *)((char *)tau + align_up(p * 4));
+    FFDAS_TRY(launch_rank_tridiag(h.stream, pl, h.lds_bytes, H, D, E, U, tau, stat));
+    FFDAS_TRY(launch_rank_tridiag_eigvec(h.stream, pl, D, E, Y, stat));
+    FFDAS_TRY(launch_rank_backtransform(h.stream, pl, h.lds_bytes, U, tau, Y, stat, H, info));
+    if (std::getenv("FFDAS_RANK_DEBUG")) {
+        int hs = 0;
+        if (hipMemcpyAsync(&hs, stat, sizeof(int), hipMemcpyDeviceToHost, h.stream) == hipSuccess &&
+            hipStreamSynchronize(h.stream) == hipSuccess)
+            std::fprintf(stderr, "ffdas: tridiagonal p=%d r=%d status %d\n", pl.p, pl.r, hs);
+    }
+    return FFDAS_SUCCESS;
+}
+
+// Eigendecomposition of the column-major p x p Hermitian H in place (eigenvectors in the
+// columns, eigenvalues ascending; the tridiagonal path forms only the columns the projection
+// reads), queued on the stream; *fail is the error to report if the solver's status word
+// (read by the caller once the call's work is done) is not zero.
+ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, EigPath path, void *H, void *D, void *E,
+                   int *info, void *scratch, ffdas_error_t *fail) {
+    if (path == EigPath::tridiag) {
+        FFDAS_TRY(eigh_tridiag(h, pl, H, D, E, info, scratch));
+        *fail = FFDAS_ERROR_CONVERGENCE_FAILED;
+        return FFDAS_SUCCESS;
+    }
+    if (path == EigPath::jacobi) {
         int sweeps = kJacobiSweeps;
         if (const char *e = std::getenv("FFDAS_RANK_SWEEPS")) sweeps = std::max(1, std::atoi(e));
         FFDAS_TRY(launch_rank_jacobi(h.stream, pl, H, info, sweeps));
@@ -104,13 +165,8 @@ ffdas_error_t eigh(ffdas_context &h, const RankPlan &pl, void *H, void *D, void 
     }
     const Solver &s = solver();
     if (!s.ok) return FFDAS_ERROR_CUSOLVER;
-    if (!h.blas) {
-        rocblas_handle bh = nullptr;
-        if (s.create(&bh) != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
-        h.blas = bh;
-    }
-    rocblas_handle bh = (rocblas_handle)h.blas;
-    if (s.set_stream(bh, h.stream) != rocblas_status_success) return FFDAS_ERROR_CUSOLVER;
+    rocblas_handle bh = nullptr;
+    FFDAS_TRY(blas_handle(h, s, &bh));
     const rocblas_evect ev = rocblas_evect_original;
     const rocblas_fill up = rocblas_fill_upper;
     rocblas_status st;
@@ -241,8 +297,10 @@ static ffdas_error_t truncate_rank_call(ffdas_handle_t handle, ffdas_tensor_desc
     const size_t b_info = align_up(sizeof(int));
     const size_t b_Q = align_up((size_t)pl.rp * pl.pp * esz);
     const size_t b_x = need_copy ? align_up((size_t)m * n * esz) : 0;
+    const EigPath path = eig_path(pl, handle->lds_bytes);
+    const size_t b_eig = path == EigPath::tridiag ? tridiag_bytes(pl) : 0;
     void *ws = nullptr;
-    FFDAS_TRY(handle->workspace(b_part + b_H + 2 * b_D + b_info + 2 * b_Q + b_x, &ws));
+    FFDAS_TRY(handle->workspace(b_part + b_H + 2 * b_D + b_info + 2 * b_Q + b_x + b_eig, &ws));
     handle->xcopy.valid = false;   // the converted-x record of the delay-and-sum calls
     char *w = (char *)ws;
     void *part = w;
@@ -252,6 +310,7 @@ static ffdas_error_t truncate_rank_call(ffdas_handle_t handle, ffdas_tensor_desc
     int *info = (int *)(w + b_part + b_H + 2 * b_D);
     void *Qh = w + b_part + b_H + 2 * b_D + b_info;
     void *Qn = (char *)Qh + b_Q;
+    void *eig_scratch = (char *)Qn + b_Q + b_x;
     const void *xs = x;
     if (need_copy) {
         void *xc = (char *)Qn + b_Q;
@@ -267,7 +326,7 @@ static ffdas_error_t truncate_rank_call(ffdas_handle_t handle, ffdas_tensor_desc
     // the projection is queued behind the eigensolver; its status word is read once at the
     // end (a failed solve leaves out undefined and returns the solver's error)
     ffdas_error_t fail = FFDAS_SUCCESS;
-    FFDAS_TRY(eigh(*handle, pl, H, D, E, info, &fail));
+    FFDAS_TRY(eigh(*handle, pl, path, H, D, E, info, eig_scratch, &fail));
     FFDAS_TRY(launch_rank_project(st, pl, H, xs, out, Qh, Qn));
     return check_info(*handle, info, fail);
 }

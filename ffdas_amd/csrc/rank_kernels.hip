@@ -1178,6 +1178,615 @@ void project_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *o
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Real FP32 eigenvectors for 64 < p <= 224: Householder tridiagonalisation in one workgroup,
+// the tridiagonal's chosen eigenpairs by bisection and inverse iteration in one workgroup,
+// then the back-transformation of those vectors alone.
+// ---------------------------------------------------------------------------------------
+
+// DPP lane moves (GFX9 controls): quad permutations, row mirrors, and the row broadcasts of
+// lane 15 / lane 31 into the rows that row_mask enables (other rows receive 0).
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dppf(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Sum over each row of 16 lanes, in every lane of the row.
+__device__ __forceinline__ float row16_sum(float v) {
+    v += dppf<0xb1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dppf<0x4e>(v);    // quad_perm [2, 3, 0, 1]
+    v += dppf<0x141>(v);   // row_half_mirror
+    v += dppf<0x140>(v);   // row_mirror
+    return v;
+}
+
+// Sum over each half-wave (lanes 0-31 and 32-63), valid in lanes 16-31 and 48-63.
+__device__ __forceinline__ float half_wave_sum_hi(float v) {
+    v = row16_sum(v);
+    return v + dppf<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+}
+
+// Sum over the wave, the same value in every lane.
+__device__ __forceinline__ float wave_total(float v) {
+    v = half_wave_sum_hi(v);
+    v += dppf<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__host__ __device__ constexpr int packed_column(int p, int k) { return k * p - k * (k + 1) / 2; }
+
+// One reduction step's register work on the blocks B0.. (rows and columns below 32 B0 are done):
+// ps[i] = tau (A' u)_i from the lanes that hold the half-wave sums (tj = 31), written for every
+// row of the live blocks (rows <= k or >= p are never read as such).
+template <int NB, int B0>
+__device__ __forceinline__ void tridiag_matvec(const float (&a)[NB][NB], const float *x, float sc,
+                                               float t, int k, int p, int ti, int tj,
+                                               float *ps) {
+    float uj[NB];
+#pragma unroll
+    for (int jb = B0; jb < NB; ++jb) {
+        const int j = tj + 32 * jb;
+        const float xv = x[j];
+        uj[jb] = j == k + 1 ? 1.f : (j > k + 1 && j < p ? xv * sc : 0.f);
+    }
+    float acc[NB];
+#pragma unroll
+    for (int ia = B0; ia < NB; ++ia) {
+        float s = 0.f;
+#pragma unroll
+        for (int jb = B0; jb < NB; ++jb) s = fmaf(a[ia][jb], uj[jb], s);
+        acc[ia] = half_wave_sum_hi(s);
+    }
+    if (tj == 31) {
+#pragma unroll
+        for (int ia = B0; ia < NB; ++ia) ps[ti + 32 * ia] = t * acc[ia];
+    }
+}
+
+// A' -= u w^T + w u^T on the blocks B0.., w = ps - K u (u and w are 0 outside rows k+1 .. p-1).
+template <int NB, int B0>
+__device__ __forceinline__ void tridiag_update(float (&a)[NB][NB], const float *x, const float *ps,
+                                               float sc, float K, int k, int p, int ti, int tj) {
+    float uj[NB], wj[NB];
+#pragma unroll
+    for (int jb = B0; jb < NB; ++jb) {
+        const int j = tj + 32 * jb;
+        const float xj = x[j], pj = ps[j];
+        const bool lj = j > k && j < p;
+        uj[jb] = j == k + 1 ? 1.f : (lj ? xj * sc : 0.f);
+        wj[jb] = lj ? fmaf(-K, uj[jb], pj) : 0.f;
+    }
+#pragma unroll
+    for (int ia = B0; ia < NB; ++ia) {
+        const int i = ti + 32 * ia;
+        const float xi = x[i], pi = ps[i];
+        const bool li = i > k && i < p;
+        const float ui = i == k + 1 ? 1.f : (li ? xi * sc : 0.f);
+        const float wi = li ? fmaf(-K, ui, pi) : 0.f;
+#pragma unroll
+        for (int jb = B0; jb < NB; ++jb)
+            a[ia][jb] = fmaf(-wi, uj[jb], fmaf(-ui, wj[jb], a[ia][jb]));
+    }
+}
+
+// Column B of the register tile into col: all NB entries of the owning lanes, rows at or above
+// the column index included (the reduction reads the rows below it, and the diagonal for D).
+template <int NB, int B>
+__device__ __forceinline__ void tridiag_put_column(const float (&a)[NB][NB], int ti, float *col) {
+#pragma unroll
+    for (int ia = 0; ia < NB; ++ia) col[ti + 32 * ia] = a[ia][B];
+}
+
+// A uniform switch over the block index b (0 .. 7) calling F<B> for B < NB: a straight-line body
+// per live-block count, no per-element branches.
+#define FFDAS_TRIDIAG_SWITCH(b, CALL)                                                          \
+    switch (b) {                                                                               \
+        case 0: CALL(0); break;                                                                \
+        case 1: if constexpr (NB > 1) CALL(1); break;                                          \
+        case 2: if constexpr (NB > 2) CALL(2); break;                                          \
+        case 3: if constexpr (NB > 3) CALL(3); break;                                          \
+        case 4: if constexpr (NB > 4) CALL(4); break;                                          \
+        case 5: if constexpr (NB > 5) CALL(5); break;                                          \
+        case 6: if constexpr (NB > 6) CALL(6); break;                                          \
+        default: if constexpr (NB > 7) CALL(7); break;                                         \
+    }
+
+// T = Q^T A Q for the p x p symmetric A (column-major, both triangles) by p - 2 Householder
+// reflections taken from the columns (LAPACK ssytd2, lower form): reflection k,
+// H_k = I - tau_k u_k u_k^T with u_k[k + 1] = 1 and u_k[i] = 0 for i <= k, zeroes column k
+// below the subdiagonal, and Q = H_0 H_1 ... H_{p-3}.  Thread (ti, tj) = (tid / 32, tid % 32)
+// of the 1024 holds A[ti + 32 a][tj + 32 b] (a, b < NB) in registers.  A wave issues one
+// instruction per four cycles, so a step is written as straight-line code: one uniform dispatch
+// on the first live 32 x 32 block picks a fully unrolled body over the live blocks, and rows or
+// columns outside k+1 .. p-1 drop out through zero u and w rather than branches.  A step takes
+// two barriers: the owners of column k have put it in LDS (double-buffered by step parity),
+// every wave forms the same reflector from it (its own DPP reductions, so no third barrier),
+// the product A u goes through LDS, and after the rank-2 update the owners of column k + 1
+// store it.  The reflectors, D, E and tau collect in LDS (dynamic: the packed reflectors take
+// p (p - 1) / 2 floats) and go to memory once at the end, so no barrier waits on stores.
+// A is first scaled by the power of two that brings its largest magnitude into [0.5, 1)
+// (exact, and eigenvectors do not change), so squares neither overflow nor underflow for Gram
+// matrices of data far from unit magnitude.
+// Out: D (p) and E (p - 1) of the scaled T, tau (p - 2) and the reflectors packed by column
+// (u_k[k + 1 .. p - 1] from Upk[k p - k (k + 1) / 2]).  stat[0..7] (status words of the
+// kernels that follow) are cleared.
+template <int NB>
+__global__ __launch_bounds__(1024) void rank_tridiag_kernel(const float *__restrict__ A, int p,
+                                                            float *__restrict__ D,
+                                                            float *__restrict__ E,
+                                                            float *__restrict__ Upk,
+                                                            float *__restrict__ tau,
+                                                            int *__restrict__ stat) {
+    constexpr int NS = (32 * NB + 63) / 64;   // column slots per lane
+    __shared__ float xs[2][64 * NS], ps[32 * NB];
+    __shared__ float red[16];
+    __shared__ float Ds[32 * NB], Es[32 * NB], Ts[32 * NB];
+    extern __shared__ float Us[];   // p (p - 1) / 2 packed reflectors
+    const int tid = threadIdx.x, ti = tid >> 5, tj = tid & 31, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (tid < 8) stat[tid] = 0;
+    for (int i = tid; i < 64 * NS; i += 1024) xs[0][i] = xs[1][i] = 0.f;
+    for (int i = tid; i < 32 * NB; i += 1024) ps[i] = 0.f;
+    float a[NB][NB];
+    float amax = 0.f;
+#pragma unroll
+    for (int ia = 0; ia < NB; ++ia)
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) {
+            const int i = ti + 32 * ia, j = tj + 32 * jb;
+            a[ia][jb] = (i < p && j < p) ? A[(size_t)j * p + i] : 0.f;
+            amax = fmaxf(amax, fabsf(a[ia][jb]));
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
+    if (lane == 0) red[wave] = amax;
+    __syncthreads();
+    amax = red[0];
+#pragma unroll
+    for (int w = 1; w < 16; ++w) amax = fmaxf(amax, red[w]);
+    if (amax > 0.f && amax <= 3.4e38f) {
+        int e;
+        (void)frexpf(amax, &e);
+#pragma unroll
+        for (int ia = 0; ia < NB; ++ia)
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) a[ia][jb] = ldexpf(a[ia][jb], -e);
+    }
+#define FFDAS_PUT(B) tridiag_put_column<NB, (B < NB ? B : 0)>(a, ti, col)
+#define FFDAS_MATVEC(B) tridiag_matvec<NB, (B < NB ? B : 0)>(a, x, sc, t, k, p, ti, tj, ps)
+#define FFDAS_UPDATE(B) tridiag_update<NB, (B < NB ? B : 0)>(a, x, ps, sc, K, k, p, ti, tj)
+    if (tj == 0) {
+        float *col = xs[0];
+        FFDAS_PUT(0);
+    }
+    for (int k = 0; k < p - 2; ++k) {
+        __syncthreads();
+        const float *x = xs[k & 1];
+        // u = (x - beta e_1) / (alpha - beta) for x = A[k+1:, k], alpha = x[0],
+        // beta = -sign(alpha) |x|, tau = (beta - alpha) / beta; no reflection (tau = 0) when x
+        // is already a multiple of e_1.  Every wave reduces the same values in the same order.
+        const float alpha = x[k + 1];
+        float ss = 0.f;
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+            const int i = lane + 64 * sl;
+            const float v = x[i];
+            ss = fmaf(v, i > k + 1 && i < p ? v : 0.f, ss);
+        }
+        ss = wave_total(ss);
+        float t = 0.f, beta = alpha, sc = 0.f;
+        if (ss != 0.f) {
+            const float nrm = sqrtf(fmaf(alpha, alpha, ss));
+            const float b = alpha >= 0.f ? -nrm : nrm;
+            t = (b - alpha) / b;
+            sc = 1.f / (alpha - b);
+            beta = b;
+        }
+        if (wave == 0) {
+            const int o = packed_column(p, k) - (k + 1);
+            for (int i = k + 1 + lane; i < p; i += 64) Us[o + i] = i == k + 1 ? 1.f : x[i] * sc;
+            if (lane == 0) {
+                Ds[k] = x[k];
+                Es[k] = beta;
+                Ts[k] = t;
+            }
+        }
+        if (t != 0.f) {   // the same value in every wave
+            const int b0 = (k + 1) >> 5;   // first block holding a row or column > k
+            FFDAS_TRIDIAG_SWITCH(b0, FFDAS_MATVEC)
+            __syncthreads();
+            // K = (tau / 2)(ps . u)
+            float dot = 0.f;
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                const int i = lane + 64 * sl;
+                const float pv = ps[i < 32 * NB ? i : 0], xv = x[i];
+                const float u = i == k + 1 ? 1.f : xv * sc;
+                dot = fmaf(i > k && i < p ? pv : 0.f, u, dot);
+            }
+            const float K = 0.5f * t * wave_total(dot);
+            FFDAS_TRIDIAG_SWITCH(b0, FFDAS_UPDATE)
+        }
+        if (tj == ((k + 1) & 31)) {
+            float *col = xs[(k + 1) & 1];
+            FFDAS_TRIDIAG_SWITCH((k + 1) >> 5, FFDAS_PUT)
+        }
+    }
+#undef FFDAS_PUT
+#undef FFDAS_MATVEC
+#undef FFDAS_UPDATE
+    // the trailing 2 x 2 block
+#pragma unroll
+    for (int ia = 0; ia < NB; ++ia)
+#pragma unroll
+        for (int jb = 0; jb < NB; ++jb) {
+            const int i = ti + 32 * ia, j = tj + 32 * jb;
+            if (i >= p || j >= p || i < p - 2) continue;
+            if (i == j) Ds[i] = a[ia][jb];
+            if (i == p - 1 && j == p - 2) Es[p - 2] = a[ia][jb];
+        }
+    __syncthreads();
+    const int npk = p * (p - 1) / 2;
+    for (int e = tid; e < npk; e += 1024) Upk[e] = Us[e];
+    for (int i = tid; i < p; i += 1024) {
+        D[i] = Ds[i];
+        if (i < p - 1) E[i] = Es[i];
+        if (i < p - 2) tau[i] = Ts[i];
+    }
+}
+
+// Number of eigenvalues of the symmetric tridiagonal T below x, by the signs of the pivots of
+// T - x I (a pivot of magnitude <= pivmin is taken as -pivmin, LAPACK's guard); de[i] holds
+// (d[i], e[i-1]^2) with e[-1] = 0, and (1e30, 0) past p (pivots that add nothing to the count),
+// read eight entries ahead of the recurrence.
+__device__ __forceinline__ int sturm_count(const float2 *de, int p, float x, float pivmin) {
+    float q = 1.f;
+    int c = 0;
+    for (int i0 = 0; i0 < p; i0 += 8) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = de[i0 + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            q = (v[u].x - x) - __fdividef(v[u].y, q);
+            q = fabsf(q) <= pivmin ? -pivmin : q;
+            c += q < 0.f;
+        }
+    }
+    return c;
+}
+
+// A fixed pseudo-random start vector entry in (-1, 1) for row i of eigenvector index j.
+__device__ __forceinline__ float start_entry(int j, int i) {
+    uint32_t h = (uint32_t)j * 0x9E3779B1u ^ ((uint32_t)i + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (float)(int32_t)h * (1.f / 2147483648.f);
+}
+
+constexpr int kEigWaves = 8;
+
+// Eigenpairs of the symmetric tridiagonal T (diagonal D, off-diagonal E, p <= PMAX) for the r
+// eigenvalue indices the projection reads: ascending index c_first + q for q < q_split,
+// c_second + (q - q_split) after.  Eigenvalues by multisection of Sturm counts (each lane of
+// a wave counts at one of 64 points of the bracket, narrowing it 65-fold a round, to within
+// 4 eps |T|); eigenvectors by inverse iteration as LAPACK's sstein does it: an LU of
+// T - lambda I with partial pivoting (pivots below eps |T| raised to it), three solves from a
+// fixed pseudo-random start, and after each solve Gram-Schmidt against the earlier vectors of
+// the same cluster (consecutive eigenvalues closer than 1e-3 |T|), whose eigenvalues are
+// first moved apart by at least 10 eps |lambda|.  One workgroup of 8 waves: wave w brackets
+// eigenvalues w, w + 8, ..., then runs clusters w, w + 8, ... (the LU and the solves on its
+// lane 0, eight entries of LDS read ahead of the recurrences; products and norms across the
+// wave).  Y (p x r): column q is the unit eigenvector of index idx(q); T = 0 gives unit
+// vectors.  A non-finite vector sets stat[0].
+template <int PMAX>
+__global__ __launch_bounds__(64 * kEigWaves) void rank_tridiag_eigvec_kernel(
+    const float *__restrict__ D, const float *__restrict__ E, int p, int r, int q_split,
+    int c_first, int c_second, float *__restrict__ Y, int *__restrict__ stat) {
+    constexpr int PP = PMAX + 8;   // room for the read-ahead past p
+    __shared__ float2 de[PP];      // (d[i], e[i-1]^2)
+    __shared__ float sd[PP], se[PP], lam[PMAX];
+    __shared__ int cstart[PMAX + 1];
+    __shared__ float4 lu[kEigWaves][PP];   // (1 / U diagonal, U's two superdiagonals, L)
+    __shared__ unsigned char piv[kEigWaves][PP];   // rows k and k + 1 interchanged
+    __shared__ float xv[kEigWaves][PP];
+    __shared__ float red[kEigWaves][4];
+    __shared__ int ncl;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr float eps = 1.1920929e-7f;
+    for (int i = tid; i < PP; i += 64 * kEigWaves) {
+        const float d = i < p ? D[i] : 0.f;
+        const float e = i < p - 1 ? E[i] : 0.f;
+        const float em = i > 0 && i < p ? E[i - 1] : 0.f;
+        sd[i] = d;
+        se[i] = e;
+        de[i] = i < p ? make_float2(d, em * em) : make_float2(1e30f, 0.f);
+    }
+    __syncthreads();
+    // |T| (row sums), Gershgorin bounds and the pivot guard
+    float nrm = 0.f, lo = 3.4e38f, hi = -3.4e38f, e2max = 0.f;
+    for (int i = tid; i < p; i += 64 * kEigWaves) {
+        const float off = fabsf(se[i]) + (i > 0 ? fabsf(se[i - 1]) : 0.f);
+        nrm = fmaxf(nrm, fabsf(sd[i]) + off);
+        lo = fminf(lo, sd[i] - off);
+        hi = fmaxf(hi, sd[i] + off);
+        e2max = fmaxf(e2max, se[i] * se[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        nrm = fmaxf(nrm, __shfl_xor(nrm, o));
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+        e2max = fmaxf(e2max, __shfl_xor(e2max, o));
+    }
+    if (lane == 0) {
+        red[wave][0] = nrm;
+        red[wave][1] = lo;
+        red[wave][2] = hi;
+        red[wave][3] = e2max;
+    }
+    __syncthreads();
+    nrm = red[0][0];
+    lo = red[0][1];
+    hi = red[0][2];
+    e2max = red[0][3];
+#pragma unroll
+    for (int w = 1; w < kEigWaves; ++w) {
+        nrm = fmaxf(nrm, red[w][0]);
+        lo = fminf(lo, red[w][1]);
+        hi = fmaxf(hi, red[w][2]);
+        e2max = fmaxf(e2max, red[w][3]);
+    }
+    const auto index_of = [&](int q) {
+        return q < q_split ? c_first + q : c_second + (q - q_split);
+    };
+    if (nrm == 0.f) {   // T = 0: every vector is an eigenvector
+        for (int e = tid; e < p * r; e += 64 * kEigWaves) {
+            const int q = e / p, i = e - q * p;
+            Y[e] = i == index_of(q) ? 1.f : 0.f;
+        }
+        return;
+    }
+    const float pivmin = 1.17549435e-38f * fmaxf(1.f, e2max);
+    const float widen = 4.f * eps * (float)p * nrm + 2.f * pivmin;
+    lo -= widen;
+    hi += widen;
+    const float tol = 4.f * eps * nrm;
+    // eigenvalue idx: count(a) <= idx < count(b) throughout
+    for (int q = wave; q < r; q += kEigWaves) {
+        const int idx = index_of(q);
+        float a = lo, b = hi;
+        for (int round = 0; round < 8 && b - a > tol; ++round) {
+            const float x = a + (b - a) * (float)(lane + 1) * (1.f / 65.f);
+            const int c = sturm_count(de, p, x, pivmin);
+            const uint64_t above = __ballot(c > idx);
+            const int f = above ? __builtin_ctzll(above) : 64;
+            const float xf = __shfl(x, f < 64 ? f : 63);
+            const float xp = __shfl(x, f > 0 ? f - 1 : 0);
+            const float na = f > 0 ? xp : a, nb = f < 64 ? xf : b;
+            a = na;
+            b = nb;
+        }
+        if (lane == 0) lam[q] = 0.5f * (a + b);
+    }
+    __syncthreads();
+    // clusters of consecutive requested eigenvalues (ascending in q)
+    const float ortol = 1e-3f * nrm;
+    if (tid == 0) {
+        int n = 0;
+        for (int q = 0; q < r; ++q)
+            if (q == 0 || lam[q] - lam[q - 1] > ortol) cstart[n++] = q;
+        cstart[n] = r;
+        ncl = n;
+    }
+    __syncthreads();
+    const float tolpiv = eps * nrm;
+    float4 *f = lu[wave];
+    unsigned char *pv = piv[wave];
+    float *x = xv[wave];
+    bool bad = false;
+    for (int cl = wave; cl < ncl; cl += kEigWaves) {
+        const int q0 = cstart[cl], q1 = cstart[cl + 1];
+        float prev = 0.f;
+        for (int q = q0; q < q1; ++q) {
+            float lm = lam[q];
+            if (q > q0) lm = fmaxf(lm, prev + 10.f * eps * fabsf(prev));
+            prev = lm;
+            const int kb = (p + 7) & ~7;   // the loops below run whole blocks of 8
+            if (lane == 0) {
+                // LU of T - lm I with partial pivoting (LAPACK slagtf), branch-free; entries
+                // from p - 1 on are (0, 0, 0, 0) without interchange (inert in the solves),
+                // then the last pivot goes to p - 1
+                const auto guard = [&](float v) {
+                    return fabsf(v) < tolpiv ? (v < 0.f ? -tolpiv : tolpiv) : v;
+                };
+                float a = sd[0] - lm, b = se[0], afin = a;
+                for (int k0 = 0; k0 < kb; k0 += 8) {
+                    float dn[8], en[9];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) dn[u] = sd[k0 + u + 1];
+#pragma unroll
+                    for (int u = 0; u < 9; ++u) en[u] = se[k0 + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int k = k0 + u;
+                        const float c = en[u];
+                        const float an = dn[u] - lm;
+                        const float bn = en[u + 1];   // se is 0 from p - 1 on
+                        const bool swap = fabsf(a) < fabsf(c);
+                        const float m1 = a != 0.f ? c * __builtin_amdgcn_rcpf(a) : 0.f;
+                        const float m2 = a * __builtin_amdgcn_rcpf(c);
+                        const bool live = k < p - 1;
+                        const float m = swap ? m2 : m1;
+                        float4 o = swap ? make_float4(c, an, bn, m) : make_float4(a, b, 0.f, m);
+                        o.x = __builtin_amdgcn_rcpf(guard(o.x));
+                        f[k] = live ? o : make_float4(0.f, 0.f, 0.f, 0.f);
+                        pv[k] = live && swap;
+                        const float na = swap ? b - m * an : an - m * b;
+                        b = swap ? -m * bn : bn;
+                        a = na;
+                        afin = k == p - 2 ? a : afin;
+                    }
+                }
+                f[p - 1] = make_float4(__builtin_amdgcn_rcpf(guard(afin)), 0.f, 0.f, 0.f);
+            }
+            for (int i = lane; i < PP; i += 64)
+                x[i] = i < p ? start_entry(index_of(q), i) : 0.f;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            for (int it = 0; it < 3; ++it) {
+                if (lane == 0) {
+                    // forward: the row operations of the factorisation (x is 0 from p on)
+                    float cur = x[0];
+                    for (int k0 = 0; k0 < kb; k0 += 8) {
+                        float xn[8], mn[8];
+                        bool sw[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            xn[u] = x[k0 + u + 1];
+                            mn[u] = f[k0 + u].w;
+                            sw[u] = pv[k0 + u] != 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const float a0 = sw[u] ? xn[u] : cur;
+                            const float a1 = sw[u] ? cur : xn[u];
+                            x[k0 + u] = a0;
+                            cur = a1 - mn[u] * a0;
+                        }
+                    }
+                    // back substitution from the padded top (the first component of f holds
+                    // the reciprocal pivot; f and x are 0 past p - 1)
+                    float x1 = 0.f, x2 = 0.f;
+                    for (int k1 = kb - 1; k1 >= 0; k1 -= 8) {
+                        float xk[8];
+                        float4 fk[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            xk[u] = x[k1 - u];
+                            fk[u] = f[k1 - u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const float v = (xk[u] - fk[u].y * x1 - fk[u].z * x2) * fk[u].x;
+                            x[k1 - u] = v;
+                            x2 = x1;
+                            x1 = v;
+                        }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                // scale by the largest magnitude first (no overflow in the products below)
+                float mx = 0.f;
+                for (int i = lane; i < p; i += 64) mx = fmaxf(mx, fabsf(x[i]));
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+                const float inv = mx > 0.f && mx <= 3.4e38f ? 1.f / mx : 0.f;
+                float xl[PMAX / 64];
+#pragma unroll
+                for (int e = 0; e < PMAX / 64; ++e) {
+                    const int i = lane + 64 * e;
+                    xl[e] = i < p ? x[i] * inv : 0.f;
+                }
+                // Gram-Schmidt against the earlier vectors of the cluster (unit, in Y)
+                for (int qq = q0; qq < q; ++qq) {
+                    const float *yq = Y + (size_t)qq * p;
+                    float yv[PMAX / 64];
+                    float s = 0.f;
+#pragma unroll
+                    for (int e = 0; e < PMAX / 64; ++e) {
+                        const int i = lane + 64 * e;
+                        yv[e] = i < p ? yq[i] : 0.f;
+                        s = fmaf(xl[e], yv[e], s);
+                    }
+                    s = wave_total(s);
+#pragma unroll
+                    for (int e = 0; e < PMAX / 64; ++e) xl[e] = fmaf(-s, yv[e], xl[e]);
+                }
+                float ss = 0.f;
+#pragma unroll
+                for (int e = 0; e < PMAX / 64; ++e) ss = fmaf(xl[e], xl[e], ss);
+                ss = wave_total(ss);
+                const float sc = ss > 0.f ? 1.f / sqrtf(ss) : 0.f;
+                if (!(ss > 0.f) || !(ss <= 3.4e38f)) bad = true;
+#pragma unroll
+                for (int e = 0; e < PMAX / 64; ++e) {
+                    const int i = lane + 64 * e;
+                    if (i < p) x[i] = xl[e] * sc;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+            for (int i = lane; i < p; i += 64) Y[(size_t)q * p + i] = x[i];
+        }
+    }
+    if (bad && lane == 0) atomicOr(stat, 1);
+}
+
+// Eigenvectors of A from those of its tridiagonal form: z = H_0 H_1 ... H_{p-3} y, the
+// reflections applied last to first, one wave per vector with NE elements per lane
+// (p <= 64 NE).  The workgroup first copies the packed reflectors into LDS (dynamic,
+// p (p - 1) / 2 floats), then each wave reads the next reflector while it applies the current
+// one.  Vector q (column q of Y) goes to column c_first + q, or c_second + (q - q_split) for
+// q >= q_split, of V (column-major p x p), where the projection reads it.  *info = stat[0]
+// (set by the eigenpair kernel when a vector is not finite).
+template <int NE>
+__global__ __launch_bounds__(256) void rank_backtransform_kernel(
+    const float *__restrict__ Upk, const float *__restrict__ tau, const float *__restrict__ Y,
+    int p, int r, int q_split, int c_first, int c_second, const int *__restrict__ stat,
+    float *__restrict__ V, int *__restrict__ info) {
+    extern __shared__ float Us[];
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *info = stat[0] != 0 ? 1 : 0;
+    const int npk = p * (p - 1) / 2;
+    for (int e = threadIdx.x; e < npk; e += 256) Us[e] = Upk[e];
+    __syncthreads();
+    if (q >= r) return;
+    const float *y = Y + (size_t)q * p;
+    float v[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const int i = lane + 64 * e;
+        v[e] = i < p ? y[i] : 0.f;
+    }
+    // u_k[i] for i > k (u_k[i] = 0 below), loaded one reflector ahead
+    const auto load = [&](int k, float (&u)[NE]) {
+        const int o = packed_column(p, k) - (k + 1);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int i = lane + 64 * e;
+            u[e] = (k >= 0 && i > k && i < p) ? Us[o + i] : 0.f;
+        }
+    };
+    float un[NE];
+    float tn = p >= 3 ? tau[p - 3] : 0.f;
+    load(p - 3, un);
+    for (int k = p - 3; k >= 0; --k) {
+        float u[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) u[e] = un[e];
+        const float t = tn;
+        tn = k >= 1 ? tau[k - 1] : 0.f;
+        load(k - 1, un);
+        float s = 0.f;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) s = fmaf(u[e], v[e], s);
+        const float ts = t * wave_total(s);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) v[e] = fmaf(-ts, u[e], v[e]);
+    }
+    const int col = q < q_split ? c_first + q : c_second + (q - q_split);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const int i = lane + 64 * e;
+        if (i < p) V[(size_t)col * p + i] = v[e];
+    }
+}
+
 }  // namespace
 
 void rank_plan_launch_shape(RankPlan &pl, int cu_count) {
@@ -1345,6 +1954,91 @@ ffdas_error_t launch_rank_copy(hipStream_t st, bool dbl, bool cplx, const void *
     else
         FFDAS_RANK_COPY(float, false);
 #undef FFDAS_RANK_COPY
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+
+size_t rank_tridiag_lds(int p) { return (size_t)p * (p - 1) / 2 * sizeof(float); }
+
+// p <= 224: the 1024 threads hold A in 7 x 7 registers each (8 x 8 for p = 256 spills to
+// scratch at the 128 registers a lane has in a 1024-thread workgroup); the tridiagonalisation's
+// static LDS stays under 8 KiB.
+constexpr int kTridiagMaxP = 224;
+
+bool rank_tridiag_fits(const RankPlan &pl, int lds_bytes) {
+    return !pl.dbl && !pl.cplx && pl.p >= 2 && pl.p <= kTridiagMaxP &&
+           rank_tridiag_lds(pl.p) + 8192 <= (size_t)lds_bytes;
+}
+
+ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_bytes,
+                                  const void *H, void *D, void *E, void *U, void *tau,
+                                  int *stat) {
+    if (!rank_tridiag_fits(pl, lds_bytes)) return FFDAS_ERROR_INVALID_ARGUMENT;
+    ProfScope ps(st, "rank_tridiag");
+    const size_t lds = rank_tridiag_lds(pl.p);
+#define FFDAS_RANK_TRIDIAG(NB)                                                                 \
+    do {                                                                                       \
+        (void)hipFuncSetAttribute((const void *)rank_tridiag_kernel<NB>,                       \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
+        hipLaunchKernelGGL((rank_tridiag_kernel<NB>), dim3(1), dim3(1024), lds, st,            \
+                           (const float *)H, pl.p, (float *)D, (float *)E, (float *)U,         \
+                           (float *)tau, stat);                                                \
+    } while (0)
+    switch ((pl.p + 31) / 32) {
+        case 1: FFDAS_RANK_TRIDIAG(1); break;
+        case 2: FFDAS_RANK_TRIDIAG(2); break;
+        case 3: FFDAS_RANK_TRIDIAG(3); break;
+        case 4: FFDAS_RANK_TRIDIAG(4); break;
+        case 5: FFDAS_RANK_TRIDIAG(5); break;
+        case 6: FFDAS_RANK_TRIDIAG(6); break;
+        default: FFDAS_RANK_TRIDIAG(7); break;
+    }
+#undef FFDAS_RANK_TRIDIAG
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+ffdas_error_t launch_rank_tridiag_eigvec(hipStream_t st, const RankPlan &pl, const void *D,
+                                         const void *E, void *Y, int *stat) {
+    if (pl.dbl || pl.cplx || pl.p < 2 || pl.p > kTridiagMaxP || pl.r < 1 || pl.r > pl.p)
+        return FFDAS_ERROR_INVALID_ARGUMENT;
+    ProfScope ps(st, "rank_tridiag_eigvec");
+#define FFDAS_RANK_EIGVEC(PMAX)                                                                \
+    hipLaunchKernelGGL((rank_tridiag_eigvec_kernel<PMAX>), dim3(1), dim3(64 * kEigWaves), 0,   \
+                       st, (const float *)D, (const float *)E, pl.p, pl.r, pl.q_split,         \
+                       pl.c_first, pl.c_second, (float *)Y, stat)
+    switch ((pl.p + 63) / 64) {
+        case 1: FFDAS_RANK_EIGVEC(64); break;
+        case 2: FFDAS_RANK_EIGVEC(128); break;
+        case 3: FFDAS_RANK_EIGVEC(192); break;
+        default: FFDAS_RANK_EIGVEC(256); break;
+    }
+#undef FFDAS_RANK_EIGVEC
+    return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
+}
+
+ffdas_error_t launch_rank_backtransform(hipStream_t st, const RankPlan &pl, int lds_bytes,
+                                        const void *U, const void *tau, const void *Y,
+                                        const int *stat, void *V, int *info) {
+    if (!rank_tridiag_fits(pl, lds_bytes) || pl.r < 1 || pl.r > pl.p)
+        return FFDAS_ERROR_INVALID_ARGUMENT;
+    ProfScope ps(st, "rank_backtransform");
+    const unsigned blocks = (unsigned)((pl.r + 3) / 4);
+    const size_t lds = rank_tridiag_lds(pl.p);
+#define FFDAS_RANK_BACK(NE)                                                                    \
+    do {                                                                                       \
+        (void)hipFuncSetAttribute((const void *)rank_backtransform_kernel<NE>,                 \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
+        hipLaunchKernelGGL((rank_backtransform_kernel<NE>), dim3(blocks), dim3(256), lds, st,  \
+                           (const float *)U, (const float *)tau, (const float *)Y, pl.p, pl.r, \
+                           pl.q_split, pl.c_first, pl.c_second, stat, (float *)V, info);       \
+    } while (0)
+    switch ((pl.p + 63) / 64) {
+        case 1: FFDAS_RANK_BACK(1); break;
+        case 2: FFDAS_RANK_BACK(2); break;
+        case 3: FFDAS_RANK_BACK(3); break;
+        default: FFDAS_RANK_BACK(4); break;
+    }
+#undef FFDAS_RANK_BACK
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
 }
 

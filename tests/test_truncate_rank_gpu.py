@@ -342,3 +342,102 @@ def test_repeated_singular_values(dev, monkeypatch, solver):
     keep = ffdas_amd.truncate_rank(x, 3, 13)      # the tied middle block alone
     want_keep = ((u[:, 3:13] * s[3:13]) @ v[:, 3:13].conj().T).to(dev)
     assert float((keep - want_keep).norm() / want_keep.norm()) < 1e-9
+
+
+def _profiled_count(name, fn):
+    import ctypes
+    import torch
+    from ffdas_amd._lib import lib
+    L = lib()
+    L.ffdas_profile_reset()
+    L.ffdas_profile_enable(1)
+    try:
+        out = fn()
+        torch.cuda.synchronize()
+    finally:
+        L.ffdas_profile_enable(0)
+    t, c = ctypes.c_double(), ctypes.c_int64()
+    L.ffdas_profile_query(name.encode(), ctypes.byref(t), ctypes.byref(c))
+    return out, c.value
+
+
+@pytest.mark.parametrize("shape, start, stop", [
+    ((65, 3000), 3, None),       # the smallest p past the Jacobi kernel, top 3 subtracted
+    ((200, 5000), 3, None),      # the clutter filter's usual form
+    ((200, 5000), 10, 150),      # keep mode: one range of 140 columns (two projection passes)
+    ((200, 5000), 10, 180),      # subtract mode with two ranges: 20 low and 10 high columns
+    ((224, 4000), 5, 218),       # the largest p of the path, two ranges
+    ((3000, 97), 2, 90),         # m > n: Gram of the columns
+])
+def test_tridiagonal_path(dev, monkeypatch, shape, start, stop):
+    """Real FP32 with 64 < p <= 224 takes the tridiagonal path by default (one-workgroup
+    Householder reduction, the chosen eigenpairs by bisection and inverse iteration, the
+    back-transformation of those vectors alone) and gives the oracle's result."""
+    import torch
+    import ffdas_amd
+    monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
+    x = _rand(shape, torch.float32, 31).to(dev)
+    got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(x, start, stop))
+    assert calls == 1
+    want = R.svd_filter(x.cpu().numpy(), start, stop)
+    assert rel_l2(got.cpu().numpy(), want) < TOL["float32"]
+
+
+def test_tridiagonal_path_limits(dev, monkeypatch):
+    """p = 225 (past the register tile) and FP64 keep rocSOLVER's syevd; forcing the path
+    for a small p (2 and 33) gives the oracle's result too."""
+    import torch
+    import ffdas_amd
+    monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
+    x = _rand((225, 1500), torch.float32, 32).to(dev)
+    got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(x, 4))
+    assert calls == 0
+    assert rel_l2(got.cpu().numpy(), R.svd_filter(x.cpu().numpy(), 4, None)) < TOL["float32"]
+    xd = _rand((100, 1500), torch.float64, 33).to(dev)
+    got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(xd, 4))
+    assert calls == 0
+    assert rel_l2(got.cpu().numpy(), R.svd_filter(xd.cpu().numpy(), 4, None)) < TOL["float64"]
+    monkeypatch.setenv("FFDAS_RANK_EIG", "tridiag")
+    for shape, start, stop in [((2, 50), 1, None), ((33, 500), 2, 20)]:
+        xs = _rand(shape, torch.float32, 34).to(dev)
+        got, calls = _profiled_count("rank_tridiag",
+                                     lambda: ffdas_amd.truncate_rank(xs, start, stop))
+        assert calls == 1
+        assert rel_l2(got.cpu().numpy(), R.svd_filter(xs.cpu().numpy(), start, stop)) < 1e-4
+
+
+def test_tridiagonal_path_ties_scaling_zero(dev, monkeypatch):
+    """The tridiagonal path with tied singular values inside the removed and kept sets (a
+    gap only at the boundary; the eigenvalues of the top cluster coincide), magnitudes far
+    from 1 (scaling by 2^k is exact: the kernel prescales the Gram matrix by a power of two,
+    so the result scales bit for bit), and all-zero data (T = 0 splits into 1 x 1 blocks)."""
+    import torch
+    import ffdas_amd
+    monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
+    m, n = 100, 4000
+    g = torch.Generator().manual_seed(35)
+    u, _ = torch.linalg.qr(torch.randn(m, m, dtype=torch.float64, generator=g))
+    v, _ = torch.linalg.qr(torch.randn(n, m, dtype=torch.float64, generator=g))
+    s = torch.tensor([50.0] * 3 + [5.0] * 10 + [1.0] * (m - 13), dtype=torch.float64)
+    x = (u * s) @ v.T
+    want = (u[:, 3:] * s[3:]) @ v[:, 3:].T
+    got, calls = _profiled_count("rank_tridiag",
+                                 lambda: ffdas_amd.truncate_rank(x.to(torch.float32).to(dev), 3))
+    assert calls == 1
+    assert float((got.cpu().double() - want).norm() / want.norm()) < 1e-4
+    keep = ffdas_amd.truncate_rank(x.to(torch.float32).to(dev), 3, 13)   # the tied block
+    want_keep = (u[:, 3:13] * s[3:13]) @ v[:, 3:13].T
+    assert float((keep.cpu().double() - want_keep).norm() / want_keep.norm()) < 1e-4
+
+    xr = _rand((120, 3000), torch.float32, 36).to(dev)
+    base = ffdas_amd.truncate_rank(xr, 3)
+    for k in (30, -30):
+        assert torch.equal(ffdas_amd.truncate_rank(xr * 2.0 ** k, 3), base * 2.0 ** k), k
+    for scale in (1e12, 1e-12):
+        xe = xr * scale
+        got = ffdas_amd.truncate_rank(xe, 2, 90)
+        assert rel_l2(got.cpu().numpy(), R.svd_filter(xe.cpu().numpy(), 2, 90)) < 1e-4
+
+    z = torch.zeros((90, 500), dtype=torch.float32, device=dev)
+    for start, stop in [(3, None), (0, 4), (1, 3), (5, 80)]:
+        assert torch.equal(ffdas_amd.truncate_rank(z, start, stop), torch.zeros_like(z))
