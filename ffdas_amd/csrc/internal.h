@@ -258,10 +258,9 @@ ffdas_error_t launch_rank_gram(hipStream_t st, const RankPlan &pl, const void *x
 bool rank_jacobi_fits(const RankPlan &pl, int lds_bytes);
 ffdas_error_t launch_rank_jacobi(hipStream_t st, const RankPlan &pl, void *H, int *info,
                                  int max_sweeps);
-// Real FP32, 2 <= p <= 224: tridiagonalisation (D, E, packed reflectors U and tau; clears
-// stat[0..7]),
-// the tridiagonal's eigenvectors for the plan's r columns (Y, p x r), and their
-// back-transformation into V's columns.
+// FP32, 2 <= p <= 224 real / 160 complex: tridiagonalisation (D, E of the real tridiagonal,
+// packed reflectors U and tau; clears stat[0..7]), the tridiagonal's eigenvectors for the
+// plan's r columns (Y, real p x r), and their back-transformation into V's columns.
 bool rank_tridiag_fits(const RankPlan &pl, int lds_bytes);
 ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_bytes,
                                   const void *H, void *D, void *E, void *U, void *tau,

@@ -2,10 +2,10 @@
 // src/truncate_rank/truncate_rank_impl.cuh:382-531; Python bindings/python/ffdas/
 // truncate_rank.py:28-82).  Gram matrix and projection run on the kernels of
 // rank_kernels.hip; the p x p Hermitian eigendecomposition runs on the one-workgroup Jacobi
-// kernel for p <= 64; above, real FP32 Gram matrices up to p = 224 take the tridiagonal path
-// (one-workgroup Householder reduction, the chosen eigenpairs by bisection and inverse
-// iteration, their back-transformation), and the rest go to rocSOLVER's syevd/heevd, loaded on
-// first use so that the library does not pull rocSOLVER in at load time.
+// kernel for p <= 64; above, FP32 Gram matrices up to p = 224 (real) or 160 (complex) take the
+// tridiagonal path (one-workgroup Householder reduction, the chosen eigenpairs by bisection
+// and inverse iteration, their back-transformation), and the rest go to rocSOLVER's
+// syevd/heevd, loaded on first use so that the library does not pull rocSOLVER in at load time.
 #include <dlfcn.h>
 
 #include <algorithm>
@@ -65,6 +65,7 @@ size_t align_up(size_t b) { return (b + 255) & ~size_t(255); }
 
 constexpr int kJacobiSweeps = 30;
 constexpr int kJacobiMaxP = 64;
+constexpr int kJacobiOverTridiagP = 16;   // FP32 p at or below which Jacobi stays ahead
 
 // The solver's status word, read on the host: the one synchronisation of the call, as the
 // reference's (truncate_rank_impl.cuh:82-87).
@@ -78,12 +79,15 @@ ffdas_error_t check_info(ffdas_context &h, const int *info, ffdas_error_t fail) 
 
 enum class EigPath { jacobi, tridiag, rocsolver };
 
-// The one-workgroup Jacobi kernel (A and V in LDS) up to p = 64: measured on gfx950, at
-// p = 64 Jacobi 0.74 ms vs rocSOLVER syevd 1.19 ms for real FP32 and 1.04 vs 1.30 ms for
-// complex; its p^3 cost per sweep loses above (p = 96 complex 3.3 ms, p = 128 real 5.4 ms,
-// against syevd's 1.5-1.8 ms).  Above, real FP32 up to p = 224 takes the tridiagonal path
-// (only the r eigenvectors the projection reads are formed); FP64 and complex take syevd/heevd.
-// FFDAS_RANK_EIG=jacobi (any p whose A and V fit in LDS), =tridiag (real FP32, p <= 224) or
+// FP32 takes the tridiagonal path wherever its tile fits (real p <= 224, complex p <= 160)
+// from p = 17 on: measured on gfx950 (whole calls), 64 x 64^3 complex 0.44 ms against the
+// Jacobi kernel's 0.70 ms and heevd's 1.45 ms, real 0.29 / 0.52 / 1.27 ms; p = 24 real 0.131
+// against Jacobi's 0.165 ms; p = 120 complex 1.15 ms against heevd's 2.62 ms; 200 x 96^3 real
+// 1.79 ms against syevd's 4.43 ms; at p = 8 Jacobi is ahead (0.107 against 0.120 ms).  FP64
+// (and FP32 up to p = 16) takes the one-workgroup Jacobi kernel (A and V in LDS) up to p = 64
+// (at p = 64 it beat syevd/heevd: 0.74 / 1.19 ms real, 1.04 / 1.30 ms complex in FP32; its p^3
+// cost per sweep loses above), and rocSOLVER's syevd/heevd beyond.
+// FFDAS_RANK_EIG=jacobi (any p whose A and V fit in LDS), =tridiag (FP32 within those p) or
 // =rocsolver overrides; a forced path that does not fit falls back to rocSOLVER's syevd/heevd.
 EigPath eig_path(const RankPlan &pl, int lds_bytes) {
     const char *force = std::getenv("FFDAS_RANK_EIG");
@@ -93,15 +97,18 @@ EigPath eig_path(const RankPlan &pl, int lds_bytes) {
         return rank_jacobi_fits(pl, lds_bytes) ? EigPath::jacobi : EigPath::rocsolver;
     if (want == "tridiag") return tri ? EigPath::tridiag : EigPath::rocsolver;
     if (want == "rocsolver") return EigPath::rocsolver;
-    if (pl.p <= kJacobiMaxP && rank_jacobi_fits(pl, lds_bytes)) return EigPath::jacobi;
-    return tri ? EigPath::tridiag : EigPath::rocsolver;
+    const bool jac = pl.p <= kJacobiMaxP && rank_jacobi_fits(pl, lds_bytes);
+    if (tri && !(jac && pl.p <= kJacobiOverTridiagP)) return EigPath::tridiag;
+    if (jac) return EigPath::jacobi;
+    return EigPath::rocsolver;
 }
 
-// Device scratch of the tridiagonal path: reflectors U (p x p, packed into the first
-// p (p - 1) / 2), the tridiagonal's eigenvectors Y (p x r), tau (p) and 8 status words.
+// Device scratch of the tridiagonal path: reflectors U (p x p elements, packed into the first
+// p (p - 1) / 2), the tridiagonal's eigenvectors Y (real p x r), tau (p elements) and 8 status
+// words.
 size_t tridiag_bytes(const RankPlan &pl) {
-    const size_t p = (size_t)pl.p;
-    return align_up(p * p * 4) + align_up(p * (size_t)pl.r * 4) + align_up(p * 4) +
+    const size_t p = (size_t)pl.p, c = pl.cplx ? 2 : 1;
+    return align_up(c * p * p * 4) + align_up(p * (size_t)pl.r * 4) + align_up(c * p * 4) +
            align_up(8 * sizeof(int));
 }
 
@@ -116,17 +123,18 @@ ffdas_error_t blas_handle(ffdas_context &h, const Solver &s, rocblas_handle *out
     return FFDAS_SUCCESS;
 }
 
-// The tridiagonal path: T = Q^T H Q (rank_tridiag_kernel), the eigenpairs of T for the columns
+// The tridiagonal path: T = Q^H H Q (rank_tridiag_kernel), the eigenpairs of T for the columns
 // the projection reads (rank_tridiag_eigvec_kernel), then z = Q y into those columns of H
 // (rank_backtransform_kernel, which also sets *info).  The other columns of H keep A.
 ffdas_error_t eigh_tridiag(ffdas_context &h, const RankPlan &pl, void *H, void *D, void *E,
                            int *info, void *scratch) {
     const size_t p = (size_t)pl.p;
     char *w = (char *)scratch;
+    const size_t c = pl.cplx ? 2 : 1;
     float *U = (float *)w;
-    float *Y = (float *)(w + align_up(p * p * 4));
-    float *tau = (float *)(w + align_up(p * p * 4) + align_up(p * (size_t)pl.r * 4));
-    int *stat = (int *)((char *)tau + align_up(p * 4));
+    float *Y = (float *)(w + align_up(c * p * p * 4));
+    float *tau = (float *)(w + align_up(c * p * p * 4) + align_up(p * (size_t)pl.r * 4));
+    int *stat = (int *)((char *)tau + align_up(c * p * 4));
     FFDAS_TRY(launch_rank_tridiag(h.stream, pl, h.lds_bytes, H, D, E, U, tau, stat));
     FFDAS_TRY(launch_rank_tridiag_eigvec(h.stream, pl, D, E, Y, stat));
     FFDAS_TRY(launch_rank_backtransform(h.stream, pl, h.lds_bytes, U, tau, Y, stat, H, info));

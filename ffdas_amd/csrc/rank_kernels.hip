@@ -1179,9 +1179,9 @@ void project_dispatch(hipStream_t st, const RankPlan &pl, const void *x, void *o
 }
 
 // ---------------------------------------------------------------------------------------
-// Real FP32 eigenvectors for 64 < p <= 224: Householder tridiagonalisation in one workgroup,
-// the tridiagonal's chosen eigenpairs by bisection and inverse iteration in one workgroup,
-// then the back-transformation of those vectors alone.
+// FP32 eigenvectors (real p <= 224, complex p <= 160): Householder tridiagonalisation in one
+// workgroup, the tridiagonal's chosen eigenpairs by bisection and inverse iteration in one
+// workgroup, then the back-transformation of those vectors alone.
 // ---------------------------------------------------------------------------------------
 
 // DPP lane moves (GFX9 controls): quad permutations, row mirrors, and the row broadcasts of
@@ -1216,127 +1216,196 @@ __device__ __forceinline__ float wave_total(float v) {
 
 __host__ __device__ constexpr int packed_column(int p, int k) { return k * p - k * (k + 1) / 2; }
 
-// One reduction step's register work on the blocks B0.. (rows and columns below 32 B0 are done):
-// ps[i] = tau (A' u)_i from the lanes that hold the half-wave sums (tj = 31), written for every
-// row of the live blocks (rows <= k or >= p are never read as such).
-template <int NB, int B0>
-__device__ __forceinline__ void tridiag_matvec(const float (&a)[NB][NB], const float *x, float sc,
-                                               float t, int k, int p, int ti, int tj,
-                                               float *ps) {
-    float uj[NB];
+// Register tile of the tridiagonalisation: thread (ti, tj) = (tid / 32, tid % 32) of the 32 TR
+// holds A[ti + TR a][tj + 32 b] for a < NBR = 32 NBC / TR, b < NBC (real parts in re, imaginary
+// parts in im for a Hermitian A).  The tile covers P = 32 NBC rows and columns.
+template <int TR, int NBC, bool CPLX>
+struct TriTile {
+    static constexpr int NBR = 32 * NBC / TR;
+    float re[NBR][NBC];
+    float im[CPLX ? NBR : 1][CPLX ? NBC : 1];
+};
+
+// u_j = 1 at j = k + 1, (x_j * s) for k + 1 < j < p, 0 elsewhere (s complex for a Hermitian A).
+template <bool CPLX>
+__device__ __forceinline__ void tri_u(int j, int k, int p, float xr, float xi, float sr, float si,
+                                      float &ur, float &ui) {
+    const bool live = j > k + 1 && j < p;
+    const float vr = CPLX ? xr * sr - xi * si : xr * sr;
+    const float vi = CPLX ? xr * si + xi * sr : 0.f;
+    ur = j == k + 1 ? 1.f : (live ? vr : 0.f);
+    ui = live ? vi : 0.f;
+}
+
+// One reduction step's matrix-vector product on the live blocks (column blocks from B0, row
+// blocks from 32 B0 / TR; rows and columns below 32 B0 are done): ps = tau A' u, written by
+// the lanes that hold the half-wave sums (tj = 31) for every row of the live blocks (rows
+// <= k or >= p are never read as such).
+template <int TR, int NBC, bool CPLX, int B0>
+__device__ __forceinline__ void tri_matvec(const TriTile<TR, NBC, CPLX> &A, const float *xr,
+                                           const float *xi, float sr, float si, float tr,
+                                           float tim, int k, int p, int ti, int tj, float *pr,
+                                           float *pim) {
+    constexpr int NBR = TriTile<TR, NBC, CPLX>::NBR, IA0 = 32 * B0 / TR;
+    float ur[NBC], ui[NBC];
 #pragma unroll
-    for (int jb = B0; jb < NB; ++jb) {
+    for (int jb = B0; jb < NBC; ++jb) {
         const int j = tj + 32 * jb;
-        const float xv = x[j];
-        uj[jb] = j == k + 1 ? 1.f : (j > k + 1 && j < p ? xv * sc : 0.f);
+        tri_u<CPLX>(j, k, p, xr[j], CPLX ? xi[j] : 0.f, sr, si, ur[jb], ui[jb]);
     }
-    float acc[NB];
+    float accr[NBR], acci[NBR];
 #pragma unroll
-    for (int ia = B0; ia < NB; ++ia) {
-        float s = 0.f;
+    for (int ia = IA0; ia < NBR; ++ia) {
+        float sre = 0.f, sim = 0.f;
 #pragma unroll
-        for (int jb = B0; jb < NB; ++jb) s = fmaf(a[ia][jb], uj[jb], s);
-        acc[ia] = half_wave_sum_hi(s);
+        for (int jb = B0; jb < NBC; ++jb) {
+            sre = fmaf(A.re[ia][jb], ur[jb], sre);
+            if constexpr (CPLX) {
+                sre = fmaf(-A.im[ia][jb], ui[jb], sre);
+                sim = fmaf(A.re[ia][jb], ui[jb], fmaf(A.im[ia][jb], ur[jb], sim));
+            }
+        }
+        accr[ia] = half_wave_sum_hi(sre);
+        if constexpr (CPLX) acci[ia] = half_wave_sum_hi(sim);
     }
     if (tj == 31) {
 #pragma unroll
-        for (int ia = B0; ia < NB; ++ia) ps[ti + 32 * ia] = t * acc[ia];
+        for (int ia = IA0; ia < NBR; ++ia) {
+            const int i = ti + TR * ia;
+            if constexpr (CPLX) {
+                pr[i] = tr * accr[ia] - tim * acci[ia];
+                pim[i] = tr * acci[ia] + tim * accr[ia];
+            } else {
+                pr[i] = tr * accr[ia];
+            }
+        }
     }
 }
 
-// A' -= u w^T + w u^T on the blocks B0.., w = ps - K u (u and w are 0 outside rows k+1 .. p-1).
-template <int NB, int B0>
-__device__ __forceinline__ void tridiag_update(float (&a)[NB][NB], const float *x, const float *ps,
-                                               float sc, float K, int k, int p, int ti, int tj) {
-    float uj[NB], wj[NB];
+// A' -= u w^H + w u^H on the live blocks, w = ps + K u (u and w are 0 outside rows k+1 .. p-1).
+template <int TR, int NBC, bool CPLX, int B0>
+__device__ __forceinline__ void tri_update(TriTile<TR, NBC, CPLX> &A, const float *xr,
+                                           const float *xi, const float *pr, const float *pim,
+                                           float sr, float si, float Kr, float Ki, int k, int p,
+                                           int ti, int tj) {
+    constexpr int NBR = TriTile<TR, NBC, CPLX>::NBR, IA0 = 32 * B0 / TR;
+    float ujr[NBC], uji[NBC], wjr[NBC], wji[NBC];
 #pragma unroll
-    for (int jb = B0; jb < NB; ++jb) {
+    for (int jb = B0; jb < NBC; ++jb) {
         const int j = tj + 32 * jb;
-        const float xj = x[j], pj = ps[j];
+        tri_u<CPLX>(j, k, p, xr[j], CPLX ? xi[j] : 0.f, sr, si, ujr[jb], uji[jb]);
         const bool lj = j > k && j < p;
-        uj[jb] = j == k + 1 ? 1.f : (lj ? xj * sc : 0.f);
-        wj[jb] = lj ? fmaf(-K, uj[jb], pj) : 0.f;
+        const float p_r = pr[j], p_i = CPLX ? pim[j] : 0.f;
+        wjr[jb] = lj ? p_r + (CPLX ? Kr * ujr[jb] - Ki * uji[jb] : Kr * ujr[jb]) : 0.f;
+        wji[jb] = lj && CPLX ? p_i + Kr * uji[jb] + Ki * ujr[jb] : 0.f;
     }
 #pragma unroll
-    for (int ia = B0; ia < NB; ++ia) {
-        const int i = ti + 32 * ia;
-        const float xi = x[i], pi = ps[i];
+    for (int ia = IA0; ia < NBR; ++ia) {
+        const int i = ti + TR * ia;
+        float uir, uii;
+        tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, uir, uii);
         const bool li = i > k && i < p;
-        const float ui = i == k + 1 ? 1.f : (li ? xi * sc : 0.f);
-        const float wi = li ? fmaf(-K, ui, pi) : 0.f;
+        const float p_r = pr[i], p_i = CPLX ? pim[i] : 0.f;
+        const float wir = li ? p_r + (CPLX ? Kr * uir - Ki * uii : Kr * uir) : 0.f;
+        const float wii = li && CPLX ? p_i + Kr * uii + Ki * uir : 0.f;
 #pragma unroll
-        for (int jb = B0; jb < NB; ++jb)
-            a[ia][jb] = fmaf(-wi, uj[jb], fmaf(-ui, wj[jb], a[ia][jb]));
+        for (int jb = B0; jb < NBC; ++jb) {
+            // u_i conj(w_j) + w_i conj(u_j)
+            float r = fmaf(uir, wjr[jb], wir * ujr[jb]);
+            if constexpr (CPLX) {
+                r = fmaf(uii, wji[jb], fmaf(wii, uji[jb], r));
+                const float m = fmaf(uii, wjr[jb], fmaf(-uir, wji[jb],
+                                                        fmaf(wii, ujr[jb], -wir * uji[jb])));
+                A.im[ia][jb] -= m;
+            }
+            A.re[ia][jb] -= r;
+        }
     }
 }
 
-// Column B of the register tile into col: all NB entries of the owning lanes, rows at or above
-// the column index included (the reduction reads the rows below it, and the diagonal for D).
-template <int NB, int B>
-__device__ __forceinline__ void tridiag_put_column(const float (&a)[NB][NB], int ti, float *col) {
+// Column B of the tile into (cr, ci): all NBR entries of the owning lanes, rows at or above the
+// column index included (the reduction reads the rows below it, and the diagonal for D).
+template <int TR, int NBC, bool CPLX, int B>
+__device__ __forceinline__ void tri_put_column(const TriTile<TR, NBC, CPLX> &A, int ti, float *cr,
+                                               float *ci) {
 #pragma unroll
-    for (int ia = 0; ia < NB; ++ia) col[ti + 32 * ia] = a[ia][B];
+    for (int ia = 0; ia < TriTile<TR, NBC, CPLX>::NBR; ++ia) {
+        cr[ti + TR * ia] = A.re[ia][B];
+        if constexpr (CPLX) ci[ti + TR * ia] = A.im[ia][B];
+    }
 }
 
-// A uniform switch over the block index b (0 .. 7) calling F<B> for B < NB: a straight-line body
-// per live-block count, no per-element branches.
+// A uniform switch over the block index b (0 .. 6) calling CALL(B) for B < NBC: a straight-line
+// body per live-block count, no per-element branches.
 #define FFDAS_TRIDIAG_SWITCH(b, CALL)                                                          \
     switch (b) {                                                                               \
         case 0: CALL(0); break;                                                                \
-        case 1: if constexpr (NB > 1) CALL(1); break;                                          \
-        case 2: if constexpr (NB > 2) CALL(2); break;                                          \
-        case 3: if constexpr (NB > 3) CALL(3); break;                                          \
-        case 4: if constexpr (NB > 4) CALL(4); break;                                          \
-        case 5: if constexpr (NB > 5) CALL(5); break;                                          \
-        case 6: if constexpr (NB > 6) CALL(6); break;                                          \
-        default: if constexpr (NB > 7) CALL(7); break;                                         \
+        case 1: if constexpr (NBC > 1) CALL(1); break;                                         \
+        case 2: if constexpr (NBC > 2) CALL(2); break;                                         \
+        case 3: if constexpr (NBC > 3) CALL(3); break;                                         \
+        case 4: if constexpr (NBC > 4) CALL(4); break;                                         \
+        case 5: if constexpr (NBC > 5) CALL(5); break;                                         \
+        default: if constexpr (NBC > 6) CALL(6); break;                                        \
     }
 
-// T = Q^T A Q for the p x p symmetric A (column-major, both triangles) by p - 2 Householder
-// reflections taken from the columns (LAPACK ssytd2, lower form): reflection k,
-// H_k = I - tau_k u_k u_k^T with u_k[k + 1] = 1 and u_k[i] = 0 for i <= k, zeroes column k
-// below the subdiagonal, and Q = H_0 H_1 ... H_{p-3}.  Thread (ti, tj) = (tid / 32, tid % 32)
-// of the 1024 holds A[ti + 32 a][tj + 32 b] (a, b < NB) in registers.  A wave issues one
-// instruction per four cycles, so a step is written as straight-line code: one uniform dispatch
-// on the first live 32 x 32 block picks a fully unrolled body over the live blocks, and rows or
-// columns outside k+1 .. p-1 drop out through zero u and w rather than branches.  A step takes
-// two barriers: the owners of column k have put it in LDS (double-buffered by step parity),
-// every wave forms the same reflector from it (its own DPP reductions, so no third barrier),
-// the product A u goes through LDS, and after the rank-2 update the owners of column k + 1
-// store it.  The reflectors, D, E and tau collect in LDS (dynamic: the packed reflectors take
-// p (p - 1) / 2 floats) and go to memory once at the end, so no barrier waits on stores.
+// T = Q^H A Q for the p x p Hermitian (or real symmetric) A, column-major with both triangles,
+// by Householder reflections taken from the columns (LAPACK ssytd2 / chetd2, lower form):
+// reflection k, H_k = I - tau_k u_k u_k^H with u_k[k + 1] = 1 and u_k[i] = 0 for i <= k, turns
+// column k below the diagonal into (beta_k, 0, ..., 0) with beta_k real, and
+// Q = H_0 H_1 ... H_{m-1} (m = p - 2 real; m = p - 1 complex, the last reflection making
+// A[p-1][p-2] real).  T is real symmetric either way.
+// The tile lives in registers (TriTile); a wave issues one instruction per four cycles and every
+// wave repeats the per-step scalar work, so fewer, fuller lanes (TR = 8 or 16 rows of threads)
+// beat 1024 threads.  A step is straight-line code: one uniform dispatch on the first live
+// 32-column block picks a fully unrolled body over the live blocks, and rows or columns outside
+// k+1 .. p-1 drop out through zero u and w rather than branches.  A step takes two barriers:
+// the owners of column k have put it in LDS (double-buffered by step parity), every wave forms
+// the same reflector from it (its own DPP reductions, so no third barrier), the product A u
+// goes through LDS, and after the rank-2 update the owners of column k + 1 store it.  The
+// reflectors, D, E and tau collect in LDS (dynamic: the packed reflectors take p (p - 1) / 2
+// elements) and go to memory once at the end, so no barrier waits on stores.
 // A is first scaled by the power of two that brings its largest magnitude into [0.5, 1)
 // (exact, and eigenvectors do not change), so squares neither overflow nor underflow for Gram
 // matrices of data far from unit magnitude.
-// Out: D (p) and E (p - 1) of the scaled T, tau (p - 2) and the reflectors packed by column
-// (u_k[k + 1 .. p - 1] from Upk[k p - k (k + 1) / 2]).  stat[0..7] (status words of the
-// kernels that follow) are cleared.
-template <int NB>
-__global__ __launch_bounds__(1024) void rank_tridiag_kernel(const float *__restrict__ A, int p,
-                                                            float *__restrict__ D,
-                                                            float *__restrict__ E,
-                                                            float *__restrict__ Upk,
-                                                            float *__restrict__ tau,
-                                                            int *__restrict__ stat) {
-    constexpr int NS = (32 * NB + 63) / 64;   // column slots per lane
-    __shared__ float xs[2][64 * NS], ps[32 * NB];
-    __shared__ float red[16];
-    __shared__ float Ds[32 * NB], Es[32 * NB], Ts[32 * NB];
-    extern __shared__ float Us[];   // p (p - 1) / 2 packed reflectors
+// Out: D (p) and E (p - 1) of the scaled T, tau (m; complex as pairs) and the reflectors packed
+// by column (u_k[k + 1 .. p - 1] from element k p - k (k + 1) / 2 of Upk).  stat[0..7] (status
+// words of the kernels that follow) are cleared.
+template <int TR, int NBC, bool CPLX>
+__global__ __launch_bounds__(32 * TR) void rank_tridiag_kernel(const float *__restrict__ Ain,
+                                                               int p, float *__restrict__ D,
+                                                               float *__restrict__ E,
+                                                               float *__restrict__ Upk,
+                                                               float *__restrict__ tau,
+                                                               int *__restrict__ stat) {
+    constexpr int NT = 32 * TR, P = 32 * NBC, NW = NT / 64, C = CPLX ? 2 : 1;
+    constexpr int NS = (P + 63) / 64;   // row slots per lane in the wave-wide loops
+    constexpr int NBR = TriTile<TR, NBC, CPLX>::NBR;
+    __shared__ float xs[2][C][64 * NS], ps[C][64 * NS];
+    __shared__ float red[NW];
+    __shared__ float Ds[P], Es[P], Ts[C * P];
+    extern __shared__ float Us[];   // p (p - 1) / 2 packed reflector elements
     const int tid = threadIdx.x, ti = tid >> 5, tj = tid & 31, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid < 8) stat[tid] = 0;
-    for (int i = tid; i < 64 * NS; i += 1024) xs[0][i] = xs[1][i] = 0.f;
-    for (int i = tid; i < 32 * NB; i += 1024) ps[i] = 0.f;
-    float a[NB][NB];
+    for (int i = tid; i < 64 * NS; i += NT) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) xs[0][c][i] = xs[1][c][i] = ps[c][i] = 0.f;
+    }
+    TriTile<TR, NBC, CPLX> A;
     float amax = 0.f;
 #pragma unroll
-    for (int ia = 0; ia < NB; ++ia)
+    for (int ia = 0; ia < NBR; ++ia)
 #pragma unroll
-        for (int jb = 0; jb < NB; ++jb) {
-            const int i = ti + 32 * ia, j = tj + 32 * jb;
-            a[ia][jb] = (i < p && j < p) ? A[(size_t)j * p + i] : 0.f;
-            amax = fmaxf(amax, fabsf(a[ia][jb]));
+        for (int jb = 0; jb < NBC; ++jb) {
+            const int i = ti + TR * ia, j = tj + 32 * jb;
+            const bool in = i < p && j < p;
+            const size_t e = (size_t)j * p + i;
+            A.re[ia][jb] = in ? Ain[C * e] : 0.f;
+            amax = fmaxf(amax, fabsf(A.re[ia][jb]));
+            if constexpr (CPLX) {
+                A.im[ia][jb] = in ? Ain[2 * e + 1] : 0.f;
+                amax = fmaxf(amax, fabsf(A.im[ia][jb]));
+            }
         }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
@@ -1344,96 +1413,132 @@ __global__ __launch_bounds__(1024) void rank_tridiag_kernel(const float *__restr
     __syncthreads();
     amax = red[0];
 #pragma unroll
-    for (int w = 1; w < 16; ++w) amax = fmaxf(amax, red[w]);
+    for (int w = 1; w < NW; ++w) amax = fmaxf(amax, red[w]);
     if (amax > 0.f && amax <= 3.4e38f) {
         int e;
         (void)frexpf(amax, &e);
 #pragma unroll
-        for (int ia = 0; ia < NB; ++ia)
+        for (int ia = 0; ia < NBR; ++ia)
 #pragma unroll
-            for (int jb = 0; jb < NB; ++jb) a[ia][jb] = ldexpf(a[ia][jb], -e);
+            for (int jb = 0; jb < NBC; ++jb) {
+                A.re[ia][jb] = ldexpf(A.re[ia][jb], -e);
+                if constexpr (CPLX) A.im[ia][jb] = ldexpf(A.im[ia][jb], -e);
+            }
     }
-#define FFDAS_PUT(B) tridiag_put_column<NB, (B < NB ? B : 0)>(a, ti, col)
-#define FFDAS_MATVEC(B) tridiag_matvec<NB, (B < NB ? B : 0)>(a, x, sc, t, k, p, ti, tj, ps)
-#define FFDAS_UPDATE(B) tridiag_update<NB, (B < NB ? B : 0)>(a, x, ps, sc, K, k, p, ti, tj)
+#define FFDAS_PUT(B) tri_put_column<TR, NBC, CPLX, (B < NBC ? B : 0)>(A, ti, cr, ci)
+#define FFDAS_MATVEC(B)                                                                        \
+    tri_matvec<TR, NBC, CPLX, (B < NBC ? B : 0)>(A, xr, xi, sr, si, tr, tim, k, p, ti, tj,     \
+                                                  ps[0], ps[C - 1])
+#define FFDAS_UPDATE(B)                                                                        \
+    tri_update<TR, NBC, CPLX, (B < NBC ? B : 0)>(A, xr, xi, ps[0], ps[C - 1], sr, si, Kr, Ki,  \
+                                                  k, p, ti, tj)
     if (tj == 0) {
-        float *col = xs[0];
+        float *cr = xs[0][0], *ci = xs[0][C - 1];
         FFDAS_PUT(0);
     }
-    for (int k = 0; k < p - 2; ++k) {
+    const int steps = CPLX ? p - 1 : p - 2;
+    for (int k = 0; k < steps; ++k) {
         __syncthreads();
-        const float *x = xs[k & 1];
+        const float *xr = xs[k & 1][0], *xi = xs[k & 1][C - 1];
         // u = (x - beta e_1) / (alpha - beta) for x = A[k+1:, k], alpha = x[0],
-        // beta = -sign(alpha) |x|, tau = (beta - alpha) / beta; no reflection (tau = 0) when x
-        // is already a multiple of e_1.  Every wave reduces the same values in the same order.
-        const float alpha = x[k + 1];
+        // beta = -sign(Re alpha) |x|, tau = (beta - Re alpha) / beta - i Im alpha / beta; no
+        // reflection (tau = 0) when x is already a real multiple of e_1.  Every wave reduces
+        // the same values in the same order.
+        const float alr = xr[k + 1], ali = CPLX ? xi[k + 1] : 0.f;
         float ss = 0.f;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             const int i = lane + 64 * sl;
-            const float v = x[i];
-            ss = fmaf(v, i > k + 1 && i < p ? v : 0.f, ss);
+            const bool live = i > k + 1 && i < p;
+            const float vr = xr[i];
+            ss = fmaf(vr, live ? vr : 0.f, ss);
+            if constexpr (CPLX) {
+                const float vi = xi[i];
+                ss = fmaf(vi, live ? vi : 0.f, ss);
+            }
         }
         ss = wave_total(ss);
-        float t = 0.f, beta = alpha, sc = 0.f;
-        if (ss != 0.f) {
-            const float nrm = sqrtf(fmaf(alpha, alpha, ss));
-            const float b = alpha >= 0.f ? -nrm : nrm;
-            t = (b - alpha) / b;
-            sc = 1.f / (alpha - b);
+        float tr = 0.f, tim = 0.f, beta = alr, sr = 0.f, si = 0.f;
+        if (ss != 0.f || ali != 0.f) {
+            const float nrm = sqrtf(fmaf(alr, alr, fmaf(ali, ali, ss)));
+            const float b = alr >= 0.f ? -nrm : nrm;
+            const float rb = 1.f / b;
+            tr = (b - alr) * rb;
+            tim = -ali * rb;
+            // 1 / (alpha - beta)
+            const float dr = alr - b;
+            const float rd = 1.f / fmaf(dr, dr, ali * ali);
+            sr = dr * rd;
+            si = -ali * rd;
             beta = b;
         }
         if (wave == 0) {
             const int o = packed_column(p, k) - (k + 1);
-            for (int i = k + 1 + lane; i < p; i += 64) Us[o + i] = i == k + 1 ? 1.f : x[i] * sc;
+            for (int i = k + 1 + lane; i < p; i += 64) {
+                float ur, ui;
+                tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, ur, ui);
+                Us[C * (o + i)] = ur;
+                if constexpr (CPLX) Us[2 * (o + i) + 1] = ui;
+            }
             if (lane == 0) {
-                Ds[k] = x[k];
+                Ds[k] = xr[k];
                 Es[k] = beta;
-                Ts[k] = t;
+                Ts[C * k] = tr;
+                if constexpr (CPLX) Ts[2 * k + 1] = tim;
             }
         }
-        if (t != 0.f) {   // the same value in every wave
-            const int b0 = (k + 1) >> 5;   // first block holding a row or column > k
+        if (tr != 0.f || tim != 0.f) {   // the same values in every wave
+            const int b0 = (k + 1) >> 5;   // first block holding a column > k
             FFDAS_TRIDIAG_SWITCH(b0, FFDAS_MATVEC)
             __syncthreads();
-            // K = (tau / 2)(ps . u)
-            float dot = 0.f;
+            // K = -(tau / 2)(ps^H u)
+            float dr = 0.f, di = 0.f;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 const int i = lane + 64 * sl;
-                const float pv = ps[i < 32 * NB ? i : 0], xv = x[i];
-                const float u = i == k + 1 ? 1.f : xv * sc;
-                dot = fmaf(i > k && i < p ? pv : 0.f, u, dot);
+                const bool live = i > k && i < p;
+                float ur, ui;
+                tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, ur, ui);
+                const float p_r = live ? ps[0][i] : 0.f;
+                dr = fmaf(p_r, ur, dr);
+                if constexpr (CPLX) {
+                    const float p_i = live ? ps[1][i] : 0.f;
+                    dr = fmaf(p_i, ui, dr);
+                    di = fmaf(p_r, ui, fmaf(-p_i, ur, di));
+                }
             }
-            const float K = 0.5f * t * wave_total(dot);
+            dr = wave_total(dr);
+            if constexpr (CPLX) di = wave_total(di);
+            const float Kr = -0.5f * (tr * dr - tim * di);
+            const float Ki = -0.5f * (tr * di + tim * dr);
             FFDAS_TRIDIAG_SWITCH(b0, FFDAS_UPDATE)
         }
         if (tj == ((k + 1) & 31)) {
-            float *col = xs[(k + 1) & 1];
+            float *cr = xs[(k + 1) & 1][0], *ci = xs[(k + 1) & 1][C - 1];
             FFDAS_TRIDIAG_SWITCH((k + 1) >> 5, FFDAS_PUT)
         }
     }
 #undef FFDAS_PUT
 #undef FFDAS_MATVEC
 #undef FFDAS_UPDATE
-    // the trailing 2 x 2 block
+    // the trailing entries: D[p-1] (and D[p-2], E[p-2] when real, whose last step is empty)
 #pragma unroll
-    for (int ia = 0; ia < NB; ++ia)
+    for (int ia = 0; ia < NBR; ++ia)
 #pragma unroll
-        for (int jb = 0; jb < NB; ++jb) {
-            const int i = ti + 32 * ia, j = tj + 32 * jb;
-            if (i >= p || j >= p || i < p - 2) continue;
-            if (i == j) Ds[i] = a[ia][jb];
-            if (i == p - 1 && j == p - 2) Es[p - 2] = a[ia][jb];
+        for (int jb = 0; jb < NBC; ++jb) {
+            const int i = ti + TR * ia, j = tj + 32 * jb;
+            if (i >= p || j >= p || i < steps) continue;
+            if (i == j) Ds[i] = A.re[ia][jb];
+            if (!CPLX && i == p - 1 && j == p - 2) Es[p - 2] = A.re[ia][jb];
         }
     __syncthreads();
-    const int npk = p * (p - 1) / 2;
-    for (int e = tid; e < npk; e += 1024) Upk[e] = Us[e];
-    for (int i = tid; i < p; i += 1024) {
+    const int npk = C * (p * (p - 1) / 2);
+    for (int e = tid; e < npk; e += NT) Upk[e] = Us[e];
+    for (int i = tid; i < p; i += NT) {
         D[i] = Ds[i];
         if (i < p - 1) E[i] = Es[i];
-        if (i < p - 2) tau[i] = Ts[i];
     }
+    for (int i = tid; i < C * steps; i += NT) tau[i] = Ts[i];
 }
 
 // Number of eigenvalues of the symmetric tridiagonal T below x, by the signs of the pivots of
@@ -1726,64 +1831,94 @@ __global__ __launch_bounds__(64 * kEigWaves) void rank_tridiag_eigvec_kernel(
     if (bad && lane == 0) atomicOr(stat, 1);
 }
 
-// Eigenvectors of A from those of its tridiagonal form: z = H_0 H_1 ... H_{p-3} y, the
-// reflections applied last to first, one wave per vector with NE elements per lane
-// (p <= 64 NE).  The workgroup first copies the packed reflectors into LDS (dynamic,
-// p (p - 1) / 2 floats), then each wave reads the next reflector while it applies the current
-// one.  Vector q (column q of Y) goes to column c_first + q, or c_second + (q - q_split) for
-// q >= q_split, of V (column-major p x p), where the projection reads it.  *info = stat[0]
-// (set by the eigenpair kernel when a vector is not finite).
-template <int NE>
+// Eigenvectors of A from those of its tridiagonal form: z = H_0 H_1 ... H_{m-1} y, the
+// reflections applied last to first (z -= tau_k u_k (u_k^H z)), one wave per vector with NE
+// elements per lane (p <= 64 NE).  The workgroup first copies the packed reflectors into LDS
+// (dynamic, p (p - 1) / 2 elements), then each wave reads the next reflector while it applies
+// the current one.  Vector q (column q of the real Y) goes to column c_first + q, or
+// c_second + (q - q_split) for q >= q_split, of V (column-major p x p, complex as pairs),
+// where the projection reads it.  *info = stat[0] (set by the eigenpair kernel when a vector
+// is not finite).
+template <int NE, bool CPLX>
 __global__ __launch_bounds__(256) void rank_backtransform_kernel(
     const float *__restrict__ Upk, const float *__restrict__ tau, const float *__restrict__ Y,
     int p, int r, int q_split, int c_first, int c_second, const int *__restrict__ stat,
     float *__restrict__ V, int *__restrict__ info) {
+    constexpr int C = CPLX ? 2 : 1;
     extern __shared__ float Us[];
     const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x == 0) *info = stat[0] != 0 ? 1 : 0;
-    const int npk = p * (p - 1) / 2;
+    const int npk = C * (p * (p - 1) / 2);
     for (int e = threadIdx.x; e < npk; e += 256) Us[e] = Upk[e];
     __syncthreads();
     if (q >= r) return;
     const float *y = Y + (size_t)q * p;
-    float v[NE];
+    float zr[NE], zi[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         const int i = lane + 64 * e;
-        v[e] = i < p ? y[i] : 0.f;
+        zr[e] = i < p ? y[i] : 0.f;
+        zi[e] = 0.f;
     }
-    // u_k[i] for i > k (u_k[i] = 0 below), loaded one reflector ahead
-    const auto load = [&](int k, float (&u)[NE]) {
+    // u_k[i] for i > k (0 below), loaded one reflector ahead
+    const auto load = [&](int k, float (&ur)[NE], float (&ui)[NE]) {
         const int o = packed_column(p, k) - (k + 1);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int i = lane + 64 * e;
-            u[e] = (k >= 0 && i > k && i < p) ? Us[o + i] : 0.f;
+            const bool in = k >= 0 && i > k && i < p;
+            ur[e] = in ? Us[C * (o + i)] : 0.f;
+            ui[e] = in && CPLX ? Us[C * (o + i) + C - 1] : 0.f;
         }
     };
-    float un[NE];
-    float tn = p >= 3 ? tau[p - 3] : 0.f;
-    load(p - 3, un);
-    for (int k = p - 3; k >= 0; --k) {
-        float u[NE];
+    const int kt = CPLX ? p - 2 : p - 3;   // the last reflection
+    float unr[NE], uni[NE];
+    float tnr = kt >= 0 ? tau[C * kt] : 0.f, tni = kt >= 0 && CPLX ? tau[C * kt + C - 1] : 0.f;
+    load(kt, unr, uni);
+    for (int k = kt; k >= 0; --k) {
+        float ur[NE], ui[NE];
 #pragma unroll
-        for (int e = 0; e < NE; ++e) u[e] = un[e];
-        const float t = tn;
-        tn = k >= 1 ? tau[k - 1] : 0.f;
-        load(k - 1, un);
-        float s = 0.f;
+        for (int e = 0; e < NE; ++e) {
+            ur[e] = unr[e];
+            ui[e] = uni[e];
+        }
+        const float t_r = tnr, t_i = tni;
+        tnr = k >= 1 ? tau[C * (k - 1)] : 0.f;
+        tni = k >= 1 && CPLX ? tau[C * (k - 1) + C - 1] : 0.f;
+        load(k - 1, unr, uni);
+        // s = u^H z
+        float sre = 0.f, sim = 0.f;
 #pragma unroll
-        for (int e = 0; e < NE; ++e) s = fmaf(u[e], v[e], s);
-        const float ts = t * wave_total(s);
+        for (int e = 0; e < NE; ++e) {
+            sre = fmaf(ur[e], zr[e], sre);
+            if constexpr (CPLX) {
+                sre = fmaf(ui[e], zi[e], sre);
+                sim = fmaf(ur[e], zi[e], fmaf(-ui[e], zr[e], sim));
+            }
+        }
+        sre = wave_total(sre);
+        if constexpr (CPLX) sim = wave_total(sim);
+        // z -= (tau s) u
+        const float ar = t_r * sre - t_i * sim, ai = t_r * sim + t_i * sre;
 #pragma unroll
-        for (int e = 0; e < NE; ++e) v[e] = fmaf(-ts, u[e], v[e]);
+        for (int e = 0; e < NE; ++e) {
+            zr[e] = fmaf(-ar, ur[e], zr[e]);
+            if constexpr (CPLX) {
+                zr[e] = fmaf(ai, ui[e], zr[e]);
+                zi[e] = fmaf(-ar, ui[e], fmaf(-ai, ur[e], zi[e]));
+            }
+        }
     }
     const int col = q < q_split ? c_first + q : c_second + (q - q_split);
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         const int i = lane + 64 * e;
-        if (i < p) V[(size_t)col * p + i] = v[e];
+        if (i < p) {
+            const size_t o = (size_t)col * p + i;
+            V[C * o] = zr[e];
+            if constexpr (CPLX) V[2 * o + 1] = zi[e];
+        }
     }
 }
 
@@ -1958,16 +2093,19 @@ ffdas_error_t launch_rank_copy(hipStream_t st, bool dbl, bool cplx, const void *
 }
 
 
-size_t rank_tridiag_lds(int p) { return (size_t)p * (p - 1) / 2 * sizeof(float); }
+size_t rank_tridiag_lds(int p, bool cplx) {
+    return (size_t)p * (p - 1) / 2 * (cplx ? 8 : 4);
+}
 
-// p <= 224: the 1024 threads hold A in 7 x 7 registers each (8 x 8 for p = 256 spills to
-// scratch at the 128 registers a lane has in a 1024-thread workgroup); the tridiagonalisation's
-// static LDS stays under 8 KiB.
-constexpr int kTridiagMaxP = 224;
+// Real p <= 224, complex p <= 160: the tile of A fits the registers of its lanes (TR = 8 rows of
+// threads up to 192 real / 128 complex columns, 16 above; a 7-block real tile at TR = 8, or a
+// 6-block complex one at TR = 16, spills); the tridiagonalisation's static LDS stays under
+// 12 KiB.
+int tridiag_max_p(bool cplx) { return cplx ? 160 : 224; }
 
 bool rank_tridiag_fits(const RankPlan &pl, int lds_bytes) {
-    return !pl.dbl && !pl.cplx && pl.p >= 2 && pl.p <= kTridiagMaxP &&
-           rank_tridiag_lds(pl.p) + 8192 <= (size_t)lds_bytes;
+    return !pl.dbl && pl.p >= 2 && pl.p <= tridiag_max_p(pl.cplx) &&
+           rank_tridiag_lds(pl.p, pl.cplx) + 12288 <= (size_t)lds_bytes;
 }
 
 ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_bytes,
@@ -1975,23 +2113,34 @@ ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_by
                                   int *stat) {
     if (!rank_tridiag_fits(pl, lds_bytes)) return FFDAS_ERROR_INVALID_ARGUMENT;
     ProfScope ps(st, "rank_tridiag");
-    const size_t lds = rank_tridiag_lds(pl.p);
-#define FFDAS_RANK_TRIDIAG(NB)                                                                 \
+    const size_t lds = rank_tridiag_lds(pl.p, pl.cplx);
+#define FFDAS_RANK_TRIDIAG(TR, NBC, CPLX)                                                      \
     do {                                                                                       \
-        (void)hipFuncSetAttribute((const void *)rank_tridiag_kernel<NB>,                       \
+        (void)hipFuncSetAttribute((const void *)rank_tridiag_kernel<TR, NBC, CPLX>,            \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
-        hipLaunchKernelGGL((rank_tridiag_kernel<NB>), dim3(1), dim3(1024), lds, st,            \
-                           (const float *)H, pl.p, (float *)D, (float *)E, (float *)U,         \
+        hipLaunchKernelGGL((rank_tridiag_kernel<TR, NBC, CPLX>), dim3(1), dim3(32 * TR), lds,  \
+                           st, (const float *)H, pl.p, (float *)D, (float *)E, (float *)U,     \
                            (float *)tau, stat);                                                \
     } while (0)
-    switch ((pl.p + 31) / 32) {
-        case 1: FFDAS_RANK_TRIDIAG(1); break;
-        case 2: FFDAS_RANK_TRIDIAG(2); break;
-        case 3: FFDAS_RANK_TRIDIAG(3); break;
-        case 4: FFDAS_RANK_TRIDIAG(4); break;
-        case 5: FFDAS_RANK_TRIDIAG(5); break;
-        case 6: FFDAS_RANK_TRIDIAG(6); break;
-        default: FFDAS_RANK_TRIDIAG(7); break;
+    const int nbc = (pl.p + 31) / 32;
+    if (pl.cplx) {
+        switch (nbc) {
+            case 1: FFDAS_RANK_TRIDIAG(8, 1, true); break;
+            case 2: FFDAS_RANK_TRIDIAG(8, 2, true); break;
+            case 3: FFDAS_RANK_TRIDIAG(8, 3, true); break;
+            case 4: FFDAS_RANK_TRIDIAG(8, 4, true); break;
+            default: FFDAS_RANK_TRIDIAG(16, 5, true); break;
+        }
+    } else {
+        switch (nbc) {
+            case 1: FFDAS_RANK_TRIDIAG(8, 1, false); break;
+            case 2: FFDAS_RANK_TRIDIAG(8, 2, false); break;
+            case 3: FFDAS_RANK_TRIDIAG(8, 3, false); break;
+            case 4: FFDAS_RANK_TRIDIAG(8, 4, false); break;
+            case 5: FFDAS_RANK_TRIDIAG(8, 5, false); break;
+            case 6: FFDAS_RANK_TRIDIAG(8, 6, false); break;
+            default: FFDAS_RANK_TRIDIAG(16, 7, false); break;
+        }
     }
 #undef FFDAS_RANK_TRIDIAG
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);
@@ -1999,7 +2148,7 @@ ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_by
 
 ffdas_error_t launch_rank_tridiag_eigvec(hipStream_t st, const RankPlan &pl, const void *D,
                                          const void *E, void *Y, int *stat) {
-    if (pl.dbl || pl.cplx || pl.p < 2 || pl.p > kTridiagMaxP || pl.r < 1 || pl.r > pl.p)
+    if (pl.dbl || pl.p < 2 || pl.p > tridiag_max_p(pl.cplx) || pl.r < 1 || pl.r > pl.p)
         return FFDAS_ERROR_INVALID_ARGUMENT;
     ProfScope ps(st, "rank_tridiag_eigvec");
 #define FFDAS_RANK_EIGVEC(PMAX)                                                                \
@@ -2023,20 +2172,29 @@ ffdas_error_t launch_rank_backtransform(hipStream_t st, const RankPlan &pl, int 
         return FFDAS_ERROR_INVALID_ARGUMENT;
     ProfScope ps(st, "rank_backtransform");
     const unsigned blocks = (unsigned)((pl.r + 3) / 4);
-    const size_t lds = rank_tridiag_lds(pl.p);
-#define FFDAS_RANK_BACK(NE)                                                                    \
+    const size_t lds = rank_tridiag_lds(pl.p, pl.cplx);
+#define FFDAS_RANK_BACK(NE, CPLX)                                                              \
     do {                                                                                       \
-        (void)hipFuncSetAttribute((const void *)rank_backtransform_kernel<NE>,                 \
+        (void)hipFuncSetAttribute((const void *)rank_backtransform_kernel<NE, CPLX>,           \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
-        hipLaunchKernelGGL((rank_backtransform_kernel<NE>), dim3(blocks), dim3(256), lds, st,  \
-                           (const float *)U, (const float *)tau, (const float *)Y, pl.p, pl.r, \
-                           pl.q_split, pl.c_first, pl.c_second, stat, (float *)V, info);       \
+        hipLaunchKernelGGL((rank_backtransform_kernel<NE, CPLX>), dim3(blocks), dim3(256), lds, \
+                           st, (const float *)U, (const float *)tau, (const float *)Y, pl.p,   \
+                           pl.r, pl.q_split, pl.c_first, pl.c_second, stat, (float *)V, info); \
     } while (0)
-    switch ((pl.p + 63) / 64) {
-        case 1: FFDAS_RANK_BACK(1); break;
-        case 2: FFDAS_RANK_BACK(2); break;
-        case 3: FFDAS_RANK_BACK(3); break;
-        default: FFDAS_RANK_BACK(4); break;
+    const int ne = (pl.p + 63) / 64;
+    if (pl.cplx) {
+        switch (ne) {
+            case 1: FFDAS_RANK_BACK(1, true); break;
+            case 2: FFDAS_RANK_BACK(2, true); break;
+            default: FFDAS_RANK_BACK(3, true); break;
+        }
+    } else {
+        switch (ne) {
+            case 1: FFDAS_RANK_BACK(1, false); break;
+            case 2: FFDAS_RANK_BACK(2, false); break;
+            case 3: FFDAS_RANK_BACK(3, false); break;
+            default: FFDAS_RANK_BACK(4, false); break;
+        }
     }
 #undef FFDAS_RANK_BACK
     return hip_status(hipGetLastError(), FFDAS_ERROR_CUDA_LAUNCH_FAILED);

@@ -183,37 +183,30 @@ def test_matlab_gateway(dev):
         mex_truncate_rank(x, 5, 4)
 
 
-@pytest.mark.parametrize("solver", ["jacobi", "rocsolver"])
+@pytest.mark.parametrize("solver", ["jacobi", "rocsolver", "tridiag"])
 @pytest.mark.parametrize("dtype", ["float32", "complex64", "float64", "complex128"])
 @pytest.mark.parametrize("shape, start, stop", [((33, 500), 2, 20), ((64, 2000), 5, None),
                                                  ((2, 50), 1, None)])
 def test_both_eigensolvers(dev, monkeypatch, solver, dtype, shape, start, stop):
-    """The one-workgroup Jacobi kernel (odd p padded with an inert index) and rocSOLVER give
-    the oracle's result."""
+    """The one-workgroup Jacobi kernel (odd p padded with an inert index), the tridiagonal path
+    (FP32; FP64 falls back to rocSOLVER) and rocSOLVER give the oracle's result."""
     import torch
     monkeypatch.setenv("FFDAS_RANK_EIG", solver)
     x = _rand(shape, getattr(torch, dtype), 16).to(dev)
     _check(x, start, stop, TOL[dtype])
 
 
-def test_jacobi_is_the_default_for_small_p(dev, monkeypatch):
-    import ctypes
+def test_default_eigensolvers(dev, monkeypatch):
+    """FP32 at p <= 64 takes the tridiagonal path, FP64 the Jacobi kernel."""
     import torch
     import ffdas_amd
-    from ffdas_amd._lib import lib
     monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
-    L = lib()
     x = _rand((64, 4096), torch.complex64, 17).to(dev)
-    L.ffdas_profile_reset()
-    L.ffdas_profile_enable(1)
-    try:
-        ffdas_amd.truncate_rank(x, 3)
-        torch.cuda.synchronize()
-    finally:
-        L.ffdas_profile_enable(0)
-    t, c = ctypes.c_double(), ctypes.c_int64()
-    L.ffdas_profile_query(b"rank_jacobi", ctypes.byref(t), ctypes.byref(c))
-    assert c.value == 1
+    _, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(x, 3))
+    assert calls == 1
+    xd = _rand((64, 4096), torch.complex128, 17).to(dev)
+    _, calls = _profiled_count("rank_jacobi", lambda: ffdas_amd.truncate_rank(xd, 3))
+    assert calls == 1
 
 
 @pytest.mark.parametrize("gram", ["lds", "wave"])
@@ -321,7 +314,7 @@ def test_zero_input(dev, dtype):
         assert torch.equal(got, torch.zeros_like(x)), (start, stop)
 
 
-@pytest.mark.parametrize("solver", ["jacobi", "rocsolver"])
+@pytest.mark.parametrize("solver", ["jacobi", "rocsolver", "tridiag"])
 def test_repeated_singular_values(dev, monkeypatch, solver):
     """Singular values with ties inside the removed and the kept sets (a gap only at the
     boundary): the truncation is still unique, x minus its top-3 part."""
@@ -361,31 +354,37 @@ def _profiled_count(name, fn):
     return out, c.value
 
 
-@pytest.mark.parametrize("shape, start, stop", [
-    ((65, 3000), 3, None),       # the smallest p past the Jacobi kernel, top 3 subtracted
-    ((200, 5000), 3, None),      # the clutter filter's usual form
-    ((200, 5000), 10, 150),      # keep mode: one range of 140 columns (two projection passes)
-    ((200, 5000), 10, 180),      # subtract mode with two ranges: 20 low and 10 high columns
-    ((224, 4000), 5, 218),       # the largest p of the path, two ranges
-    ((3000, 97), 2, 90),         # m > n: Gram of the columns
+@pytest.mark.parametrize("dtype, shape, start, stop", [
+    ("float32", (65, 3000), 3, None),      # the smallest p past the Jacobi kernel
+    ("float32", (200, 5000), 3, None),     # the clutter filter's usual form
+    ("float32", (200, 5000), 10, 150),     # keep mode: one range of 140 columns (two passes)
+    ("float32", (200, 5000), 10, 180),     # subtract mode, two ranges: 20 low, 10 high columns
+    ("float32", (224, 4000), 5, 218),      # the largest real p of the path, two ranges
+    ("float32", (3000, 97), 2, 90),        # m > n: Gram of the columns
+    ("complex64", (65, 3000), 3, None),
+    ("complex64", (120, 4000), 5, None),
+    ("complex64", (160, 3000), 10, 150),   # the largest complex p, subtract mode, two ranges
+    ("complex64", (140, 2000), 20, 80),    # keep mode
+    ("complex64", (2500, 99), 2, 90),      # m > n: the conjugated Gram of the columns
 ])
-def test_tridiagonal_path(dev, monkeypatch, shape, start, stop):
-    """Real FP32 with 64 < p <= 224 takes the tridiagonal path by default (one-workgroup
-    Householder reduction, the chosen eigenpairs by bisection and inverse iteration, the
-    back-transformation of those vectors alone) and gives the oracle's result."""
+def test_tridiagonal_path(dev, monkeypatch, dtype, shape, start, stop):
+    """FP32 with 64 < p <= 224 (real) or 160 (complex) takes the tridiagonal path by default
+    (one-workgroup Householder reduction, the chosen eigenpairs by bisection and inverse
+    iteration, the back-transformation of those vectors alone) and gives the oracle's
+    result."""
     import torch
     import ffdas_amd
     monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
-    x = _rand(shape, torch.float32, 31).to(dev)
+    x = _rand(shape, getattr(torch, dtype), 31).to(dev)
     got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(x, start, stop))
     assert calls == 1
     want = R.svd_filter(x.cpu().numpy(), start, stop)
-    assert rel_l2(got.cpu().numpy(), want) < TOL["float32"]
+    assert rel_l2(got.cpu().numpy(), want) < TOL[dtype]
 
 
 def test_tridiagonal_path_limits(dev, monkeypatch):
-    """p = 225 (past the register tile) and FP64 keep rocSOLVER's syevd; forcing the path
-    for a small p (2 and 33) gives the oracle's result too."""
+    """p = 225 real and 161 complex (past the register tile) and FP64 keep rocSOLVER; forcing
+    the path for a small p (2 and 33) gives the oracle's result too."""
     import torch
     import ffdas_amd
     monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
@@ -393,20 +392,28 @@ def test_tridiagonal_path_limits(dev, monkeypatch):
     got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(x, 4))
     assert calls == 0
     assert rel_l2(got.cpu().numpy(), R.svd_filter(x.cpu().numpy(), 4, None)) < TOL["float32"]
+    xc = _rand((161, 1500), torch.complex64, 32).to(dev)
+    got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(xc, 4))
+    assert calls == 0
+    assert rel_l2(got.cpu().numpy(), R.svd_filter(xc.cpu().numpy(), 4, None)) < TOL["complex64"]
     xd = _rand((100, 1500), torch.float64, 33).to(dev)
     got, calls = _profiled_count("rank_tridiag", lambda: ffdas_amd.truncate_rank(xd, 4))
     assert calls == 0
     assert rel_l2(got.cpu().numpy(), R.svd_filter(xd.cpu().numpy(), 4, None)) < TOL["float64"]
     monkeypatch.setenv("FFDAS_RANK_EIG", "tridiag")
-    for shape, start, stop in [((2, 50), 1, None), ((33, 500), 2, 20)]:
-        xs = _rand(shape, torch.float32, 34).to(dev)
+    for shape, start, stop, dt in [((2, 50), 1, None, torch.float32),
+                                   ((33, 500), 2, 20, torch.float32),
+                                   ((2, 50), 1, None, torch.complex64),
+                                   ((47, 900), 3, 40, torch.complex64)]:
+        xs = _rand(shape, dt, 34).to(dev)
         got, calls = _profiled_count("rank_tridiag",
                                      lambda: ffdas_amd.truncate_rank(xs, start, stop))
         assert calls == 1
         assert rel_l2(got.cpu().numpy(), R.svd_filter(xs.cpu().numpy(), start, stop)) < 1e-4
 
 
-def test_tridiagonal_path_ties_scaling_zero(dev, monkeypatch):
+@pytest.mark.parametrize("dtype", ["float32", "complex64"])
+def test_tridiagonal_path_ties_scaling_zero(dev, monkeypatch, dtype):
     """The tridiagonal path with tied singular values inside the removed and kept sets (a
     gap only at the boundary; the eigenvalues of the top cluster coincide), magnitudes far
     from 1 (scaling by 2^k is exact: the kernel prescales the Gram matrix by a power of two,
@@ -414,22 +421,30 @@ def test_tridiagonal_path_ties_scaling_zero(dev, monkeypatch):
     import torch
     import ffdas_amd
     monkeypatch.delenv("FFDAS_RANK_EIG", raising=False)
+    dt = getattr(torch, dtype)
+    wide = torch.complex128 if dt.is_complex else torch.float64
     m, n = 100, 4000
     g = torch.Generator().manual_seed(35)
-    u, _ = torch.linalg.qr(torch.randn(m, m, dtype=torch.float64, generator=g))
-    v, _ = torch.linalg.qr(torch.randn(n, m, dtype=torch.float64, generator=g))
-    s = torch.tensor([50.0] * 3 + [5.0] * 10 + [1.0] * (m - 13), dtype=torch.float64)
-    x = (u * s) @ v.T
-    want = (u[:, 3:] * s[3:]) @ v[:, 3:].T
-    got, calls = _profiled_count("rank_tridiag",
-                                 lambda: ffdas_amd.truncate_rank(x.to(torch.float32).to(dev), 3))
-    assert calls == 1
-    assert float((got.cpu().double() - want).norm() / want.norm()) < 1e-4
-    keep = ffdas_amd.truncate_rank(x.to(torch.float32).to(dev), 3, 13)   # the tied block
-    want_keep = (u[:, 3:13] * s[3:13]) @ v[:, 3:13].T
-    assert float((keep.cpu().double() - want_keep).norm() / want_keep.norm()) < 1e-4
 
-    xr = _rand((120, 3000), torch.float32, 36).to(dev)
+    def orth(rows, cols):
+        a = torch.randn(rows, cols, dtype=torch.float64, generator=g)
+        if dt.is_complex:
+            a = torch.complex(a, torch.randn(rows, cols, dtype=torch.float64, generator=g))
+        return torch.linalg.qr(a)[0]
+
+    u, v = orth(m, m), orth(n, m)
+    s = torch.tensor([50.0] * 3 + [5.0] * 10 + [1.0] * (m - 13), dtype=torch.float64).to(wide)
+    x = (u * s) @ v.conj().T
+    want = (u[:, 3:] * s[3:]) @ v[:, 3:].conj().T
+    got, calls = _profiled_count("rank_tridiag",
+                                 lambda: ffdas_amd.truncate_rank(x.to(dt).to(dev), 3))
+    assert calls == 1
+    assert float((got.cpu().to(wide) - want).norm() / want.norm()) < 1e-4
+    keep = ffdas_amd.truncate_rank(x.to(dt).to(dev), 3, 13)   # the tied block
+    want_keep = (u[:, 3:13] * s[3:13]) @ v[:, 3:13].conj().T
+    assert float((keep.cpu().to(wide) - want_keep).norm() / want_keep.norm()) < 1e-4
+
+    xr = _rand((120, 3000), dt, 36).to(dev)
     base = ffdas_amd.truncate_rank(xr, 3)
     for k in (30, -30):
         assert torch.equal(ffdas_amd.truncate_rank(xr * 2.0 ** k, 3), base * 2.0 ** k), k
@@ -438,6 +453,6 @@ def test_tridiagonal_path_ties_scaling_zero(dev, monkeypatch):
         got = ffdas_amd.truncate_rank(xe, 2, 90)
         assert rel_l2(got.cpu().numpy(), R.svd_filter(xe.cpu().numpy(), 2, 90)) < 1e-4
 
-    z = torch.zeros((90, 500), dtype=torch.float32, device=dev)
+    z = torch.zeros((90, 500), dtype=dt, device=dev)
     for start, stop in [(3, None), (0, 4), (1, 3), (5, 80)]:
         assert torch.equal(ffdas_amd.truncate_rank(z, start, stop), torch.zeros_like(z))
