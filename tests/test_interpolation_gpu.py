@@ -167,6 +167,36 @@ def test_affine_field_on_a_large_grid(dev):
         assert np.abs(r[k, ok] - (k + 1) * want[ok]).max() < 1e-5 * (k + 1)
 
 
+@pytest.mark.parametrize("mode", ["nearest", "linear"])
+def test_locate_by_bins_matches_per_point_scan(dev, monkeypatch, mode):
+    """Location by bins (points ordered by bin, one wave per bin testing the bin's candidates
+    once for all its points; the default) and the per-point scan (FFDAS_INTERP_LOCATE=point)
+    pick the same tetrahedron, the lowest passing id of the point's bin, so the interpolated
+    values agree bit for bit.  The sector grid is fine along range (about 4.7 cells per bin),
+    so bins hold more candidates than one wave of 64; queries are spread inside and outside,
+    on grid vertices (shared by several tetrahedra: the id order decides), and clustered in a
+    few bins (many points per bin)."""
+    import torch
+    import ffdas_amd
+    g = _sector(600, 20, 20, r0=0.3, r1=2.0)
+    rng = np.random.default_rng(44)
+    flat = g.reshape(-1, 3)
+    lo, hi = flat.min(0), flat.max(0)
+    centre = flat[rng.integers(0, flat.shape[0])]
+    cluster = centre + rng.uniform(-1.0, 1.0, (5000, 3)) * (hi - lo) / 128
+    q = np.concatenate([rng.uniform(lo - 0.1, hi + 0.1, (40000, 3)),
+                        flat[rng.integers(0, flat.shape[0], 5000)], cluster]).astype(np.float32)
+    x = _t(rng.standard_normal((2,) + g.shape[:3]).astype(np.float32), dev)
+    interp = ffdas_amd.Interpolator(_t(g, dev), mode=mode)
+    monkeypatch.setenv("FFDAS_INTERP_LOCATE", "point")
+    a = interp(x, _t(q, dev), fill=float("nan"))
+    monkeypatch.delenv("FFDAS_INTERP_LOCATE")
+    b = interp(x, _t(q, dev), fill=float("nan"))
+    assert torch.equal(torch.isnan(a), torch.isnan(b))
+    assert torch.equal(torch.nan_to_num(a), torch.nan_to_num(b))
+    assert 0.2 < float((~torch.isnan(b[0])).float().mean()) < 0.95
+
+
 def test_abi_error_codes(dev):
     import torch
     import ffdas_amd
