@@ -1216,16 +1216,6 @@ __device__ __forceinline__ float wave_total(float v) {
 
 __host__ __device__ constexpr int packed_column(int p, int k) { return k * p - k * (k + 1) / 2; }
 
-// Register tile of the tridiagonalisation: thread (ti, tj) = (tid / 32, tid % 32) of the 32 TR
-// holds A[ti + TR a][tj + 32 b] for a < NBR = 32 NBC / TR, b < NBC (real parts in re, imaginary
-// parts in im for a Hermitian A).  The tile covers P = 32 NBC rows and columns.
-template <int TR, int NBC, bool CPLX>
-struct TriTile {
-    static constexpr int NBR = 32 * NBC / TR;
-    float re[NBR][NBC];
-    float im[CPLX ? NBR : 1][CPLX ? NBC : 1];
-};
-
 // u_j = 1 at j = k + 1, (x_j * s) for k + 1 < j < p, 0 elsewhere (s complex for a Hermitian A).
 template <bool CPLX>
 __device__ __forceinline__ void tri_u(int j, int k, int p, float xr, float xi, float sr, float si,
@@ -1237,176 +1227,72 @@ __device__ __forceinline__ void tri_u(int j, int k, int p, float xr, float xi, f
     ui = live ? vi : 0.f;
 }
 
-// One reduction step's matrix-vector product on the live blocks (column blocks from B0, row
-// blocks from 32 B0 / TR; rows and columns below 32 B0 are done): ps = tau A' u, written by
-// the lanes that hold the half-wave sums (tj = 31) for every row of the live blocks (rows
-// <= k or >= p are never read as such).
-template <int TR, int NBC, bool CPLX, int B0>
-__device__ __forceinline__ void tri_matvec(const TriTile<TR, NBC, CPLX> &A, const float *xr,
-                                           const float *xi, float sr, float si, float tr,
-                                           float tim, int k, int p, int ti, int tj, float *pr,
-                                           float *pim) {
-    constexpr int NBR = TriTile<TR, NBC, CPLX>::NBR, IA0 = 32 * B0 / TR;
-    float ur[NBC], ui[NBC];
-#pragma unroll
-    for (int jb = B0; jb < NBC; ++jb) {
-        const int j = tj + 32 * jb;
-        tri_u<CPLX>(j, k, p, xr[j], CPLX ? xi[j] : 0.f, sr, si, ur[jb], ui[jb]);
-    }
-    float accr[NBR], acci[NBR];
-#pragma unroll
-    for (int ia = IA0; ia < NBR; ++ia) {
-        float sre = 0.f, sim = 0.f;
-#pragma unroll
-        for (int jb = B0; jb < NBC; ++jb) {
-            sre = fmaf(A.re[ia][jb], ur[jb], sre);
-            if constexpr (CPLX) {
-                sre = fmaf(-A.im[ia][jb], ui[jb], sre);
-                sim = fmaf(A.re[ia][jb], ui[jb], fmaf(A.im[ia][jb], ur[jb], sim));
-            }
-        }
-        accr[ia] = half_wave_sum_hi(sre);
-        if constexpr (CPLX) acci[ia] = half_wave_sum_hi(sim);
-    }
-    if (tj == 31) {
-#pragma unroll
-        for (int ia = IA0; ia < NBR; ++ia) {
-            const int i = ti + TR * ia;
-            if constexpr (CPLX) {
-                pr[i] = tr * accr[ia] - tim * acci[ia];
-                pim[i] = tr * acci[ia] + tim * accr[ia];
-            } else {
-                pr[i] = tr * accr[ia];
-            }
-        }
-    }
+// Sum over each group of G = 2 or 4 consecutive lanes, in every lane of the group.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    v += dppf<0xb1>(v);                         // quad_perm [1, 0, 3, 2]
+    if constexpr (G == 4) v += dppf<0x4e>(v);   // quad_perm [2, 3, 0, 1]
+    return v;
 }
-
-// A' -= u w^H + w u^H on the live blocks, w = ps + K u (u and w are 0 outside rows k+1 .. p-1).
-template <int TR, int NBC, bool CPLX, int B0>
-__device__ __forceinline__ void tri_update(TriTile<TR, NBC, CPLX> &A, const float *xr,
-                                           const float *xi, const float *pr, const float *pim,
-                                           float sr, float si, float Kr, float Ki, int k, int p,
-                                           int ti, int tj) {
-    constexpr int NBR = TriTile<TR, NBC, CPLX>::NBR, IA0 = 32 * B0 / TR;
-    float ujr[NBC], uji[NBC], wjr[NBC], wji[NBC];
-#pragma unroll
-    for (int jb = B0; jb < NBC; ++jb) {
-        const int j = tj + 32 * jb;
-        tri_u<CPLX>(j, k, p, xr[j], CPLX ? xi[j] : 0.f, sr, si, ujr[jb], uji[jb]);
-        const bool lj = j > k && j < p;
-        const float p_r = pr[j], p_i = CPLX ? pim[j] : 0.f;
-        wjr[jb] = lj ? p_r + (CPLX ? Kr * ujr[jb] - Ki * uji[jb] : Kr * ujr[jb]) : 0.f;
-        wji[jb] = lj && CPLX ? p_i + Kr * uji[jb] + Ki * ujr[jb] : 0.f;
-    }
-#pragma unroll
-    for (int ia = IA0; ia < NBR; ++ia) {
-        const int i = ti + TR * ia;
-        float uir, uii;
-        tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, uir, uii);
-        const bool li = i > k && i < p;
-        const float p_r = pr[i], p_i = CPLX ? pim[i] : 0.f;
-        const float wir = li ? p_r + (CPLX ? Kr * uir - Ki * uii : Kr * uir) : 0.f;
-        const float wii = li && CPLX ? p_i + Kr * uii + Ki * uir : 0.f;
-#pragma unroll
-        for (int jb = B0; jb < NBC; ++jb) {
-            // u_i conj(w_j) + w_i conj(u_j)
-            float r = fmaf(uir, wjr[jb], wir * ujr[jb]);
-            if constexpr (CPLX) {
-                r = fmaf(uii, wji[jb], fmaf(wii, uji[jb], r));
-                const float m = fmaf(uii, wjr[jb], fmaf(-uir, wji[jb],
-                                                        fmaf(wii, ujr[jb], -wir * uji[jb])));
-                A.im[ia][jb] -= m;
-            }
-            A.re[ia][jb] -= r;
-        }
-    }
-}
-
-// Column B of the tile into (cr, ci): all NBR entries of the owning lanes, rows at or above the
-// column index included (the reduction reads the rows below it, and the diagonal for D).
-template <int TR, int NBC, bool CPLX, int B>
-__device__ __forceinline__ void tri_put_column(const TriTile<TR, NBC, CPLX> &A, int ti, float *cr,
-                                               float *ci) {
-#pragma unroll
-    for (int ia = 0; ia < TriTile<TR, NBC, CPLX>::NBR; ++ia) {
-        cr[ti + TR * ia] = A.re[ia][B];
-        if constexpr (CPLX) ci[ti + TR * ia] = A.im[ia][B];
-    }
-}
-
-// A uniform switch over the block index b (0 .. 6) calling CALL(B) for B < NBC: a straight-line
-// body per live-block count, no per-element branches.
-#define FFDAS_TRIDIAG_SWITCH(b, CALL)                                                          \
-    switch (b) {                                                                               \
-        case 0: CALL(0); break;                                                                \
-        case 1: if constexpr (NBC > 1) CALL(1); break;                                         \
-        case 2: if constexpr (NBC > 2) CALL(2); break;                                         \
-        case 3: if constexpr (NBC > 3) CALL(3); break;                                         \
-        case 4: if constexpr (NBC > 4) CALL(4); break;                                         \
-        case 5: if constexpr (NBC > 5) CALL(5); break;                                         \
-        default: if constexpr (NBC > 6) CALL(6); break;                                        \
-    }
 
 // T = Q^H A Q for the p x p Hermitian (or real symmetric) A, column-major with both triangles,
-// by Householder reflections taken from the columns (LAPACK ssytd2 / chetd2, lower form):
-// reflection k, H_k = I - tau_k u_k u_k^H with u_k[k + 1] = 1 and u_k[i] = 0 for i <= k, turns
-// column k below the diagonal into (beta_k, 0, ..., 0) with beta_k real, and
-// Q = H_0 H_1 ... H_{m-1} (m = p - 2 real; m = p - 1 complex, the last reflection making
-// A[p-1][p-2] real).  T is real symmetric either way.
-// The tile lives in registers (TriTile); a wave issues one instruction per four cycles and every
-// wave repeats the per-step scalar work, so fewer, fuller lanes (TR = 8 or 16 rows of threads)
-// beat 1024 threads.  A step is straight-line code: one uniform dispatch on the first live
-// 32-column block picks a fully unrolled body over the live blocks, and rows or columns outside
-// k+1 .. p-1 drop out through zero u and w rather than branches.  A step takes two barriers:
-// the owners of column k have put it in LDS (double-buffered by step parity), every wave forms
-// the same reflector from it (its own DPP reductions, so no third barrier), the product A u
-// goes through LDS, and after the rank-2 update the owners of column k + 1 store it.  The
-// reflectors, D, E and tau collect in LDS (dynamic: the packed reflectors take p (p - 1) / 2
-// elements) and go to memory once at the end, so no barrier waits on stores.
+// by Householder reflections (LAPACK ssytd2 / chetd2, lower form): reflection k,
+// H_k = I - tau_k u_k u_k^H with u_k[k + 1] = 1 and u_k[i] = 0 for i <= k, turns column k
+// below the diagonal into (beta_k, 0, ..., 0) with beta_k real, and Q = H_0 H_1 ... H_{m-1}
+// (m = p - 2 real; m = p - 1 complex, the last reflection making A[p-1][p-2] real).  T is real
+// symmetric either way.
+// Layout: the G = 4 lanes G i .. G i + G - 1 hold row i of A (P = G NC rows), lane g of them
+// the NC columns g NC .. g NC + NC - 1 in registers, so the product A u needs only a sum over
+// those lanes.
+// Every wave keeps its own copy of u and w in LDS (written and read by the wave alone, so no
+// barrier), zero outside k+1 .. p-1, which keeps the product and the rank-2 update free of
+// masks.  Column k below the diagonal is row k conjugated (A stays Hermitian): the lanes of row
+// k + 1 store it after the update, double-buffered by step parity.  Two barriers a step: before
+// the reflector (every wave forms the same one from the stored row with its own DPP
+// reductions) and after the product.  The reflectors, D, E and tau collect in LDS (dynamic:
+// the packed reflectors take p (p - 1) / 2 elements) and go to memory once at the end.
 // A is first scaled by the power of two that brings its largest magnitude into [0.5, 1)
 // (exact, and eigenvectors do not change), so squares neither overflow nor underflow for Gram
 // matrices of data far from unit magnitude.
 // Out: D (p) and E (p - 1) of the scaled T, tau (m; complex as pairs) and the reflectors packed
 // by column (u_k[k + 1 .. p - 1] from element k p - k (k + 1) / 2 of Upk).  stat[0..7] (status
 // words of the kernels that follow) are cleared.
-template <int TR, int NBC, bool CPLX>
-__global__ __launch_bounds__(32 * TR) void rank_tridiag_kernel(const float *__restrict__ Ain,
+template <int NC, int G, bool CPLX>
+__global__ __launch_bounds__(G * G * NC) void rank_tridiag_kernel(const float *__restrict__ Ain,
                                                                int p, float *__restrict__ D,
                                                                float *__restrict__ E,
                                                                float *__restrict__ Upk,
                                                                float *__restrict__ tau,
                                                                int *__restrict__ stat) {
-    constexpr int NT = 32 * TR, P = 32 * NBC, NW = NT / 64, C = CPLX ? 2 : 1;
+    constexpr int P = G * NC, NT = G * P, NW = NT / 64, C = CPLX ? 2 : 1;
     constexpr int NS = (P + 63) / 64;   // row slots per lane in the wave-wide loops
-    constexpr int NBR = TriTile<TR, NBC, CPLX>::NBR;
-    __shared__ float xs[2][C][64 * NS], ps[C][64 * NS];
+    constexpr int L = 64 * NS;
+    __shared__ float xs[2][C][L], ps[C][L];
+    __shared__ __align__(16) float uw[NW][2][C][L];   // each wave's u and w
     __shared__ float red[NW];
     __shared__ float Ds[P], Es[P], Ts[C * P];
     extern __shared__ float Us[];   // p (p - 1) / 2 packed reflector elements
-    const int tid = threadIdx.x, ti = tid >> 5, tj = tid & 31, lane = tid & 63;
+    const int tid = threadIdx.x, row = tid / G, j0 = (tid % G) * NC, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid < 8) stat[tid] = 0;
-    for (int i = tid; i < 64 * NS; i += NT) {
+    for (int i = tid; i < L; i += NT) {
 #pragma unroll
         for (int c = 0; c < C; ++c) xs[0][c][i] = xs[1][c][i] = ps[c][i] = 0.f;
     }
-    TriTile<TR, NBC, CPLX> A;
+    float ar[NC], ai[CPLX ? NC : 1];
     float amax = 0.f;
 #pragma unroll
-    for (int ia = 0; ia < NBR; ++ia)
-#pragma unroll
-        for (int jb = 0; jb < NBC; ++jb) {
-            const int i = ti + TR * ia, j = tj + 32 * jb;
-            const bool in = i < p && j < p;
-            const size_t e = (size_t)j * p + i;
-            A.re[ia][jb] = in ? Ain[C * e] : 0.f;
-            amax = fmaxf(amax, fabsf(A.re[ia][jb]));
-            if constexpr (CPLX) {
-                A.im[ia][jb] = in ? Ain[2 * e + 1] : 0.f;
-                amax = fmaxf(amax, fabsf(A.im[ia][jb]));
-            }
+    for (int c = 0; c < NC; ++c) {
+        const int j = j0 + c;
+        const bool in = row < p && j < p;
+        const size_t e = (size_t)j * p + row;
+        ar[c] = in ? Ain[C * e] : 0.f;
+        amax = fmaxf(amax, fabsf(ar[c]));
+        if constexpr (CPLX) {
+            ai[c] = in ? Ain[2 * e + 1] : 0.f;
+            amax = fmaxf(amax, fabsf(ai[c]));
         }
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
     if (lane == 0) red[wave] = amax;
@@ -1418,24 +1304,23 @@ __global__ __launch_bounds__(32 * TR) void rank_tridiag_kernel(const float *__re
         int e;
         (void)frexpf(amax, &e);
 #pragma unroll
-        for (int ia = 0; ia < NBR; ++ia)
+        for (int c = 0; c < NC; ++c) {
+            ar[c] = ldexpf(ar[c], -e);
+            if constexpr (CPLX) ai[c] = ldexpf(ai[c], -e);
+        }
+    }
+    // row r, conjugated (column r), into xs[b] by the lanes that hold it
+    const auto put_row = [&](int r, int b) {
+        if (row != r) return;
 #pragma unroll
-            for (int jb = 0; jb < NBC; ++jb) {
-                A.re[ia][jb] = ldexpf(A.re[ia][jb], -e);
-                if constexpr (CPLX) A.im[ia][jb] = ldexpf(A.im[ia][jb], -e);
-            }
-    }
-#define FFDAS_PUT(B) tri_put_column<TR, NBC, CPLX, (B < NBC ? B : 0)>(A, ti, cr, ci)
-#define FFDAS_MATVEC(B)                                                                        \
-    tri_matvec<TR, NBC, CPLX, (B < NBC ? B : 0)>(A, xr, xi, sr, si, tr, tim, k, p, ti, tj,     \
-                                                  ps[0], ps[C - 1])
-#define FFDAS_UPDATE(B)                                                                        \
-    tri_update<TR, NBC, CPLX, (B < NBC ? B : 0)>(A, xr, xi, ps[0], ps[C - 1], sr, si, Kr, Ki,  \
-                                                  k, p, ti, tj)
-    if (tj == 0) {
-        float *cr = xs[0][0], *ci = xs[0][C - 1];
-        FFDAS_PUT(0);
-    }
+        for (int c = 0; c < NC; ++c) {
+            xs[b][0][j0 + c] = ar[c];
+            if constexpr (CPLX) xs[b][1][j0 + c] = -ai[c];
+        }
+    };
+    float *ur = uw[wave][0][0], *ui = uw[wave][0][C - 1];
+    float *wr = uw[wave][1][0], *wi = uw[wave][1][C - 1];
+    put_row(0, 0);
     const int steps = CPLX ? p - 1 : p - 2;
     for (int k = 0; k < steps; ++k) {
         __syncthreads();
@@ -1465,20 +1350,26 @@ __global__ __launch_bounds__(32 * TR) void rank_tridiag_kernel(const float *__re
             const float rb = 1.f / b;
             tr = (b - alr) * rb;
             tim = -ali * rb;
-            // 1 / (alpha - beta)
-            const float dr = alr - b;
+            const float dr = alr - b;   // 1 / (alpha - beta)
             const float rd = 1.f / fmaf(dr, dr, ali * ali);
             sr = dr * rd;
             si = -ali * rd;
             beta = b;
         }
+        // this wave's u
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+            const int i = lane + 64 * sl;
+            float vr, vi;
+            tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, vr, vi);
+            ur[i] = vr;
+            if constexpr (CPLX) ui[i] = vi;
+        }
         if (wave == 0) {
             const int o = packed_column(p, k) - (k + 1);
             for (int i = k + 1 + lane; i < p; i += 64) {
-                float ur, ui;
-                tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, ur, ui);
-                Us[C * (o + i)] = ur;
-                if constexpr (CPLX) Us[2 * (o + i) + 1] = ui;
+                Us[C * (o + i)] = ur[i];
+                if constexpr (CPLX) Us[2 * (o + i) + 1] = ui[i];
             }
             if (lane == 0) {
                 Ds[k] = xr[k];
@@ -1488,49 +1379,125 @@ __global__ __launch_bounds__(32 * TR) void rank_tridiag_kernel(const float *__re
             }
         }
         if (tr != 0.f || tim != 0.f) {   // the same values in every wave
-            const int b0 = (k + 1) >> 5;   // first block holding a column > k
-            FFDAS_TRIDIAG_SWITCH(b0, FFDAS_MATVEC)
+            // ps = tau A' u: the quad of row i sums its four column ranges (16-byte reads of
+            // this wave's u: j0 and the arrays are 32-byte aligned)
+            float s_r = 0.f, s_i = 0.f;
+            const float4 *ur4 = reinterpret_cast<const float4 *>(ur + j0);
+            const float4 *ui4 = reinterpret_cast<const float4 *>(ui + j0);
+#pragma unroll
+            for (int c4 = 0; c4 < NC / 4; ++c4) {
+                const float4 vr4 = ur4[c4];
+                const float ucr[4] = {vr4.x, vr4.y, vr4.z, vr4.w};
+                float uci[4] = {0.f, 0.f, 0.f, 0.f};
+                if constexpr (CPLX) {
+                    const float4 vi4 = ui4[c4];
+                    uci[0] = vi4.x;
+                    uci[1] = vi4.y;
+                    uci[2] = vi4.z;
+                    uci[3] = vi4.w;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 4 * c4 + q;
+                    s_r = fmaf(ar[c], ucr[q], s_r);
+                    if constexpr (CPLX) {
+                        s_r = fmaf(-ai[c], uci[q], s_r);
+                        s_i = fmaf(ar[c], uci[q], fmaf(ai[c], ucr[q], s_i));
+                    }
+                }
+            }
+            s_r = group_sum<G>(s_r);
+            if constexpr (CPLX) s_i = group_sum<G>(s_i);
+            if (j0 == 0) {
+                if constexpr (CPLX) {
+                    ps[0][row] = tr * s_r - tim * s_i;
+                    ps[1][row] = tr * s_i + tim * s_r;
+                } else {
+                    ps[0][row] = tr * s_r;
+                }
+            }
             __syncthreads();
-            // K = -(tau / 2)(ps^H u)
+            // K = -(tau / 2)(ps^H u), then this wave's w = ps + K u (0 outside k+1 .. p-1)
             float dr = 0.f, di = 0.f;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 const int i = lane + 64 * sl;
                 const bool live = i > k && i < p;
-                float ur, ui;
-                tri_u<CPLX>(i, k, p, xr[i], CPLX ? xi[i] : 0.f, sr, si, ur, ui);
                 const float p_r = live ? ps[0][i] : 0.f;
-                dr = fmaf(p_r, ur, dr);
+                const float u_r = ur[i];
+                dr = fmaf(p_r, u_r, dr);
                 if constexpr (CPLX) {
                     const float p_i = live ? ps[1][i] : 0.f;
-                    dr = fmaf(p_i, ui, dr);
-                    di = fmaf(p_r, ui, fmaf(-p_i, ur, di));
+                    const float u_i = ui[i];
+                    dr = fmaf(p_i, u_i, dr);
+                    di = fmaf(p_r, u_i, fmaf(-p_i, u_r, di));
                 }
             }
             dr = wave_total(dr);
             if constexpr (CPLX) di = wave_total(di);
             const float Kr = -0.5f * (tr * dr - tim * di);
             const float Ki = -0.5f * (tr * di + tim * dr);
-            FFDAS_TRIDIAG_SWITCH(b0, FFDAS_UPDATE)
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                const int i = lane + 64 * sl;
+                const bool live = i > k && i < p;
+                const float u_r = ur[i];
+                if constexpr (CPLX) {
+                    const float u_i = ui[i];
+                    wr[i] = live ? ps[0][i] + Kr * u_r - Ki * u_i : 0.f;
+                    wi[i] = live ? ps[1][i] + Kr * u_i + Ki * u_r : 0.f;
+                } else {
+                    wr[i] = live ? ps[0][i] + Kr * u_r : 0.f;
+                }
+            }
+            // row i: A[i][j] -= u_i conj(w_j) + w_i conj(u_j)
+            const float uir = ur[row], wir = wr[row];
+            const float uii = CPLX ? ui[row] : 0.f, wii = CPLX ? wi[row] : 0.f;
+            const float4 *wr4 = reinterpret_cast<const float4 *>(wr + j0);
+            const float4 *wi4 = reinterpret_cast<const float4 *>(wi + j0);
+#pragma unroll
+            for (int c4 = 0; c4 < NC / 4; ++c4) {
+                const float4 a4 = ur4[c4], b4 = wr4[c4];
+                const float ujr[4] = {a4.x, a4.y, a4.z, a4.w}, wjr[4] = {b4.x, b4.y, b4.z, b4.w};
+                float uji[4] = {0.f, 0.f, 0.f, 0.f}, wji[4] = {0.f, 0.f, 0.f, 0.f};
+                if constexpr (CPLX) {
+                    const float4 c4i = ui4[c4], d4 = wi4[c4];
+                    uji[0] = c4i.x;
+                    uji[1] = c4i.y;
+                    uji[2] = c4i.z;
+                    uji[3] = c4i.w;
+                    wji[0] = d4.x;
+                    wji[1] = d4.y;
+                    wji[2] = d4.z;
+                    wji[3] = d4.w;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 4 * c4 + q;
+                    float r = fmaf(uir, wjr[q], wir * ujr[q]);
+                    if constexpr (CPLX) {
+                        r = fmaf(uii, wji[q], fmaf(wii, uji[q], r));
+                        const float m = fmaf(uii, wjr[q],
+                                             fmaf(-uir, wji[q], fmaf(wii, ujr[q], -wir * uji[q])));
+                        ai[c] -= m;
+                    }
+                    ar[c] -= r;
+                }
+            }
         }
-        if (tj == ((k + 1) & 31)) {
-            float *cr = xs[(k + 1) & 1][0], *ci = xs[(k + 1) & 1][C - 1];
-            FFDAS_TRIDIAG_SWITCH((k + 1) >> 5, FFDAS_PUT)
+        put_row(k + 1, (k + 1) & 1);
+    }
+    // the last rows: D[p-1] (and D[p-2], E[p-2] when real, whose last step is empty)
+    __syncthreads();
+    if (!CPLX) put_row(p - 1, (p - 1) & 1);
+    __syncthreads();
+    if (tid == 0) {
+        Ds[p - 1] = xs[(p - 1) & 1][0][p - 1];
+        if (!CPLX) {
+            Ds[p - 2] = xs[(p - 2) & 1][0][p - 2];
+            Es[p - 2] = xs[(p - 2) & 1][0][p - 1];
         }
     }
-#undef FFDAS_PUT
-#undef FFDAS_MATVEC
-#undef FFDAS_UPDATE
-    // the trailing entries: D[p-1] (and D[p-2], E[p-2] when real, whose last step is empty)
-#pragma unroll
-    for (int ia = 0; ia < NBR; ++ia)
-#pragma unroll
-        for (int jb = 0; jb < NBC; ++jb) {
-            const int i = ti + TR * ia, j = tj + 32 * jb;
-            if (i >= p || j >= p || i < steps) continue;
-            if (i == j) Ds[i] = A.re[ia][jb];
-            if (!CPLX && i == p - 1 && j == p - 2) Es[p - 2] = A.re[ia][jb];
-        }
     __syncthreads();
     const int npk = C * (p * (p - 1) / 2);
     for (int e = tid; e < npk; e += NT) Upk[e] = Us[e];
@@ -2097,15 +2064,13 @@ size_t rank_tridiag_lds(int p, bool cplx) {
     return (size_t)p * (p - 1) / 2 * (cplx ? 8 : 4);
 }
 
-// Real p <= 224, complex p <= 160: the tile of A fits the registers of its lanes (TR = 8 rows of
-// threads up to 192 real / 128 complex columns, 16 above; a 7-block real tile at TR = 8, or a
-// 6-block complex one at TR = 16, spills); the tridiagonalisation's static LDS stays under
-// 12 KiB.
+// Real p <= 224, complex p <= 160: a row of A fits the registers of its four lanes (56 real
+// or 40 complex columns each); the tridiagonalisation's static LDS stays under 44 KiB.
 int tridiag_max_p(bool cplx) { return cplx ? 160 : 224; }
 
 bool rank_tridiag_fits(const RankPlan &pl, int lds_bytes) {
     return !pl.dbl && pl.p >= 2 && pl.p <= tridiag_max_p(pl.cplx) &&
-           rank_tridiag_lds(pl.p, pl.cplx) + 12288 <= (size_t)lds_bytes;
+           rank_tridiag_lds(pl.p, pl.cplx) + 45056 <= (size_t)lds_bytes;
 }
 
 ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_bytes,
@@ -2114,32 +2079,35 @@ ffdas_error_t launch_rank_tridiag(hipStream_t st, const RankPlan &pl, int lds_by
     if (!rank_tridiag_fits(pl, lds_bytes)) return FFDAS_ERROR_INVALID_ARGUMENT;
     ProfScope ps(st, "rank_tridiag");
     const size_t lds = rank_tridiag_lds(pl.p, pl.cplx);
-#define FFDAS_RANK_TRIDIAG(TR, NBC, CPLX)                                                      \
+#define FFDAS_RANK_TRIDIAG(NC, G, CPLX)                                                        \
     do {                                                                                       \
-        (void)hipFuncSetAttribute((const void *)rank_tridiag_kernel<TR, NBC, CPLX>,            \
+        (void)hipFuncSetAttribute((const void *)rank_tridiag_kernel<NC, G, CPLX>,              \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
-        hipLaunchKernelGGL((rank_tridiag_kernel<TR, NBC, CPLX>), dim3(1), dim3(32 * TR), lds,  \
+        hipLaunchKernelGGL((rank_tridiag_kernel<NC, G, CPLX>), dim3(1), dim3(G * G * NC), lds, \
                            st, (const float *)H, pl.p, (float *)D, (float *)E, (float *)U,     \
                            (float *)tau, stat);                                                \
     } while (0)
-    const int nbc = (pl.p + 31) / 32;
+    // P rows >= p, 4 lanes a row of NC (a multiple of 8) columns each.  Two lanes a row (half
+    // the waves, so half the per-step work every wave repeats) measured slower at p = 200 real
+    // (0.53 against 0.50 ms): the step is bound by its chain of dependent phases, not by issue.
+    const int nc8 = (pl.p + 31) / 32;
     if (pl.cplx) {
-        switch (nbc) {
-            case 1: FFDAS_RANK_TRIDIAG(8, 1, true); break;
-            case 2: FFDAS_RANK_TRIDIAG(8, 2, true); break;
-            case 3: FFDAS_RANK_TRIDIAG(8, 3, true); break;
-            case 4: FFDAS_RANK_TRIDIAG(8, 4, true); break;
-            default: FFDAS_RANK_TRIDIAG(16, 5, true); break;
+        switch (nc8) {
+            case 1: FFDAS_RANK_TRIDIAG(8, 4, true); break;
+            case 2: FFDAS_RANK_TRIDIAG(16, 4, true); break;
+            case 3: FFDAS_RANK_TRIDIAG(24, 4, true); break;
+            case 4: FFDAS_RANK_TRIDIAG(32, 4, true); break;
+            default: FFDAS_RANK_TRIDIAG(40, 4, true); break;
         }
     } else {
-        switch (nbc) {
-            case 1: FFDAS_RANK_TRIDIAG(8, 1, false); break;
-            case 2: FFDAS_RANK_TRIDIAG(8, 2, false); break;
-            case 3: FFDAS_RANK_TRIDIAG(8, 3, false); break;
-            case 4: FFDAS_RANK_TRIDIAG(8, 4, false); break;
-            case 5: FFDAS_RANK_TRIDIAG(8, 5, false); break;
-            case 6: FFDAS_RANK_TRIDIAG(8, 6, false); break;
-            default: FFDAS_RANK_TRIDIAG(16, 7, false); break;
+        switch (nc8) {
+            case 1: FFDAS_RANK_TRIDIAG(8, 4, false); break;
+            case 2: FFDAS_RANK_TRIDIAG(16, 4, false); break;
+            case 3: FFDAS_RANK_TRIDIAG(24, 4, false); break;
+            case 4: FFDAS_RANK_TRIDIAG(32, 4, false); break;
+            case 5: FFDAS_RANK_TRIDIAG(40, 4, false); break;
+            case 6: FFDAS_RANK_TRIDIAG(48, 4, false); break;
+            default: FFDAS_RANK_TRIDIAG(56, 4, false); break;
         }
     }
 #undef FFDAS_RANK_TRIDIAG
