@@ -1,7 +1,8 @@
 """greens on the GPU against the FP64 oracle (oracle/greens.py): the reference's own cases
 (bindings/python/tests/test_greens.py) at its tolerance (rtol 1e-3, atol 1e-4; the kernel
 computes in FP32, its error is about 1e-6 of the sum's scale), all three event tilings (1
-and 4 events per thread on the VALU, 16-event tiles on the MFMA from 8 events up), partial
+event per thread on the VALU, 4 events per thread on the VALU for 2-4 events, 16-event tiles
+on the MFMA from 5 events up, each at its boundary: 1, 2, 4, 5 events), partial
 tiles of targets, sources, frequencies and events, FP16 input, a 2-D target grid and the
 C-ABI error codes."""
 import ctypes
@@ -59,7 +60,8 @@ def test_reference_zero_input(dev):
 
 @pytest.mark.parametrize("B, nin, nk, nout", [(1, 37, 13, 300), (4, 64, 16, 257),
                                               (7, 20, 9, 70), (2, 1, 1, 1), (8, 37, 13, 300),
-                                              (20, 66, 17, 131), (16, 3, 8, 16)])
+                                              (20, 66, 17, 131), (16, 3, 8, 16),
+                                              (3, 29, 11, 97), (5, 33, 12, 150)])
 def test_parity_with_the_oracle(dev, B, nin, nk, nout):
     import ffdas_amd
     rng = np.random.default_rng(B * 1000 + nin)
