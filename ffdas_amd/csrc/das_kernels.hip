@@ -913,12 +913,28 @@ __global__ __launch_bounds__(256) void transpose_gout_kernel(
             pos[3 * (int64_t)q] = g.dstpos[3 * (int64_t)n];
             pos[3 * (int64_t)q + 1] = g.dstpos[3 * (int64_t)n + 1];
             pos[3 * (int64_t)q + 2] = g.dstpos[3 * (int64_t)n + 2];
-            for (int j = 0; j < g.K; j++) {
-                ofs[(int64_t)j * N + q] = g.offsets[(int64_t)j * N + n];
-                const float wv = g.weights[(int64_t)j * N + n];
-                wts[(int64_t)j * N + q] = wv;
-                wmax = fmaxf(wmax, fabsf(wv));
-                if (g.sparse) sp[(int64_t)j * N + q] = g.sparse[(int64_t)j * N + n];
+            // eight rows loaded before any is stored: a store may alias the next load as far as
+            // the compiler knows, so a row-by-row loop waited a memory round trip per row
+            for (int j0 = 0; j0 < g.K; j0 += 8) {
+                float ov[8], wv[8];
+                int sv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = j0 + u;
+                    const bool in = j < g.K;
+                    ov[u] = in ? g.offsets[(int64_t)j * N + n] : 0.0f;
+                    wv[u] = in ? g.weights[(int64_t)j * N + n] : 0.0f;
+                    sv[u] = in && g.sparse ? g.sparse[(int64_t)j * N + n] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = j0 + u;
+                    if (j >= g.K) break;
+                    ofs[(int64_t)j * N + q] = ov[u];
+                    wts[(int64_t)j * N + q] = wv[u];
+                    wmax = fmaxf(wmax, fabsf(wv[u]));
+                    if (g.sparse) sp[(int64_t)j * N + q] = sv[u];
+                }
             }
         }
     }
