@@ -280,6 +280,10 @@ ffdas_error_t ffdas_destroy(ffdas_handle_t h) {
     if (!h) return FFDAS_ERROR_INVALID_HANDLE;
     ffdas_error_t err = FFDAS_SUCCESS;
     ffdas::release_rank_solver(*h);
+    if (h->status_host) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipHostFree(h->status_host);
+    }
     if (h->side) {
         (void)hipStreamSynchronize(h->side);
         (void)hipStreamDestroy(h->side);

@@ -69,6 +69,9 @@ struct ffdas_context {
 
     // rocBLAS handle used by the rank-truncation eigensolver (created on first use)
     void *blas = nullptr;
+    // pinned host word into which truncate_rank reads its solver status (allocated on first
+    // use: a copy into pageable memory goes through a staging buffer)
+    int *status_host = nullptr;
 
     // A second stream (default priority, non-blocking) on which ffdas_das_backward runs the
     // grad_x scatter beside the parameter adjoint, and the events that fork it from and join

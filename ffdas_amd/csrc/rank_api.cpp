@@ -70,11 +70,16 @@ constexpr int kJacobiOverTridiagP = 16;   // FP32 p at or below which Jacobi sta
 // The solver's status word, read on the host: the one synchronisation of the call, as the
 // reference's (truncate_rank_impl.cuh:82-87).
 ffdas_error_t check_info(ffdas_context &h, const int *info, ffdas_error_t fail) {
-    int hinfo = 0;
-    if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, h.stream) != hipSuccess)
+    if (!h.status_host && hipHostMalloc((void **)&h.status_host, sizeof(int)) != hipSuccess) {
+        h.status_host = nullptr;
+        return FFDAS_ERROR_ALLOCATION_FAILED;
+    }
+    *h.status_host = -1;
+    if (hipMemcpyAsync(h.status_host, info, sizeof(int), hipMemcpyDeviceToHost, h.stream) !=
+        hipSuccess)
         return FFDAS_ERROR_CUDA_MEMCPY_FAILED;
     if (hipStreamSynchronize(h.stream) != hipSuccess) return FFDAS_ERROR_CUDA_SYNC_FAILED;
-    return hinfo == 0 ? FFDAS_SUCCESS : fail;
+    return *h.status_host == 0 ? FFDAS_SUCCESS : fail;
 }
 
 enum class EigPath { jacobi, tridiag, rocsolver };
