@@ -1170,7 +1170,9 @@ __device__ __forceinline__ unsigned block_max_u32(unsigned v, unsigned *red) {
     return r;
 }
 
-// grid.x = C*T rows (m = row / T, s = row % T), grid.y = frame chunks of FS frames.
+// grid.x = rows row0 .. row0 + gridDim.x - 1 of the C*T rows (m = row / T, s = row % T),
+// grid.y = frame chunks of FS frames.  The transposed grad_out holds records of fsl frames
+// per voxel (fsl a multiple of FS), so a launch with half the frames reads half records.
 // The row lives in LDS as fixed point [slot(k)][word] (word = frame or frame pair, times the
 // complex part, at an odd pitch), so the lanes of one atomic differ only in slot; each lane owns
 // one voxel per step (axial order: geometry and grad_out were permuted to it, so the
@@ -1194,7 +1196,8 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
                                                          int log2cnt, void *__restrict__ gxv,
                                                          int64_t gx_b, int64_t gx_c, int64_t gx_t,
                                                          const unsigned *__restrict__ sp_start,
-                                                         const int *__restrict__ sp_list) {
+                                                         const int *__restrict__ sp_list,
+                                                         int row0, int fsl) {
     using A = AccT<ACC64>;
     using U64 = unsigned long long;
     constexpr bool PACK = !ACC64 && FS >= 2;
@@ -1210,7 +1213,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
     const A *__restrict__ gt = static_cast<const A *>(gtv);
     A *__restrict__ gx = static_cast<A *>(gxv);
 
-    const int row = blockIdx.x;
+    const int row = row0 + (int)blockIdx.x;
     const int m = row / g.T;
     const int s = row - m * g.T;
     const int chunk = blockIdx.y;
@@ -1258,7 +1261,8 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
     const unsigned e_lo = dense ? 0u : sp_start[s];
     const unsigned e_hi = dense ? 0u : sp_start[s + 1];
     const int nitems = dense ? per_r * ((ax.R + RT - 1) / RT) : (int)((e_hi - e_lo + 63u) >> 6);
-    const A *gchunk = gt + (int64_t)chunk * g.N * FS * CW;
+    const int64_t rec = b0 / fsl;   // the record group holding frames b0 .. b0 + FS - 1
+    const A *gchunk = gt + (rec * g.N * fsl + (b0 - rec * fsl)) * CW;
 
     struct Vox {
         A gv[FS * CW];
@@ -1282,7 +1286,7 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
             v.q = code / g.K;
             o = (int64_t)(code - v.q * g.K) * g.N + v.q;
         }
-        const A *gp = gchunk + (int64_t)v.q * FS * CW;
+        const A *gp = gchunk + (int64_t)v.q * fsl * CW;
 #pragma unroll
         for (int i = 0; i < FS * CW; i++) v.gv[i] = gp[i];
         v.x = g.dstpos[3 * (int64_t)v.q];
@@ -2034,9 +2038,27 @@ hipError_t launch_bwdx_plain_t(hipStream_t st, const dev::Geo &g, const dev::Til
     }
 }
 
+// Rows of a dense grad_x launch given to a second launch with half the frames per workgroup:
+// with one row-chunk per workgroup, the last round of workgroups often fills only part of
+// the CUs (1408 rows of 16 frames on 256 CUs: 5.5 rounds).  When that last round is at most
+// half full, its rows go to workgroups of FS / 2 frames (two per row-chunk, each about 0.6
+// of the time), and the first launch covers whole rounds.  The sums are the same integers
+// either way (the exponent depends on neither FS nor the chunking), so grad_x is bitwise
+// unchanged.  0 = no split.
+int bwdx_tail_rows(int rows, int nchunks, int cu_count, int per_cu, int per_cu_half) {
+    if (cu_count <= 0 || per_cu <= 0 || per_cu_half <= 0) return 0;
+    const int64_t cap = (int64_t)cu_count * per_cu, cap2 = (int64_t)cu_count * per_cu_half;
+    if ((int64_t)rows * nchunks <= cap) return 0;   // a single round: nothing to balance
+    int tr = 0;
+    while (tr < rows && ((int64_t)(rows - tr) * nchunks) % cap != 0) tr++;
+    if (tr == 0 || tr == rows) return 0;
+    const int64_t w2 = (int64_t)tr * nchunks * 2;
+    return w2 <= cap && w2 <= cap2 ? tr : 0;
+}
+
 template <bool ACC64, bool CPLX, bool PRECISE, int FS>
 hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &ax, int threads,
-                          const void *gout, int64_t gs_b, void *gt, void *gx) {
+                          const void *gout, int64_t gs_b, void *gt, void *gx, int cu_count) {
     using A = typename std::conditional<ACC64, double, float>::type;
     const size_t esz = 8 * (CPLX ? 2 : 1);
     const size_t lds = bwdx_lds_bytes(g.S, FS, CPLX, ACC64);
@@ -2099,20 +2121,43 @@ hipError_t launch_bwdx_fs(hipStream_t st, const dev::Geo &g, const dev::Axial &a
     }
     if (const char *env = std::getenv("FFDAS_BWDX_THREADS")) threads = std::atoi(env);
     if (threads != 256 && threads != 512 && threads != 1024) threads = 1024;
+    const int rows = g.C * g.T;
+    int tail = 0;
+    if constexpr (FS >= 4) {
+        const char *env = std::getenv("FFDAS_BWDX_TAIL");
+        if (!g.sparse && !(env && env[0] == '0')) {
+            const auto kh = dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS / 2>;
+            const size_t lds_h = bwdx_lds_bytes(g.S, FS / 2, CPLX, ACC64);
+            (void)hipFuncSetAttribute((const void *)kh, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds_h);
+            int per_cu = 0, per_cu_h = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, (const void *)dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS>,
+                    threads, lds) == hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_h, (const void *)kh, threads,
+                                                             lds_h) == hipSuccess)
+                tail = bwdx_tail_rows(rows, nchunks, cu_count, per_cu, per_cu_h);
+            if (tail > 0)
+                kh<<<dim3(tail, (g.B + FS / 2 - 1) / (FS / 2)), threads, lds_h, st>>>(
+                    ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t, sp_start, sp_list,
+                    rows - tail, FS);
+        }
+    }
+    grid.x = rows - tail;
     dev::das_bwd_x_kernel<ACC64, CPLX, PRECISE, FS><<<grid, threads, lds, st>>>(
-        ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t, sp_start, sp_list);
+        ga, ax, gtr, maxes, log2cnt, gx, gx_b, gx_c, gx_t, sp_start, sp_list, 0, FS);
     return hipGetLastError();
 }
 
 template <bool ACC64, bool CPLX, bool PRECISE>
 hipError_t launch_bwdx_t(hipStream_t st, const dev::Geo &g, const dev::Axial &tl, int threads,
-                         const void *gout, int64_t gs_b, void *gt, void *gx, int FS) {
+                         const void *gout, int64_t gs_b, void *gt, void *gx, int FS, int cus) {
     switch (FS) {
-        case 16: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 16>(st, g, tl, threads, gout, gs_b, gt, gx);
-        case 8: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 8>(st, g, tl, threads, gout, gs_b, gt, gx);
-        case 4: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 4>(st, g, tl, threads, gout, gs_b, gt, gx);
-        case 2: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 2>(st, g, tl, threads, gout, gs_b, gt, gx);
-        default: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 1>(st, g, tl, threads, gout, gs_b, gt, gx);
+        case 16: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 16>(st, g, tl, threads, gout, gs_b, gt, gx, cus);
+        case 8: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 8>(st, g, tl, threads, gout, gs_b, gt, gx, cus);
+        case 4: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 4>(st, g, tl, threads, gout, gs_b, gt, gx, cus);
+        case 2: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 2>(st, g, tl, threads, gout, gs_b, gt, gx, cus);
+        default: return launch_bwdx_fs<ACC64, CPLX, PRECISE, 1>(st, g, tl, threads, gout, gs_b, gt, gx, cus);
     }
 }
 
@@ -2676,11 +2721,11 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
         // whatever the storage type: the scatter is bound by LDS atomics, and the extra
         // delay arithmetic overlaps with them
         if (p.acc64) {
-            e = p.cplx ? launch_bwdx_t<true, true, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<true, false, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<true, true, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS, h.cu_count)
+                       : launch_bwdx_t<true, false, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS, h.cu_count);
         } else {
-            e = p.cplx ? launch_bwdx_t<false, true, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS)
-                       : launch_bwdx_t<false, false, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS);
+            e = p.cplx ? launch_bwdx_t<false, true, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS, h.cu_count)
+                       : launch_bwdx_t<false, false, true>(sx, g, ax, p.tune.bwdx_threads, gout, gs_b, gt, gx, FS, h.cu_count);
         }
         if (e != hipSuccess) return hip_status(e, FFDAS_ERROR_CUDA_LAUNCH_FAILED);
     }

@@ -43,8 +43,9 @@ def short_name(kernel: str) -> str:
     return kernel.split("(")[0][-60:]
 
 
-def read_counter(run_dir: str, counter: str) -> dict:
-    """{short kernel name: [values per dispatch]} for one counter."""
+def read_counter(run_dir: str, counter: str, variants: bool = False) -> dict:
+    """{short kernel name: [values per dispatch]} for one counter; with variants,
+    {short kernel name: {full kernel name: [values per dispatch]}} (template instances apart)."""
     files = glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise FileNotFoundError(f"no counter_collection.csv under {run_dir}")
@@ -57,31 +58,43 @@ def read_counter(run_dir: str, counter: str) -> dict:
                     continue
                 key = (f, int(r["Dispatch_Id"]))
                 per[key] += float(r["Counter_Value"])
-                names[key] = short_name(r["Kernel_Name"])
+                names[key] = (short_name(r["Kernel_Name"]), r["Kernel_Name"])
+    if variants:
+        nested = collections.defaultdict(lambda: collections.defaultdict(list))
+        for key in sorted(per):
+            nested[names[key][0]][names[key][1]].append(per[key])
+        return nested
     out = collections.defaultdict(list)
     for key in sorted(per):
-        out[names[key]].append(per[key])
+        out[names[key][0]].append(per[key])
     return out
 
 
 def summarize(fetch_dir: str, write_dir: str) -> dict:
-    fetch = read_counter(fetch_dir, "FETCH_SIZE")
-    write = read_counter(write_dir, "WRITE_SIZE")
+    fetch = read_counter(fetch_dir, "FETCH_SIZE", variants=True)
+    write = read_counter(write_dir, "WRITE_SIZE", variants=True)
     if "copyBuffer" not in fetch or "copyBuffer" not in write:
         raise RuntimeError("the calibration copy (copyBuffer) is missing from the runs")
-    f_copy = max(fetch["copyBuffer"]) * 1024.0
-    w_copy = max(write["copyBuffer"]) * 1024.0
+    f_copy = max(max(v) for v in fetch["copyBuffer"].values()) * 1024.0
+    w_copy = max(max(v) for v in write["copyBuffer"].values()) * 1024.0
     kf, kw = COPY_BYTES / f_copy, COPY_BYTES / w_copy
+
+    def per_launch(runs):
+        # the median dispatch of each template instance (warm-ups and odd-sized calls
+        # aside), summed over the instances: one operation that launches a kernel in two
+        # instances (grad_x's whole rounds and its half-frame last round) counts both
+        return sum(statistics.median(v) for v in runs.values()) * 1024.0
+
     kernels = {}
     for name in sorted(set(fetch) | set(write)):
         if name == "copyBuffer" or not name.endswith("kernel"):
             continue
-        # the median dispatch of each kernel (warm-ups and odd-sized calls aside)
-        fb = statistics.median(fetch[name]) * 1024.0 * kf if name in fetch else None
-        wb = statistics.median(write[name]) * 1024.0 * kw if name in write else None
+        fb = per_launch(fetch[name]) * kf if name in fetch else None
+        wb = per_launch(write[name]) * kw if name in write else None
         kernels[name] = {"fetch_bytes": fb, "write_bytes": wb,
                          "traffic_bytes": (fb or 0.0) + (wb or 0.0),
-                         "dispatches": len(fetch.get(name, []))}
+                         "dispatches": sum(len(v) for v in fetch.get(name, {}).values()),
+                         "instances": len(fetch.get(name, {}))}
     return {"calibration": {"copy_bytes": COPY_BYTES, "fetch_counter_bytes": f_copy,
                             "write_counter_bytes": w_copy, "fetch_factor": kf,
                             "write_factor": kw},

@@ -159,6 +159,31 @@ def test_grad_x_grid_shapes(dev, N):
     assert rel_l2(gx.cpu().numpy(), ref["grad_x"]) < 1e-5
 
 
+@pytest.mark.parametrize("B, cplx", [(16, False), (24, False), (20, False), (8, True)])
+def test_grad_x_last_round_split(dev, monkeypatch, B, cplx):
+    """More rows than one round of workgroups (320 rows on 256 CUs): the rows of the last,
+    partly filled round run in workgroups of half the frames, reading half records of the
+    transposed grad_out (24 and 20 frames: two record groups, the last one partial).  The
+    fixed-point sums are the same integers, so grad_x is bitwise equal to the unsplit launch
+    (FFDAS_BWDX_TAIL=0), and it matches the oracle."""
+    import torch
+    w = W.small_random(seed=B, B=B, C=64, T=5, S=400, N=(16, 8), complex_iq=cplx)
+    w.offsets[:] = np.random.default_rng(B).uniform(-10, 360, w.offsets.shape).astype(np.float32)
+    rng = np.random.default_rng(B + 1)
+    gout = rng.standard_normal((B,) + w.dstpos.shape[:-1])
+    if cplx:
+        gout = gout + 1j * rng.standard_normal(gout.shape)
+    gout = gout.astype(np.complex64 if cplx else np.float32)
+    kw = dict(need_offsets=False, need_weights=False, need_scale=False)
+    split = _grads(w, dev, gout, **kw)[0]
+    monkeypatch.setenv("FFDAS_BWDX_TAIL", "0")
+    whole = _grads(w, dev, gout, **kw)[0]
+    assert torch.equal(split, whole)
+    ref = oracle.das_adjoint(w.x, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             wavenum=w.wavenum)
+    assert rel_l2(split.cpu().numpy(), ref["grad_x"]) < 1e-5
+
+
 @pytest.mark.parametrize("case", ["dense", "dir", "sparse", "complex", "scaled", "float64", "batched"])
 def test_plain_adjoint_alg1(dev, case):
     """ALG1 runs the plain kernels: natural-layout parameter gather and the global-atomic
