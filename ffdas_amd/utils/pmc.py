@@ -113,7 +113,9 @@ def to_markdown(s: dict, command: str) -> str:
     for k, v in s["kernels"].items():
         fb = v["fetch_bytes"] or 0.0
         wb = v["write_bytes"] or 0.0
-        lines.append(f"| {k} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} |")
+        n = v.get("instances", 1)
+        name = k if n <= 1 else f"{k} ({n} template instances per operation, summed)"
+        lines.append(f"| {name} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} |")
     return "\n".join(lines) + "\n"
 
 

@@ -77,16 +77,20 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("groups", [None, "1"])
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1]}-{c[2]}-{c[3]}")
+# the waves-per-tile override (FFDAS_CGROUPS=1) applies to the frame-inner kernels only,
+# so ALG1 runs with the default groups
+GROUPED = [(c, g) for c in CASES for g in (None, "1") if g is None or c[2] != "ALG1"]
+
+
+@pytest.mark.parametrize("case, groups", GROUPED,
+                         ids=lambda v: v if isinstance(v, str) or v is None else
+                         f"{v[0]}-{v[1]}-{v[2]}-{v[3]}")
 @pytest.mark.parametrize("sparse", [False, True])
 def test_host_beta_accumulates(dev, monkeypatch, case, sparse, groups):
     import torch
     from ffdas_amd._lib import Algorithm, ComputeType
     kind, ct, alg, beta, cplx, bound = case
     if groups is not None:
-        if alg == "ALG1":
-            pytest.skip("the waves-per-tile override applies to the frame-inner kernels")
         monkeypatch.setenv("FFDAS_CGROUPS", groups)
     w = W.small_random(seed=11, B=5, C=9, T=4, S=160, N=(7, 11), complex_iq=cplx,
                        srcdir=True, sparse=sparse, K=3 if sparse else None)
