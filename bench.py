@@ -809,8 +809,8 @@ def bench_truncate_rank(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                                   "described in DESIGN.md but not built", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU only: sharding a call "
+                                   "across GPUs is not built", "n_gpus": a.gpus}))
         sys.exit(2)
     ops = {"truncate_rank": bench_truncate_rank, "interpolate": bench_interpolate,
            "greens": bench_greens}
