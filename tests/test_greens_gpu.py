@@ -2,7 +2,7 @@
 (bindings/python/tests/test_greens.py) at its tolerance (rtol 1e-3, atol 1e-4; the kernel
 computes in FP32, its error is about 1e-6 of the sum's scale), all three event tilings (1
 event per thread on the VALU, 4 events per thread on the VALU for 2-4 events, 16-event tiles
-on the MFMA from 5 events up, each at its boundary: 1, 2, 4, 5 events), partial
+on the MFMA from 5 events up, each at its boundaries: 1, 2, 4, 5, 8, 9 events), partial
 tiles of targets, sources, frequencies and events, FP16 input, a 2-D target grid and the
 C-ABI error codes."""
 import ctypes
@@ -61,7 +61,9 @@ def test_reference_zero_input(dev):
 @pytest.mark.parametrize("B, nin, nk, nout", [(1, 37, 13, 300), (4, 64, 16, 257),
                                               (7, 20, 9, 70), (2, 1, 1, 1), (8, 37, 13, 300),
                                               (20, 66, 17, 131), (16, 3, 8, 16),
-                                              (3, 29, 11, 97), (5, 33, 12, 150)])
+                                              (3, 29, 11, 97), (5, 33, 12, 150),
+                                              (6, 40, 33, 77), (8, 1, 1, 1), (9, 21, 18, 45),
+                                              (5, 3, 16, 5)])
 def test_parity_with_the_oracle(dev, B, nin, nk, nout):
     import ffdas_amd
     rng = np.random.default_rng(B * 1000 + nin)
@@ -83,7 +85,7 @@ def test_two_dimensional_input_and_fp16(dev):
     assert tuple(got2.shape) == (40, 10)
     np.testing.assert_allclose(got2.cpu().numpy(), G.greens(src, k, x[0], dst), rtol=1e-3,
                                atol=1e-4)
-    for xin in (x, np.concatenate([x] * 9)):          # the VALU and the MFMA kernel
+    for xin in (x, np.concatenate([x] * 7), np.concatenate([x] * 9)):   # VALU, MFMA part/whole
         xh = _t(xin, dev).to(torch.complex32)
         goth = ffdas_amd.greens(_t(src, dev), _t(k, dev), xh, _t(dst, dev)).cpu().numpy()
         want = G.greens(src, k, xh.to(torch.complex64).cpu().numpy(), dst)
