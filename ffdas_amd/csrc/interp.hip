@@ -527,6 +527,14 @@ struct El {
     R c[CW];
 };
 
+// Outputs are written once and never read back by the kernel: streamed past the caches so
+// they do not evict the vertex blocks that neighbouring query points gather.
+template <typename R, int CW>
+__device__ __forceinline__ void put_out(El<R, CW> *p, const El<R, CW> &v) {
+#pragma unroll
+    for (int c = 0; c < CW; c++) __builtin_nontemporal_store(v.c[c], &p->c[c]);
+}
+
 constexpr int kFramesPerThread = 8;   // frames per chunk: the smallest, for the extent check
 
 // frames per chunk: 16 for 4-byte elements (a 64-byte vector load per vertex), else 8
@@ -583,11 +591,11 @@ __global__ __launch_bounds__(256) void interp_nearest_kernel(const Frames<R, CW>
         const Frames<R, CW> v = xfi[by * npts + k];
 #pragma unroll
         for (int f = 0; f < FPC; f++)
-            if (f < nf) out[(b0 + f) * n + i] = v.f[f];
+            if (f < nf) put_out(out + (b0 + f) * n + i, v.f[f]);
     } else {
 #pragma unroll
         for (int f = 0; f < FPC; f++)
-            if (f < nf) out[(b0 + f) * n + i] = fill;
+            if (f < nf) put_out(out + (b0 + f) * n + i, fill);
     }
 }
 
@@ -604,8 +612,13 @@ __global__ __launch_bounds__(256) void interp_linear_kernel(const Frames<R, CW> 
     xcd_tile(bx, by);
     const int64_t i = (int64_t)bx * 256 + threadIdx.x;
     if (i >= n) return;
-    const int4 k = idx[i];
-    const float4 w = wt[i];
+    // the lookup is read once per frame chunk: streamed, like the outputs
+    typedef int I4 __attribute__((ext_vector_type(4)));
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const I4 kv = __builtin_nontemporal_load(reinterpret_cast<const I4 *>(idx) + i);
+    const F4 wv = __builtin_nontemporal_load(reinterpret_cast<const F4 *>(wt) + i);
+    const int4 k = make_int4(kv.x, kv.y, kv.z, kv.w);
+    const float4 w = make_float4(wv.x, wv.y, wv.z, wv.w);
     const int64_t b0 = (int64_t)by * FPC;
     const int nf = (int)(B - b0 < FPC ? B - b0 : FPC);
     const Frames<R, CW> *xc = xfi + by * npts;
@@ -623,7 +636,7 @@ __global__ __launch_bounds__(256) void interp_linear_kernel(const Frames<R, CW> 
             r.c[c] = in ? (R)w.x * v0.f[f].c[c] + (R)w.y * v1.f[f].c[c] + (R)w.z * v2.f[f].c[c] +
                               (R)w.w * v3.f[f].c[c]
                         : fill.c[c];
-        if (f < nf) out[(b0 + f) * n + i] = r;
+        if (f < nf) put_out(out + (b0 + f) * n + i, r);
     }
 }
 
