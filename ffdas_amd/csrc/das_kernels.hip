@@ -1422,7 +1422,9 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
         E = exponent(logc);
     }
 
-    // write back: consecutive threads -> consecutive samples
+    // write back: consecutive threads -> consecutive samples, streamed past the caches (grad_x
+    // is written once and not read again here; a step that reuses its channel data then
+    // still finds that data in the last-level cache)
     const double inv = ldexp(1.0, -E);
     for (int k = threadIdx.x; k < g.S; k += blockDim.x) {
         const U64 *wp = lds + (size_t)lds_slot(k) * PS;
@@ -1435,8 +1437,12 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
                 for (int c = 0; c < CW; c++) {
                     long long v0, v1;
                     fx_unpack_pair(wp[p * CW + c], nk, v0, v1);
-                    if (2 * p < nf) dst[(int64_t)(2 * p) * gx_b * CW + c] = (A)((double)v0 * inv);
-                    if (2 * p + 1 < nf) dst[(int64_t)(2 * p + 1) * gx_b * CW + c] = (A)((double)v1 * inv);
+                    if (2 * p < nf)
+                        __builtin_nontemporal_store((A)((double)v0 * inv),
+                                                    dst + (int64_t)(2 * p) * gx_b * CW + c);
+                    if (2 * p + 1 < nf)
+                        __builtin_nontemporal_store((A)((double)v1 * inv),
+                                                    dst + (int64_t)(2 * p + 1) * gx_b * CW + c);
                 }
             }
         } else {
@@ -1445,8 +1451,8 @@ __global__ __launch_bounds__(1024) void das_bwd_x_kernel(Geo g, Axial ax, const 
                 if (f >= nf) break;
 #pragma unroll
                 for (int c = 0; c < CW; c++)
-                    dst[(int64_t)f * gx_b * CW + c] =
-                        (A)((double)(long long)wp[f * CW + c] * inv);
+                    __builtin_nontemporal_store((A)((double)(long long)wp[f * CW + c] * inv),
+                                                dst + (int64_t)f * gx_b * CW + c);
             }
         }
     }
