@@ -157,16 +157,27 @@ def time_steps(fn, steps, warmup):
 
 def kernel_times(p, x, steps):
     """Average duration of each library kernel over `steps` forward+adjoint steps, from the
-    library's own hipEvent pairs recorded around every launch on the stream it runs on."""
+    library's own hipEvent pairs recorded around every launch on the stream it runs on.  The
+    adjoint's grad_x scatter, which in the timed steps runs on a second stream beside the
+    parameter adjoint (16-bit storage), is serialised here (FFDAS_BWD_CONCURRENT=0) so that
+    each duration is the kernel's own rather than a share of the overlap."""
     import torch
     from ffdas_amd.utils import profiling
-    for _ in range(2):
-        step(p, x)
-    torch.cuda.synchronize()
-    with profiling.profiled():
-        for _ in range(steps):
+    old = os.environ.get("FFDAS_BWD_CONCURRENT")
+    os.environ["FFDAS_BWD_CONCURRENT"] = "0"
+    try:
+        for _ in range(2):
             step(p, x)
         torch.cuda.synchronize()
+        with profiling.profiled():
+            for _ in range(steps):
+                step(p, x)
+            torch.cuda.synchronize()
+    finally:
+        if old is None:
+            del os.environ["FFDAS_BWD_CONCURRENT"]
+        else:
+            os.environ["FFDAS_BWD_CONCURRENT"] = old
     return {k: v["avg_ms"] for k, v in profiling.kernel_stats().items()}
 
 
@@ -896,6 +907,9 @@ def main():
                    "ratio_vs_fp32_plain": round(value / v32, 3),
                    "ratio_vs_fp32_tuned": round(value / v32o, 3),
                    "rel_l2_vs_fp64": acc[0], "rel_l2_grad_x_vs_fp64": acc[1],
+                   "kernel_ms_note": "per-kernel durations from a pass with the adjoint's "
+                                     "grad_x scatter and parameter adjoint serialised; the "
+                                     "timed steps run them side by side (16-bit storage)",
                    "autotuned": None if tuned is None else
                    {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()},
                    "wall_s": round(wall, 3)},

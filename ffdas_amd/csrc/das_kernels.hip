@@ -2655,15 +2655,19 @@ ffdas_error_t launch_backward(ffdas_context &h, const DasProblem &p, const DasGe
     hipStream_t st = h.stream;
     hipError_t e = hipSuccess;
 
-    // With FFDAS_BWD_CONCURRENT=1 and both the parameter gradients and grad_x asked for, the
+    // When both the parameter gradients and grad_x are asked for with 16-bit storage, the
     // grad_x scatter (bound by LDS atomics) runs on the handle's second stream beside the
     // parameter adjoint (bound by its gathers), forked from and joined back into the
-    // handle's stream.  Off by default: the gain is small (grad_x holds most of a CU's LDS,
-    // so the two share few CUs), and per-kernel durations then overlap.
+    // handle's stream: a grad_x workgroup leaves registers and waves for a parameter-adjoint
+    // wave on the same CU, and the two use different units.  Measured on the 2-D benchmark:
+    // FP16 storage 0.994-0.998 -> 0.976-0.979 ms a step; FP32 storage 1.407 -> 1.437 ms (its
+    // heavier gathers lose more to the sharing than the overlap gains), so 32- and 64-bit
+    // storage stay serial.  FFDAS_BWD_CONCURRENT=0 / 1 forces serial / concurrent.
     const bool params = gofs || gwt || gscale;
-    bool concurrent = false;
+    const bool can = params && gx && !p.plain && !use_sorted_bwdx(p);
+    bool concurrent = can && (p.k_storage == Storage::F16 || p.k_storage == Storage::BF16);
     if (const char *env = std::getenv("FFDAS_BWD_CONCURRENT"))
-        concurrent = std::atoi(env) != 0 && params && gx && !p.plain && !use_sorted_bwdx(p);
+        concurrent = can && std::atoi(env) != 0;
     hipStream_t sx = st;
     if (concurrent) {
         FFDAS_TRY(h.side_stream(&sx));

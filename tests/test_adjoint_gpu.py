@@ -184,6 +184,40 @@ def test_grad_x_last_round_split(dev, monkeypatch, B, cplx):
     assert rel_l2(split.cpu().numpy(), ref["grad_x"]) < 1e-5
 
 
+@pytest.mark.parametrize("cplx, fp16", [(False, True), (True, True), (False, False)])
+def test_concurrent_adjoint_matches_serial(dev, monkeypatch, cplx, fp16):
+    """grad_x on the handle's second stream beside the parameter adjoint (the default for
+    16-bit storage when both are asked for; forced with FFDAS_BWD_CONCURRENT=1 for FP32)
+    gives bitwise the results of the two run one after the other (FFDAS_BWD_CONCURRENT=0),
+    and matches the oracle's adjoint (FP32 storage) or its storage-rounded data (FP16)."""
+    import torch
+    w = W.small_random(seed=77, B=16, C=24, T=3, S=600, N=(40, 24), complex_iq=cplx)
+    w.offsets[:] = np.random.default_rng(7).uniform(-10, 520, w.offsets.shape).astype(np.float32)
+    rng = np.random.default_rng(8)
+    gout = rng.standard_normal((16,) + w.dstpos.shape[:-1])
+    if cplx:
+        gout = gout + 1j * rng.standard_normal(gout.shape)
+    gout = gout.astype(np.complex64 if cplx else np.float32)
+    if not fp16:
+        monkeypatch.setenv("FFDAS_BWD_CONCURRENT", "1")
+    both = _grads(w, dev, gout, use_fp16=fp16)
+    monkeypatch.setenv("FFDAS_BWD_CONCURRENT", "0")
+    serial = _grads(w, dev, gout, use_fp16=fp16)
+    for a, b in zip(both, serial):
+        assert torch.equal(a, b)
+    xr = w.x
+    if fp16:   # the kernel gathers the FP16-rounded data
+        xr = (w.x.real.astype(np.float16).astype(np.float32) +
+              1j * w.x.imag.astype(np.float16).astype(np.float32)).astype(np.complex64) \
+            if cplx else w.x.astype(np.float16).astype(np.float32)
+    ref = oracle.das_adjoint(xr, w.srcpos, w.dstpos, w.offsets, w.weights, gout,
+                             wavenum=w.wavenum)
+    tol = 5e-4 if fp16 else 1e-5
+    assert rel_l2(both[0].cpu().numpy(), ref["grad_x"]) < 1e-5
+    assert rel_l2(both[1].cpu().numpy(), ref["grad_offsets"]) < max(tol, 1e-4)
+    assert rel_l2(both[2].cpu().numpy(), ref["grad_weights"]) < tol
+
+
 @pytest.mark.parametrize("case", ["dense", "dir", "sparse", "complex", "scaled", "float64", "batched"])
 def test_plain_adjoint_alg1(dev, case):
     """ALG1 runs the plain kernels: natural-layout parameter gather and the global-atomic
